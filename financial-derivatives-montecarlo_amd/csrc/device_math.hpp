@@ -1,0 +1,182 @@
+// device_math.hpp -- per-lane math shared by the gfx950 path kernels.
+//
+//  * splitmix32 / Hong-Hickernell row products (methods_cuda.cu:753-795)
+//  * inverse normal CDF: own least-squares fit (gen/fit_icdf.py) of the
+//    Giles-form ratio, evaluated with the hardware log2 / sqrt
+//  * Philox4x32-10 (Salmon et al., SC'11) + Box-Muller for pseudo-random paths
+//  * Euler steppers and payoff accumulators (methods_cuda.cu:271-352)
+//  * wave64 FP64 reductions
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "icdf_coeffs.h"
+#include "kernel_params.hpp"
+
+namespace fdmc {
+
+__host__ __device__ __forceinline__ uint32_t splitmix32(uint32_t x) {
+    x += 0x9E3779B9u;
+    x = (x ^ (x >> 16)) * 0x21F0AAADu;
+    x = (x ^ (x >> 15)) * 0x735A2D97u;
+    return x ^ (x >> 15);
+}
+
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_ln(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Inverse standard normal CDF of u = (s + 1/2) 2^-32, never 0 or 1.  The value
+// is folded to the lower half (q = min(u, 1-u)) from the integer so that both
+// tails keep full relative precision; w = -ln(4 q (1-q)).
+__device__ __forceinline__ float icdf_u32(uint32_t s) {
+    const bool upper = (s & 0x80000000u) != 0u;
+    const uint32_t f = upper ? ~s : s;
+    const float q = fmaf((float)f, 0x1p-32f, 0x1p-33f);
+    const float t = fmaf(-q, q, q);                       // q (1 - q)
+    const float w = -(fast_log2(t) + 2.0f) * 0.69314718055994531f;
+    float p;
+    if (w < FDMC_ICDF_W_SPLIT) {
+        constexpr float c[] = FDMC_ICDF_CENTRAL_COEFFS;
+        const float x = w - FDMC_ICDF_C_CENTRAL;
+        p = c[0];
+#pragma unroll
+        for (int i = 1; i <= FDMC_ICDF_DEG_CENTRAL; i++) p = fmaf(p, x, c[i]);
+    } else {
+        constexpr float c[] = FDMC_ICDF_TAIL_COEFFS;
+        const float x = fast_sqrt(w) - FDMC_ICDF_C_TAIL;
+        p = c[0];
+#pragma unroll
+        for (int i = 1; i <= FDMC_ICDF_DEG_TAIL; i++) p = fmaf(p, x, c[i]);
+    }
+    const float y = fmaf(-2.0f, q, 1.0f) * p;               // (1 - 2q) f(w)
+    return upper ? y : -y;
+}
+
+// ---- Philox4x32-10 -------------------------------------------------------
+struct u32x4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Two normals by Box-Muller from Philox words (a, b):
+// r = sqrt(-2 ln((a + 1/2) 2^-32)); (r cos 2 pi b 2^-32, r sin 2 pi b 2^-32).
+// __builtin_amdgcn_{cos,sin}f take their argument in revolutions.
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& z0, float& z1) {
+    const float u1 = fmaf((float)a, 0x1p-32f, 0x1p-33f);
+    const float r = fast_sqrt(-2.0f * 0.69314718055994531f * fast_log2(u1));
+    const float th = (float)(b >> 8) * 0x1p-24f;
+    z0 = r * __builtin_amdgcn_cosf(th);
+    z1 = r * __builtin_amdgcn_sinf(th);
+}
+
+// Four normals of stream (seed, tag, path), block b (normals 4b .. 4b+3).
+__device__ __forceinline__ void prng_normals4(uint32_t seed, uint32_t tag, uint64_t path,
+                                              uint32_t b, float z[4]) {
+    const u32x4 w = philox4x32_10(u32x4{b, (uint32_t)path, (uint32_t)(path >> 32), tag}, seed,
+                                  0x46444D43u);
+    box_muller(w.x, w.y, z[0], z[1]);
+    box_muller(w.z, w.w, z[2], z[3]);
+}
+
+// ---- wave64 / workgroup FP64 reductions ------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum NA per-thread values over a workgroup of NW waves; result valid in
+// thread 0.  lds must hold NW * NA doubles.
+template <int NA, int NW>
+__device__ __forceinline__ void block_sum(double (&v)[NA], double* lds) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+        const double s = wave_sum(v[i]);
+        if (lane == 0) lds[wv * NA + i] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) s += lds[w * NA + i];
+            v[i] = s;
+        }
+    }
+}
+
+// ---- steppers (methods_cuda.cu:271-291, 388-406) ---------------------------
+// One-factor Euler step at time t with step h and increment dw.
+template <int MK>
+__device__ __forceinline__ float step_1f(const LegParams& P, float S, float dw, float t, float h) {
+    if constexpr (MK == MK_GBM) {
+        return S + P.mu * S * h + P.sigma * S * dw;
+    } else if constexpr (MK == MK_DUPIRE) {
+        // sigma0 exp(-alpha t) (S/S0)^(beta-1)
+        const float sl = P.sigma0 * fast_exp(-P.alpha * t) * fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));
+        return S + P.mu * S * h + sl * S * dw;
+    } else {  // MK_LVQ: a + b exp(-c t) ln(S/s_ref)^2
+        const float l = fast_ln(S * P.inv_sref);
+        const float sl = P.lv_a + P.lv_b * fast_exp(-P.lv_c * t) * l * l;
+        return S + P.mu * S * h + sl * S * dw;
+    }
+}
+
+// Full-truncation exponential-Euler Heston step (methods_cuda.cu:278-284) with
+// e^{-kappa h} precomputed.
+__device__ __forceinline__ void step_heston(const LegParams& P, float& S, float& V, float dw1,
+                                            float dw2, float h, float em) {
+    const float sq = fast_sqrt(fmaxf(V, 0.0f));
+    S = S + P.mu * S * h + sq * S * dw1;
+    V = P.theta + em * (V - P.theta) + P.xi * sq * dw2;
+}
+
+// ---- payoff accumulators (methods_cuda.cu:298-352) --------------------------
+template <int PK>
+__device__ __forceinline__ float run_init(const LegParams& P) {
+    if constexpr (PK == PK_LOOKBACK) return P.S0;
+    else return 0.0f;
+}
+template <int PK>
+__device__ __forceinline__ void run_update(float& run, float S) {
+    if constexpr (PK == PK_ASIAN) run += S;
+    else if constexpr (PK == PK_GEOM_ASIAN) run += fast_ln(S);
+    else if constexpr (PK == PK_LOOKBACK) run = fminf(run, S);
+    else if constexpr (PK == PK_BARRIER) run = fmaxf(run, S);
+}
+template <int PK>
+__device__ __forceinline__ constexpr bool is_path_dep() {
+    return PK == PK_ASIAN || PK == PK_GEOM_ASIAN || PK == PK_LOOKBACK || PK == PK_BARRIER;
+}
+// Terminal payoff; bgk is the precomputed exp(-beta sigma sqrt(h)) of the level.
+template <int PK>
+__device__ __forceinline__ float payoff(const LegParams& P, float S_T, float run, int n, float bgk) {
+    if constexpr (PK == PK_EUROPEAN || PK == PK_BASKET) {
+        return fmaxf(S_T - P.K, 0.0f) * P.discount;
+    } else if constexpr (PK == PK_ASIAN) {
+        return fmaxf(run / (float)n - P.K, 0.0f);
+    } else if constexpr (PK == PK_GEOM_ASIAN) {
+        return fmaxf(fast_exp(run / (float)n) - P.K, 0.0f);
+    } else if constexpr (PK == PK_LOOKBACK) {
+        return S_T - run * bgk;
+    } else {  // PK_BARRIER
+        return (run / bgk >= P.B) ? 0.0f : fmaxf(S_T - P.K, 0.0f) * P.discount;
+    }
+}
+
+}  // namespace fdmc

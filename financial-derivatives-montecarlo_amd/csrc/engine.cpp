@@ -1,0 +1,850 @@
+// engine.cpp -- host side of the engine: context, validation, estimator loops
+// and the C ABI of include/fdmc.h.
+//
+// The estimator loops keep the reference's control logic (MC pilot + N rule,
+// QMC doubling, Giles MLMC/MLQMC allocation, control variates;
+// methods_cuda.cu:1173-2278) but drive fused gfx950 kernels: one launch per
+// pass covers every replica, noise is generated in registers, sums are reduced
+// deterministically on the device, and each pass costs one small D2H copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fdmc.h"
+#include "engine_internal.hpp"
+#include "kernel_params.hpp"
+
+#include "joe_kuo6_table.inc"
+
+using namespace fdmc;
+
+// ---------------------------------------------------------------- errors
+namespace {
+thread_local std::string g_last_error;
+
+struct Fail {
+    fdmc_status code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(fdmc_status c, const std::string& m) { throw Fail{c, m}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(FDMC_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(call) hip_check((call), #call)
+
+template <class F>
+fdmc_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return FDMC_OK;
+    } catch (const Fail& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return FDMC_E_ALLOC;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return FDMC_E_INTERNAL;
+    }
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+constexpr int D_MAX_SOBOL = 20000;  // models.hpp:9
+constexpr uint32_t TAG_QMC_FALLBACK = 0x51u, TAG_MLMC = 0x100u, TAG_CV_PILOT = 0x200u,
+                   TAG_CV_MAIN = 0x201u;
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* ensure(size_t n) {
+        if (n <= bytes) return p;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = std::max(n, (size_t)4096);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            fail(FDMC_E_ALLOC, "hipMalloc of " + std::to_string(want) + " bytes failed");
+        }
+        bytes = want;
+        return p;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+}  // namespace
+
+struct fdmc_context {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    uint32_t* d_dirv = nullptr;   // [FDMC_JK_NDIM][32]
+    DevBuf partials, sums, salts, sdir, gen_sdir;
+    std::vector<uint32_t> sdir_key;  // salts + D of the cached sdir table
+    int sdir_D = -1;
+    double* h_sums = nullptr;     // pinned
+    size_t h_sums_cap = 0;
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int64_t timed_launches = 0;
+    double timed_ms = 0.0;
+};
+
+struct fdmc_bb {
+    fdmc_context* ctx;
+    int N;
+    double T;
+    float* d_w;   // [N][4] {wl, wr, sd, 0}
+};
+
+struct fdmc_pca {
+    fdmc_context* ctx;
+    int m;
+    void* d_f16;
+    void* d_bf16;
+    std::vector<double> M;  // column-major FP64 copy
+};
+
+// ---------------------------------------------------------------- direction numbers
+static void expand_dirvectors(std::vector<uint32_t>& v) {
+    v.assign((size_t)FDMC_JK_NDIM * 32, 0u);
+    for (int d = 0; d < FDMC_JK_NDIM; d++) {
+        uint32_t m[32];
+        if (d == 0) {
+            for (int j = 0; j < 32; j++) m[j] = 1u;
+        } else {
+            const uint32_t poly = fdmc_jk_poly[d];
+            const int s = 31 - __builtin_clz(poly);
+            const uint32_t a = (poly >> 1) & ((1u << (s - 1)) - 1u);
+            const uint32_t* mi = fdmc_jk_m + fdmc_jk_off[d];
+            for (int j = 0; j < s && j < 32; j++) m[j] = mi[j];
+            for (int j = s; j < 32; j++) {
+                uint32_t x = m[j - s] ^ (m[j - s] << s);
+                for (int k = 1; k < s; k++)
+                    if ((a >> (s - 1 - k)) & 1u) x ^= m[j - k] << k;
+                m[j] = x;
+            }
+        }
+        for (int j = 0; j < 32; j++) v[(size_t)d * 32 + j] = m[j] << (31 - j);
+    }
+}
+
+// ---------------------------------------------------------------- parameters
+namespace {
+
+int noise_dim(const fdmc_model& m) {
+    switch (m.kind) {
+    case FDMC_MODEL_HESTON: return 2;
+    case FDMC_MODEL_MULTI_DUPIRE:
+    case FDMC_MODEL_MULTI_GBM: return m.n_assets;
+    default: return 1;
+    }
+}
+bool is_basket(const fdmc_model& m) {
+    return m.kind == FDMC_MODEL_MULTI_DUPIRE || m.kind == FDMC_MODEL_MULTI_GBM;
+}
+
+void validate_model(const fdmc_model* m) {
+    if (!m) fail(FDMC_E_INVALID, "model is null");
+    if (m->kind < 0 || m->kind > FDMC_MODEL_MULTI_GBM) fail(FDMC_E_INVALID, "unknown model kind");
+    if (!(m->T > 0.0)) fail(FDMC_E_INVALID, "model T must be positive");
+    if (!is_basket(*m) && !(m->S0 > 0.0)) fail(FDMC_E_INVALID, "model S0 must be positive");
+    if (is_basket(*m)) {
+        if (m->n_assets <= 0) fail(FDMC_E_INVALID, "basket n_assets must be positive");
+        if (!m->S0_assets) fail(FDMC_E_INVALID, "basket S0_assets is null");
+        if (m->kind == FDMC_MODEL_MULTI_GBM && !m->sigma_assets)
+            fail(FDMC_E_INVALID, "MULTI_GBM sigma_assets is null");
+    }
+}
+void validate_payoff(const fdmc_payoff* p) {
+    if (!p) fail(FDMC_E_INVALID, "payoff is null");
+    if (p->kind < 0 || p->kind > FDMC_PAYOFF_BASKET) fail(FDMC_E_INVALID, "unknown payoff kind");
+}
+
+// make_params (methods_cuda.cu:986-1051) for one leg.
+LegParams make_leg(const fdmc_model& m, const fdmc_payoff& p) {
+    LegParams L{};
+    L.mu = (float)m.mu;
+    L.S0 = (float)m.S0;
+    switch (p.kind) {
+    case FDMC_PAYOFF_EUROPEAN:
+    case FDMC_PAYOFF_BASKET:
+        L.K = (float)p.K; L.discount = (float)std::exp(-p.r * p.T); break;
+    case FDMC_PAYOFF_BARRIER:
+        L.K = (float)p.K; L.B = (float)p.B; L.discount = (float)std::exp(-p.r * p.T); break;
+    case FDMC_PAYOFF_ASIAN:
+    case FDMC_PAYOFF_GEOM_ASIAN:
+        L.K = (float)p.K; L.discount = 1.0f; break;
+    default:
+        L.discount = 1.0f; break;
+    }
+    switch (m.kind) {
+    case FDMC_MODEL_GBM: L.sigma = (float)m.sigma; break;
+    case FDMC_MODEL_HESTON:
+        L.kappa = (float)m.kappa; L.theta = (float)m.theta; L.xi = (float)m.xi; L.v0 = (float)m.v0;
+        L.L10 = (float)m.L[2]; L.L11 = (float)m.L[3]; break;
+    case FDMC_MODEL_DUPIRE:
+        L.sigma0 = (float)m.sigma0; L.alpha = (float)m.alpha;
+        L.beta_m1 = (float)m.beta_d - 1.0f; L.inv_S0 = 1.0f / (float)m.S0;
+        L.sigma = (float)m.sigma0; break;
+    case FDMC_MODEL_LOCALVOL_QUAD:
+        L.lv_a = (float)m.lv_a; L.lv_b = (float)m.lv_b; L.lv_c = (float)m.lv_c;
+        L.inv_sref = 1.0f / (float)m.lv_sref; break;
+    default: break;
+    }
+    return L;
+}
+
+float bgk_factor(const fdmc_payoff& p, float h) {
+    if (p.kind != FDMC_PAYOFF_LOOKBACK && p.kind != FDMC_PAYOFF_BARRIER) return 1.0f;
+    return std::exp(-0.5826f * (float)p.sigma * std::sqrt(h));  // payoffs.hpp:8, :332-337
+}
+
+int model_kind_dev(const fdmc_model& m) {
+    switch (m.kind) {
+    case FDMC_MODEL_GBM: return MK_GBM;
+    case FDMC_MODEL_HESTON: return MK_HESTON;
+    case FDMC_MODEL_DUPIRE: return MK_DUPIRE;
+    case FDMC_MODEL_LOCALVOL_QUAD: return MK_LVQ;
+    case FDMC_MODEL_MULTI_DUPIRE: return MK_MULTI_DUPIRE;
+    default: return MK_MULTI_GBM;
+    }
+}
+// Single-asset models price a Basket payoff as a European on S_T, as the
+// reference's MC_TABLE does (methods_cuda.cu:345-347).
+int payoff_kind_dev(const fdmc_payoff& p) {
+    return p.kind == FDMC_PAYOFF_BASKET ? PK_EUROPEAN : p.kind;
+}
+
+bool same_model(const fdmc_model& a, const fdmc_model& b) {
+    return a.kind == b.kind && a.S0 == b.S0 && a.mu == b.mu && a.T == b.T && a.sigma == b.sigma &&
+           a.sigma0 == b.sigma0 && a.alpha == b.alpha && a.beta_d == b.beta_d && a.lv_a == b.lv_a &&
+           a.lv_b == b.lv_b && a.lv_c == b.lv_c && a.lv_sref == b.lv_sref && a.kappa == b.kappa &&
+           a.theta == b.theta && a.xi == b.xi && a.v0 == b.v0;
+}
+
+// ---------------------------------------------------------------- device helpers
+void ensure_host_sums(fdmc_context* c, size_t n) {
+    if (n <= c->h_sums_cap) return;
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
+    c->h_sums = nullptr;
+    c->h_sums_cap = 0;
+    HIPC(hipHostMalloc((void**)&c->h_sums, n * sizeof(double), hipHostMallocDefault));
+    c->h_sums_cap = n;
+}
+
+// Scrambled direction records for the given salts and D (cached per context).
+const uint32_t* get_sdir(fdmc_context* c, const std::vector<uint32_t>& salts, int D) {
+    if (D == c->sdir_D && salts == c->sdir_key) return (const uint32_t*)c->sdir.p;
+    const size_t n = salts.size() * (size_t)D * SDIR_WORDS;
+    c->sdir.ensure(n * sizeof(uint32_t));
+    c->salts.ensure(salts.size() * sizeof(uint32_t));
+    HIPC(hipMemcpyAsync(c->salts.p, salts.data(), salts.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                        c->stream));
+    HIPC(launch_build_sdir(c->d_dirv, (const uint32_t*)c->salts.p, (int)salts.size(), D,
+                           (uint32_t*)c->sdir.p, c->stream));
+    c->sdir_key = salts;
+    c->sdir_D = D;
+    return (const uint32_t*)c->sdir.p;
+}
+
+// ---------------------------------------------------------------- one fused pass
+struct PassSpec {
+    const fdmc_model* model;
+    const fdmc_payoff* payoff;
+    const fdmc_model* cmodel;     // control leg or null
+    const fdmc_payoff* cpayoff;
+    double E_ctrl = 0.0, beta = 0.0;
+    int n_steps = 0;
+    int R = 1;
+    uint32_t salt_base = 0;       // QMC: replica salt = salt_base + r; PRNG: seed = salt_base + r
+    int64_t first = 0, count = 0;
+    int mode = FDMC_NOISE_RAW;
+    bool prng = false;            // pseudo-random noise (MC, CV, QMC fallback)
+    uint32_t tag = 0;
+    const fdmc_bb* bb = nullptr;
+};
+
+// Runs one pass; writes R x nacc sums into out, returns nacc.
+int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
+    const fdmc_model& m = *s.model;
+    const fdmc_payoff& p = *s.payoff;
+    if (s.n_steps <= 0) fail(FDMC_E_INVALID, "n_steps must be positive");
+    if (s.R <= 0 || s.R > 65535) fail(FDMC_E_INVALID, "R out of range");
+    if (s.count <= 0) fail(FDMC_E_INVALID, "number of points must be positive");
+    if (is_basket(m)) fail(FDMC_E_UNSUPPORTED, "basket models are not supported by this pass yet");
+    const int nf = noise_dim(m);
+    const int64_t D64 = (int64_t)s.n_steps * nf;
+    const bool use_prng = s.prng || D64 > D_MAX_SOBOL;
+    if (!use_prng && s.first + s.count > (int64_t)4294967296LL)
+        fail(FDMC_E_RANGE, "Sobol point index beyond 32 bits (offset + n > 2^32)");
+    const int D = (int)D64;
+
+    PathLaunch a{};
+    const float T = (float)m.T;
+    a.h = T / (float)s.n_steps;
+    a.sqrt_h = std::sqrt(a.h);
+    a.em = std::exp(-(float)m.kappa * a.h);
+    a.leg[0] = make_leg(m, p);
+    a.bgk[0] = bgk_factor(p, a.h);
+    a.bgk[1] = 1.0f;
+    int cv = CV_NONE, cpk = 0;
+    if (s.cmodel && s.cpayoff) {
+        a.leg[1] = make_leg(*s.cmodel, *s.cpayoff);
+        a.bgk[1] = bgk_factor(*s.cpayoff, a.h);
+        cpk = payoff_kind_dev(*s.cpayoff);
+        if (same_model(m, *s.cmodel)) cv = CV_SHARED;
+        else if (s.cmodel->kind == FDMC_MODEL_GBM) cv = CV_GBM;
+        else fail(FDMC_E_UNSUPPORTED, "control leg must use the main model or a GBM model");
+        a.beta_cv = (float)s.beta;
+        a.E_ctrl = (float)s.E_ctrl;
+    }
+    a.n_steps = s.n_steps;
+    a.nf = nf;
+    a.D = D;
+    const int mk = model_kind_dev(m), pk = payoff_kind_dev(p);
+
+    int nz, lb = 0;
+    const void* fn = nullptr;
+    if (use_prng) {
+        if (s.mode != FDMC_NOISE_RAW && !s.prng)
+            fail(FDMC_E_UNSUPPORTED, "BB/PCA need Sobol points (dimension <= 20000)");
+        nz = NZ_PRNG;
+        a.seed = s.salt_base;
+        a.tag = s.prng ? s.tag : TAG_QMC_FALLBACK;
+        fn = path_kernel_prng(mk, pk, cv, cpk, 0);
+    } else if (s.mode == FDMC_NOISE_BB) {
+        if (nf != 1) fail(FDMC_E_UNSUPPORTED, "Brownian bridge needs a one-factor model");
+        if (!s.bb) fail(FDMC_E_INVALID, "BB mode needs a DeviceBBData");
+        if (s.bb->N != s.n_steps) fail(FDMC_E_INVALID, "BB data has N != n_steps");
+        if (s.n_steps > SOBOL_LDS_MAX_D)
+            fail(FDMC_E_UNSUPPORTED, "Brownian bridge supports at most 1024 steps");
+        nz = NZ_SOBOL_BB;
+        lb = s.n_steps >= 16 ? 4 : 0;
+        a.G = s.n_steps >> lb;
+        a.bbw = s.bb->d_w;
+        fn = path_kernel_bb(mk, pk, cv, cpk, lb);
+    } else if (s.mode == FDMC_NOISE_PCA) {
+        fail(FDMC_E_UNSUPPORTED, "PCA noise mode is not available for this estimator yet");
+    } else {
+        nz = NZ_SOBOL_RAW;
+        fn = path_kernel_raw(mk, pk, cv, cpk, 0);
+    }
+    if (!fn) fail(FDMC_E_UNSUPPORTED, "model/payoff/control combination not supported");
+
+    if (nz != NZ_PRNG) {
+        std::vector<uint32_t> salts((size_t)s.R);
+        for (int r = 0; r < s.R; r++) salts[r] = s.salt_base + (uint32_t)r;
+        a.sdir = get_sdir(c, salts, D);
+    }
+
+    // Tiles of 256 points in absolute index space; a few tiles per workgroup.
+    const int64_t t_first = s.first >> 8, t_last = (s.first + s.count - 1) >> 8;
+    const int64_t ntiles = t_last - t_first + 1;
+    const int64_t target_wgs = (int64_t)c->n_cu * 32;
+    int64_t per_rep = std::max<int64_t>(1, target_wgs / s.R);
+    int64_t tpw = std::max<int64_t>(1, (ntiles + per_rep - 1) / per_rep);
+    if (tpw > (1 << 20)) tpw = 1 << 20;
+    const int64_t nwg = (ntiles + tpw - 1) / tpw;
+    if (nwg > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many workgroups");
+    a.first = s.first;
+    a.count = s.count;
+    a.tiles_per_wg = (int)tpw;
+    a.nwg_x = (int)nwg;
+    const int nacc = (cv == CV_NONE) ? 2 : 7;
+    a.partials = (double*)c->partials.ensure((size_t)s.R * nwg * nacc * sizeof(double));
+    double* d_sums = (double*)c->sums.ensure((size_t)s.R * nacc * sizeof(double));
+
+    const size_t lds = path_lds_bytes(nz, D, a.G);
+    if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* args[] = {(void*)&a};
+    if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
+    HIPC(hipLaunchKernel(fn, dim3((unsigned)nwg, (unsigned)s.R), dim3(256), args, lds, c->stream));
+    if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
+    HIPC(launch_reduce_partials(a.partials, s.R, (int)nwg, nacc, d_sums, c->stream));
+    ensure_host_sums(c, (size_t)s.R * nacc);
+    HIPC(hipMemcpyAsync(c->h_sums, d_sums, (size_t)s.R * nacc * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        float ms = 0.0f;
+        HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->timed_ms += ms;
+        c->timed_launches++;
+    }
+    out.assign(c->h_sums, c->h_sums + (size_t)s.R * nacc);
+    return nacc;
+}
+
+void mean_var(const std::vector<double>& v, double& m, double& var) {
+    const int R = (int)v.size();
+    double s = 0.0;
+    for (double x : v) s += x;
+    s /= R;
+    double q = 0.0;
+    for (double x : v) q += (x - s) * (x - s);
+    m = s;
+    var = (R > 1) ? q / (R - 1) : 0.0;
+}
+
+// Doubling loop of run_qmc_cuda / run_qmc_cv_cuda (methods_cuda.cu:1624-1665,
+// 2203-2273): accumulate only the new points of each replica, stop when
+// var_of_means < eps^2/2.
+void qmc_loop(fdmc_context* c, PassSpec s, double eps, const fdmc_qmc_config& cfg, fdmc_result* out) {
+    const double t0 = now_s();
+    if (cfg.R <= 0) fail(FDMC_E_INVALID, "QMCConfig.R must be positive");
+    if (cfg.max_doublings <= 0) fail(FDMC_E_INVALID, "QMCConfig.max_doublings must be positive");
+    if (!(eps > 0.0)) fail(FDMC_E_INVALID, "eps must be positive");
+    int64_t N = cfg.N0 > 0 ? cfg.N0 : (1LL << 14);
+    if (N & (N - 1)) fail(FDMC_E_INVALID, "QMCConfig.N0 must be a power of two");
+    const int64_t cap = cfg.N_max > 0 ? cfg.N_max : (int64_t)4294967296LL;
+    const bool ctrl = s.cmodel && s.cpayoff;
+    const int col = ctrl ? 2 : 0;
+    s.R = cfg.R;
+    s.salt_base = cfg.salt_base;
+    std::vector<double> tot, part, means((size_t)cfg.R);
+    int64_t done = 0, total = 0;
+    double vom = 1e30, gm = 0.0;
+    int doublings = 0;
+    bool conv = false;
+    *out = fdmc_result{};
+    const int D = s.n_steps * noise_dim(*s.model);
+    for (int d = 0; d < cfg.max_doublings; d++) {
+        if (N > (int64_t)4294967296LL && D <= D_MAX_SOBOL)
+            fail(FDMC_E_RANGE, "N_per_replica exceeds the 32-bit Sobol offset");
+        s.first = done;
+        s.count = N - done;
+        const int nacc = run_pass(c, s, part);
+        if (tot.empty()) tot.assign(part.size(), 0.0);
+        for (size_t i = 0; i < part.size(); i++) tot[i] += part[i];
+        out->path_steps += (int64_t)cfg.R * (N - done) * s.n_steps;
+        done = N;
+        doublings++;
+        for (int r = 0; r < cfg.R; r++) means[r] = tot[(size_t)r * nacc + col] / (double)N;
+        double v;
+        mean_var(means, gm, v);
+        vom = v / cfg.R;
+        total = (int64_t)cfg.R * N;
+        if (vom < eps * eps / 2.0) { conv = true; break; }
+        if (2 * N > cap) break;
+        N *= 2;
+    }
+    out->price = gm;
+    out->std_error = std::sqrt(vom);
+    out->n_samples = total;
+    out->levels = doublings;
+    out->flags = (conv ? 0u : FDMC_FLAG_NOT_CONVERGED) | (D > D_MAX_SOBOL ? FDMC_FLAG_PRNG_FALLBACK : 0u);
+    out->time_s = now_s() - t0;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+int32_t fdmc_abi_version(void) { return FDMC_ABI_VERSION; }
+
+const char* fdmc_last_error(void) { return g_last_error.c_str(); }
+
+fdmc_status fdmc_context_create(int32_t device, fdmc_context** out) {
+    return guarded([&] {
+        if (!out) fail(FDMC_E_INVALID, "out is null");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fail(FDMC_E_DEVICE, "no HIP device available");
+        if (device < 0 || device >= n) fail(FDMC_E_INVALID, "device index out of range");
+        HIPC(hipSetDevice(device));
+        auto* c = new fdmc_context;
+        c->device = device;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->n_cu = prop.multiProcessorCount;
+        try {
+            HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            std::vector<uint32_t> dv;
+            expand_dirvectors(dv);
+            HIPC(hipMalloc((void**)&c->d_dirv, dv.size() * sizeof(uint32_t)));
+            HIPC(hipMemcpy(c->d_dirv, dv.data(), dv.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        } catch (...) {
+            fdmc_context_destroy(c);
+            throw;
+        }
+        *out = c;
+    });
+}
+
+fdmc_status fdmc_context_destroy(fdmc_context* c) {
+    if (!c) return FDMC_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->partials.release();
+    c->sums.release();
+    c->salts.release();
+    c->sdir.release();
+    c->gen_sdir.release();
+    if (c->d_dirv) (void)hipFree(c->d_dirv);
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FDMC_OK;
+}
+
+fdmc_status fdmc_context_stream(fdmc_context* c, void** s) {
+    return guarded([&] {
+        if (!c || !s) fail(FDMC_E_INVALID, "null argument");
+        *s = (void*)c->stream;
+    });
+}
+
+fdmc_status fdmc_timing_enable(fdmc_context* c, int32_t enable) {
+    return guarded([&] {
+        if (!c) fail(FDMC_E_INVALID, "null context");
+        if (enable && !c->ev0) {
+            HIPC(hipEventCreate(&c->ev0));
+            HIPC(hipEventCreate(&c->ev1));
+        }
+        c->timing = enable != 0;
+    });
+}
+
+fdmc_status fdmc_timing_read(fdmc_context* c, int64_t* launches, double* total_ms, int32_t reset) {
+    return guarded([&] {
+        if (!c) fail(FDMC_E_INVALID, "null context");
+        if (launches) *launches = c->timed_launches;
+        if (total_ms) *total_ms = c->timed_ms;
+        if (reset) {
+            c->timed_launches = 0;
+            c->timed_ms = 0.0;
+        }
+    });
+}
+
+fdmc_status fdmc_synchronize(fdmc_context* c) {
+    return guarded([&] {
+        if (!c) fail(FDMC_E_INVALID, "null context");
+        HIPC(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ---- BB / PCA ------------------------------------------------------------------
+fdmc_status fdmc_bb_upload(fdmc_context* c, int32_t N, double T, const int32_t* map_idx,
+                           const int32_t* left_idx, const int32_t* right_idx, const double* wl,
+                           const double* wr, const double* sd, fdmc_bb** out) {
+    return guarded([&] {
+        if (!c || !out || !map_idx || !left_idx || !right_idx || !wl || !wr || !sd)
+            fail(FDMC_E_INVALID, "null argument");
+        if (N <= 0 || (N & (N - 1))) fail(FDMC_E_INVALID, "BB N must be a power of two");
+        // The fused kernels assume the dyadic coarse-to-fine schedule of
+        // bb_precompute (utils.cpp:54-101); check it entry by entry.
+        if (map_idx[0] != N) fail(FDMC_E_INVALID, "BB entry 0 must map to W_N");
+        for (int j = 1; j < N; j++) {
+            const int lev = 31 - __builtin_clz((unsigned)j);
+            const int q = j - (1 << lev), st = N >> lev;
+            if (left_idx[j] != q * st || right_idx[j] != q * st + st || map_idx[j] != q * st + st / 2)
+                fail(FDMC_E_INVALID, "BB tables are not the dyadic schedule of bb_precompute");
+        }
+        std::vector<float> w((size_t)N * 4);
+        for (int j = 0; j < N; j++) {
+            w[4 * j + 0] = (float)wl[j];
+            w[4 * j + 1] = (float)wr[j];
+            w[4 * j + 2] = (float)sd[j];
+            w[4 * j + 3] = 0.0f;
+        }
+        auto* b = new fdmc_bb{c, N, T, nullptr};
+        if (hipMalloc((void**)&b->d_w, w.size() * sizeof(float)) != hipSuccess) {
+            delete b;
+            fail(FDMC_E_ALLOC, "hipMalloc failed");
+        }
+        HIPC(hipMemcpy(b->d_w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+        *out = b;
+    });
+}
+
+fdmc_status fdmc_bb_free(fdmc_bb* b) {
+    if (!b) return FDMC_OK;
+    if (b->d_w) (void)hipFree(b->d_w);
+    delete b;
+    return FDMC_OK;
+}
+
+fdmc_status fdmc_pca_upload(fdmc_context* c, int32_t m, const double* M, fdmc_pca** out) {
+    return guarded([&] {
+        if (!c || !M || !out) fail(FDMC_E_INVALID, "null argument");
+        if (m <= 0) fail(FDMC_E_INVALID, "PCA m must be positive");
+        const size_t n = (size_t)m * m;
+        std::vector<uint16_t> f16(n), bf16(n);
+        for (size_t i = 0; i < n; i++) {
+            const _Float16 h = (_Float16)M[i];
+            std::memcpy(&f16[i], &h, 2);
+            const float f = (float)M[i];
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+            bf16[i] = (uint16_t)(u >> 16);
+        }
+        auto* p = new fdmc_pca{c, m, nullptr, nullptr, std::vector<double>(M, M + n)};
+        if (hipMalloc(&p->d_f16, n * 2) != hipSuccess || hipMalloc(&p->d_bf16, n * 2) != hipSuccess) {
+            if (p->d_f16) (void)hipFree(p->d_f16);
+            delete p;
+            fail(FDMC_E_ALLOC, "hipMalloc failed");
+        }
+        HIPC(hipMemcpy(p->d_f16, f16.data(), n * 2, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(p->d_bf16, bf16.data(), n * 2, hipMemcpyHostToDevice));
+        *out = p;
+    });
+}
+
+fdmc_status fdmc_pca_free(fdmc_pca* p) {
+    if (!p) return FDMC_OK;
+    if (p->d_f16) (void)hipFree(p->d_f16);
+    if (p->d_bf16) (void)hipFree(p->d_bf16);
+    delete p;
+    return FDMC_OK;
+}
+
+// ---- estimators ------------------------------------------------------------------
+fdmc_status fdmc_qmc_accumulate(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff,
+                                const fdmc_model* cmodel, const fdmc_payoff* cpayoff, double E_ctrl,
+                                double beta, int32_t n_steps, int32_t R, uint32_t salt_base,
+                                int64_t offset, int64_t n_points, int32_t mode, const fdmc_bb* bb,
+                                const fdmc_pca* pca, double* sums_out, int32_t* n_acc_out) {
+    return guarded([&] {
+        if (!c || !sums_out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if ((cmodel == nullptr) != (cpayoff == nullptr))
+            fail(FDMC_E_INVALID, "control model and payoff must both be given");
+        if (cmodel) { validate_model(cmodel); validate_payoff(cpayoff); }
+        if (offset < 0) fail(FDMC_E_INVALID, "offset must be non-negative");
+        (void)pca;
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.cmodel = cmodel; s.cpayoff = cpayoff;
+        s.E_ctrl = E_ctrl; s.beta = beta; s.n_steps = n_steps; s.R = R; s.salt_base = salt_base;
+        s.first = offset; s.count = n_points; s.mode = mode; s.bb = bb;
+        std::vector<double> v;
+        const int nacc = run_pass(c, s, v);
+        std::memcpy(sums_out, v.data(), v.size() * sizeof(double));
+        if (n_acc_out) *n_acc_out = nacc;
+    });
+}
+
+fdmc_status fdmc_run_qmc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double eps,
+                         int32_t n_steps, const fdmc_qmc_config* cfg, int32_t mode, const fdmc_bb* bb,
+                         const fdmc_pca* pca, fdmc_result* out) {
+    return guarded([&] {
+        if (!c || !cfg || !out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (mode == FDMC_NOISE_BB && noise_dim(*model) != 1)
+            fail(FDMC_E_UNSUPPORTED, "Brownian bridge needs a one-factor model");
+        (void)pca;
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.n_steps = n_steps; s.mode = mode; s.bb = bb;
+        qmc_loop(c, s, eps, *cfg, out);
+    });
+}
+
+fdmc_status fdmc_run_qmc_cv(fdmc_context* c, const fdmc_model* mm, const fdmc_model* cm, const fdmc_payoff* mp,
+                            const fdmc_payoff* cp, double E_ctrl, double beta, double eps, int32_t n_steps,
+                            const fdmc_qmc_config* cfg, int32_t mode, const fdmc_bb* bb, fdmc_result* out) {
+    return guarded([&] {
+        if (!c || !cfg || !out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(mm); validate_model(cm); validate_payoff(mp); validate_payoff(cp);
+        PassSpec s;
+        s.model = mm; s.payoff = mp; s.cmodel = cm; s.cpayoff = cp; s.E_ctrl = E_ctrl; s.beta = beta;
+        s.n_steps = n_steps; s.mode = mode; s.bb = bb;
+        qmc_loop(c, s, eps, *cfg, out);
+    });
+}
+
+fdmc_status fdmc_run_mc_fixed(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff,
+                              int32_t n_steps, int64_t n_paths, uint32_t seed, double* mean_out,
+                              double* var_out) {
+    return guarded([&] {
+        if (!c || !mean_out || !var_out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (n_paths <= 0) fail(FDMC_E_INVALID, "n_paths must be positive");
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.n_steps = n_steps; s.prng = true; s.tag = 0;
+        s.salt_base = seed; s.first = 0; s.count = n_paths;
+        std::vector<double> v;
+        run_pass(c, s, v);
+        const double m = v[0] / (double)n_paths;
+        const double var = v[1] / (double)n_paths - m * m;
+        *mean_out = m;
+        *var_out = std::max(0.0, var) / (double)n_paths;
+    });
+}
+
+fdmc_status fdmc_run_mc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double eps,
+                        int32_t n_steps, const fdmc_mc_config* cfg, fdmc_result* out) {
+    return guarded([&] {
+        if (!c || !cfg || !out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (!(eps > 0.0)) fail(FDMC_E_INVALID, "eps must be positive");
+        if (cfg->pilot_n <= 1) fail(FDMC_E_INVALID, "pilot_n must be > 1");
+        const double t0 = now_s();
+        *out = fdmc_result{};
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.n_steps = n_steps; s.prng = true; s.tag = 0;
+        s.salt_base = cfg->seed; s.first = 0; s.count = cfg->pilot_n;
+        std::vector<double> v;
+        run_pass(c, s, v);  // pilot (run_mc_cuda, methods_cuda.cu:1276-1280)
+        const double pm = v[0] / cfg->pilot_n;
+        const double pvar = std::max(0.0, v[1] / cfg->pilot_n - pm * pm);
+        int64_t N = (int64_t)std::ceil(2.0 * pvar / (eps * eps));
+        N = std::max<int64_t>(N, cfg->pilot_n);
+        s.salt_base = cfg->seed + 1u;
+        s.count = N;
+        run_pass(c, s, v);
+        const double m = v[0] / (double)N;
+        const double var = std::max(0.0, v[1] / (double)N - m * m) / (double)N;
+        out->price = m;
+        out->std_error = std::sqrt(var);
+        out->n_samples = N;
+        out->path_steps = (N + cfg->pilot_n) * (int64_t)n_steps;
+        out->time_s = now_s() - t0;
+    });
+}
+
+fdmc_status fdmc_cv_pilot(fdmc_context* c, const fdmc_model* mm, const fdmc_model* cm, const fdmc_payoff* mp,
+                          const fdmc_payoff* cp, double E_ctrl, int32_t n_steps, int32_t N_pilot, uint32_t seed,
+                          fdmc_cv_pilot_result* out) {
+    return guarded([&] {
+        if (!c || !out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(mm); validate_model(cm); validate_payoff(mp); validate_payoff(cp);
+        if (N_pilot <= 1) fail(FDMC_E_INVALID, "N_pilot must be > 1");
+        PassSpec s;
+        s.model = mm; s.payoff = mp; s.cmodel = cm; s.cpayoff = cp; s.E_ctrl = E_ctrl; s.beta = 0.0;
+        s.n_steps = n_steps; s.prng = true; s.tag = TAG_CV_PILOT; s.salt_base = seed;
+        s.first = 0; s.count = N_pilot;
+        std::vector<double> hs;
+        run_pass(c, s, hs);
+        // methods_cuda.cu:2061-2075
+        const double N = (double)N_pilot;
+        const double Ea = hs[0] / N, Ea2 = hs[1] / N, Eg = hs[4] / N, Eg2 = hs[5] / N, Eag = hs[6] / N;
+        const double vm = std::max(0.0, Ea2 - Ea * Ea);
+        const double vc = std::max(1e-30, Eg2 - Eg * Eg);
+        const double cov = Eag - Ea * Eg;
+        out->beta = std::clamp(cov / vc, 0.0, 5.0);
+        out->var_plain = vm;
+        out->var_cv = std::max(0.0, vm - cov * cov / vc);
+    });
+}
+
+fdmc_status fdmc_run_mc_cv(fdmc_context* c, const fdmc_model* mm, const fdmc_model* cm, const fdmc_payoff* mp,
+                           const fdmc_payoff* cp, double E_ctrl, double beta, double eps, int32_t n_steps,
+                           const fdmc_mc_config* cfg, fdmc_result* out) {
+    return guarded([&] {
+        if (!c || !cfg || !out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(mm); validate_model(cm); validate_payoff(mp); validate_payoff(cp);
+        if (!(eps > 0.0)) fail(FDMC_E_INVALID, "eps must be positive");
+        const double t0 = now_s();
+        *out = fdmc_result{};
+        PassSpec s;
+        s.model = mm; s.payoff = mp; s.cmodel = cm; s.cpayoff = cp; s.E_ctrl = E_ctrl; s.beta = beta;
+        s.n_steps = n_steps; s.prng = true; s.tag = TAG_CV_PILOT; s.salt_base = cfg->seed;
+        s.first = 0; s.count = cfg->pilot_n;
+        std::vector<double> hs;
+        run_pass(c, s, hs);  // methods_cuda.cu:2116-2120
+        const double mu = hs[2] / cfg->pilot_n;
+        const double v = std::max(0.0, hs[3] / cfg->pilot_n - mu * mu);
+        int64_t N = (int64_t)std::ceil(2.0 * v / (eps * eps));
+        N = std::max<int64_t>(N, cfg->pilot_n);
+        s.salt_base = cfg->seed + 100u;
+        s.tag = TAG_CV_MAIN;
+        s.count = N;
+        run_pass(c, s, hs);
+        const double m = hs[2] / (double)N;
+        const double vf = std::max(0.0, hs[3] / (double)N - m * m) / (double)N;
+        out->price = m;
+        out->std_error = std::sqrt(vf);
+        out->n_samples = N;
+        out->path_steps = (N + cfg->pilot_n) * (int64_t)n_steps;
+        out->time_s = now_s() - t0;
+    });
+}
+
+fdmc_status fdmc_run_mlmc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double eps,
+                          const fdmc_mlmc_config* cfg, fdmc_result* out) {
+    return guarded([&] {
+        (void)c; (void)model; (void)payoff; (void)eps; (void)cfg; (void)out;
+        fail(FDMC_E_UNSUPPORTED, "run_mlmc: multilevel kernel not built yet");
+    });
+}
+
+fdmc_status fdmc_run_mlqmc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double eps,
+                           const fdmc_mlmc_config* ml, const fdmc_qmc_config* qc, int32_t mode,
+                           const fdmc_bb* const* bbl, int32_t n_bb, const fdmc_pca* const* pcal, int32_t n_pca,
+                           fdmc_result* out) {
+    return guarded([&] {
+        (void)c; (void)model; (void)payoff; (void)eps; (void)ml; (void)qc; (void)mode; (void)bbl;
+        (void)n_bb; (void)pcal; (void)n_pca; (void)out;
+        fail(FDMC_E_UNSUPPORTED, "run_mlqmc: multilevel kernel not built yet");
+    });
+}
+
+// ---- generator hooks ----------------------------------------------------------------
+static fdmc_status sobol_out(fdmc_context* c, uint32_t salt, int32_t D, int64_t n, int64_t offset, void* d_out,
+                             bool normal) {
+    return guarded([&] {
+        if (!c || !d_out) fail(FDMC_E_INVALID, "null argument");
+        if (D <= 0 || D > D_MAX_SOBOL) fail(FDMC_E_RANGE, "D must be in [1, 20000]");
+        if (n <= 0 || offset < 0) fail(FDMC_E_INVALID, "n must be positive, offset non-negative");
+        if (offset + n > (int64_t)4294967296LL)
+            fail(FDMC_E_RANGE, "offset + n exceeds the 32-bit Sobol index (methods_cuda.cu:834-837)");
+        std::vector<uint32_t> salts{salt};
+        const size_t recs = (size_t)D * SDIR_WORDS;
+        c->gen_sdir.ensure(recs * sizeof(uint32_t));
+        c->salts.ensure(sizeof(uint32_t));
+        HIPC(hipMemcpyAsync(c->salts.p, salts.data(), sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        HIPC(launch_build_sdir(c->d_dirv, (const uint32_t*)c->salts.p, 1, D, (uint32_t*)c->gen_sdir.p,
+                               c->stream));
+        c->sdir_D = -1;  // salts buffer reused: invalidate the pass cache key
+        HIPC(launch_sobol_gen((const uint32_t*)c->gen_sdir.p, D, n, (uint32_t)offset, normal, d_out, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    });
+}
+
+fdmc_status fdmc_sobol_scrambled_u32(fdmc_context* c, uint32_t salt, int32_t D, int64_t n, int64_t offset,
+                                     uint32_t* d_out) {
+    return sobol_out(c, salt, D, n, offset, d_out, false);
+}
+
+fdmc_status fdmc_sobol_normal(fdmc_context* c, uint32_t salt, int32_t D, int64_t n, int64_t offset,
+                              float* d_out) {
+    return sobol_out(c, salt, D, n, offset, d_out, true);
+}
+
+fdmc_status fdmc_icdf_u32(fdmc_context* c, const uint32_t* d_in, int64_t n, float* d_out) {
+    return guarded([&] {
+        if (!c || !d_in || !d_out) fail(FDMC_E_INVALID, "null argument");
+        if (n <= 0) fail(FDMC_E_INVALID, "n must be positive");
+        HIPC(launch_icdf(d_in, n, d_out, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

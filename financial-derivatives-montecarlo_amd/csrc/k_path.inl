@@ -1,0 +1,271 @@
+// k_path.inl -- fused single-level path kernel (included by k_path_*.hip).
+//
+// One lane = one path; a 256-lane workgroup walks `tiles_per_wg` consecutive
+// 256-point tiles of one replica (blockIdx.y).  Per path it fuses what the
+// reference runs as 4-6 kernels with HBM round trips between them
+// (methods_cuda.cu:1551-1621): scrambled Sobol coordinate -> inverse CDF ->
+// Brownian bridge (or raw scaling) -> Euler steps -> running payoff (+ control
+// leg) -> FP64 partial sums.  Nothing per path touches HBM.
+//
+// LDS (dynamic):
+//   rec  [D][20] u32  TA[8] | TB[8] | C[4]   (Sobol, D <= SOBOL_LDS_MAX_D)
+//   skel [G+1][256] f32 bridge skeleton, one column per lane (BB)
+//   red  [4][7] f64   workgroup reduction
+// Brownian bridge: the dyadic schedule of bb_precompute (utils.cpp:54-101) is
+// split at block size B = 16: entries j < G (coarse levels) build W at the block
+// boundaries in LDS; the 15 interior entries of block i are
+// j = (G << l) + (i << l) + q (l = 0..3, q < 2^l) and are built in registers,
+// immediately followed by the 16 Euler steps of the block.  Each W value is the
+// same formula wl W_left + wr W_right + sd z as kernel_bb_transform (:561-594).
+#pragma once
+
+#include "device_math.hpp"
+#include "engine_internal.hpp"
+
+namespace fdmc {
+
+struct SobolLane {
+    const uint32_t* rec;     // LDS records (LDS mode) or null
+    const uint32_t* grec;    // global records of this replica
+    uint32_t ga, gb, w;      // lane Gray bits 0..2, 3..5; wave index
+    uint32_t hi_gray;        // gray(base + 64 w) (global mode)
+    bool lds_mode;
+};
+
+__device__ __forceinline__ uint32_t sobol_coord(const SobolLane& s, int d) {
+    if (s.lds_mode) {
+        const uint32_t* r = s.rec + d * 20;
+        return r[s.ga] ^ r[8 + s.gb] ^ r[16 + s.w];
+    }
+    const uint32_t* r = s.grec + (int64_t)d * SDIR_WORDS;
+    uint32_t x = r[s.ga] ^ r[8 + s.gb];
+    for (uint32_t g = s.hi_gray; g; g &= g - 1u) x ^= r[32 + __builtin_ctz(g)];
+    return x;
+}
+
+template <int MK, int PK>
+struct Leg {
+    float S, V, run;
+    __device__ __forceinline__ void init(const LegParams& P) {
+        S = P.S0;
+        V = P.v0;
+        run = run_init<PK>(P);
+    }
+    __device__ __forceinline__ void step1(const LegParams& P, float dw, float t, float h) {
+        S = step_1f<MK>(P, S, dw, t, h);
+        if constexpr (is_path_dep<PK>()) run_update<PK>(run, S);
+    }
+    __device__ __forceinline__ void stepH(const LegParams& P, float dw1, float dw2, float h, float em) {
+        step_heston(P, S, V, dw1, dw2, h, em);
+        if constexpr (is_path_dep<PK>()) run_update<PK>(run, S);
+    }
+    __device__ __forceinline__ float pay(const LegParams& P, int n, float bgk) const {
+        return payoff<PK>(P, S, run, n, bgk);
+    }
+};
+
+// Path state: main leg + optional control leg.
+template <int MK, int PK, int CV, int CPK>
+struct PathState {
+    Leg<MK, PK> m;
+    Leg<MK_GBM, CPK> c;      // CV_GBM: independent GBM leg; CV_SHARED: only .run is used
+    __device__ __forceinline__ void init(const PathLaunch& a) {
+        m.init(a.leg[0]);
+        if constexpr (CV == CV_GBM) c.init(a.leg[1]);
+        if constexpr (CV == CV_SHARED) c.run = run_init<CPK>(a.leg[1]);
+    }
+    __device__ __forceinline__ void step(const PathLaunch& a, float dw, int k) {
+        const float t = (float)k * a.h;
+        m.step1(a.leg[0], dw, t, a.h);
+        if constexpr (CV == CV_GBM) c.step1(a.leg[1], dw, t, a.h);
+        if constexpr (CV == CV_SHARED) {
+            if constexpr (is_path_dep<CPK>()) run_update<CPK>(c.run, m.S);
+        }
+    }
+    __device__ __forceinline__ void stepH(const PathLaunch& a, float dw1, float dw2) {
+        m.stepH(a.leg[0], dw1, dw2, a.h, a.em);
+    }
+    __device__ __forceinline__ void finish(const PathLaunch& a, float& ym, float& yc) const {
+        ym = m.pay(a.leg[0], a.n_steps, a.bgk[0]);
+        if constexpr (CV == CV_GBM) yc = c.pay(a.leg[1], a.n_steps, a.bgk[1]);
+        else if constexpr (CV == CV_SHARED) yc = payoff<CPK>(a.leg[1], m.S, c.run, a.n_steps, a.bgk[1]);
+        else yc = 0.0f;
+    }
+};
+
+template <int MK, int PK, int CV, int CPK, int NZ, int LB>
+__global__ __launch_bounds__(256) void k_path(PathLaunch a) {
+    constexpr int NACC = (CV == CV_NONE) ? 2 : 7;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = blockIdx.y;
+    const int D = a.D;
+    const bool lds_mode = (NZ != NZ_PRNG) && D <= SOBOL_LDS_MAX_D;
+    uint32_t* rec = (uint32_t*)smem;
+    const int rec_words = (NZ != NZ_PRNG && lds_mode) ? D * 20 : 0;
+    float* skel = (float*)(smem + (size_t)rec_words * 4);
+    const int skel_words = (NZ == NZ_SOBOL_BB) ? (a.G + 1) * 256 : 0;
+    double* red = (double*)(smem + (((size_t)(rec_words + skel_words) * 4 + 15) & ~(size_t)15));
+
+    const uint32_t* grec = (NZ != NZ_PRNG) ? a.sdir + (int64_t)r * D * SDIR_WORDS : nullptr;
+    SobolLane sl;
+    {
+        const uint32_t g = (uint32_t)(lane ^ (lane >> 1));
+        sl.rec = rec;
+        sl.grec = grec;
+        sl.ga = g & 7u;
+        sl.gb = g >> 3;
+        sl.w = (uint32_t)(tid >> 6);
+        sl.hi_gray = 0;
+        sl.lds_mode = lds_mode;
+    }
+    if (NZ != NZ_PRNG && lds_mode) {
+        for (int i = tid; i < D * 16; i += 256) {
+            const int d = i >> 4, k = i & 15;
+            rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
+        }
+    }
+
+    double acc[NACC];
+#pragma unroll
+    for (int i = 0; i < NACC; i++) acc[i] = 0.0;
+
+    const int64_t last = a.first + a.count;
+    const int64_t tile0 = (a.first >> 8) + (int64_t)blockIdx.x * a.tiles_per_wg;
+    for (int it = 0; it < a.tiles_per_wg; it++) {
+        const int64_t base = (tile0 + it) << 8;
+        if (base >= last) break;
+        if constexpr (NZ != NZ_PRNG) {
+            const uint32_t b32 = (uint32_t)base;
+            const uint32_t gbase = b32 ^ (b32 >> 1);
+            if (lds_mode) {
+                __syncthreads();
+                for (int d = tid; d < D; d += 256) {
+                    const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+                    uint32_t s = 0;
+                    for (uint32_t g = gbase; g; g &= g - 1u) s ^= gr[32 + __builtin_ctz(g)];
+                    rec[d * 20 + 16] = s ^ gr[16];
+                    rec[d * 20 + 17] = s ^ gr[17];
+                    rec[d * 20 + 18] = s ^ gr[18];
+                    rec[d * 20 + 19] = s ^ gr[19];
+                }
+                __syncthreads();
+            } else {
+                const uint32_t hi = b32 + (sl.w << 6);
+                sl.hi_gray = hi ^ (hi >> 1);
+            }
+        }
+        const int64_t idx = base + tid;
+        const bool valid = idx >= a.first && idx < last;
+
+        PathState<MK, PK, CV, CPK> ps;
+        ps.init(a);
+
+        if constexpr (NZ == NZ_PRNG) {
+            // Philox normals, 4 per call; normal n = k * nf + c.
+            const int nn = a.n_steps * a.nf;
+            float z[4];
+            if constexpr (MK == MK_HESTON) {
+                for (int b = 0; 4 * b < nn; b++) {
+                    prng_normals4(a.seed + (uint32_t)r, a.tag, (uint64_t)idx, (uint32_t)b, z);
+#pragma unroll
+                    for (int h2 = 0; h2 < 2; h2++) {
+                        if (2 * b + h2 < a.n_steps) {
+                            const float d1 = z[2 * h2] * a.sqrt_h, d2 = z[2 * h2 + 1] * a.sqrt_h;
+                            ps.stepH(a, d1, a.leg[0].L10 * d1 + a.leg[0].L11 * d2);
+                        }
+                    }
+                }
+            } else {
+                for (int b = 0; 4 * b < nn; b++) {
+                    prng_normals4(a.seed + (uint32_t)r, a.tag, (uint64_t)idx, (uint32_t)b, z);
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        if (4 * b + c < a.n_steps) ps.step(a, z[c] * a.sqrt_h, 4 * b + c);
+                }
+            }
+        } else if constexpr (NZ == NZ_SOBOL_RAW) {
+            if constexpr (MK == MK_HESTON) {
+                for (int k = 0; k < a.n_steps; k++) {
+                    const float d1 = icdf_u32(sobol_coord(sl, 2 * k)) * a.sqrt_h;
+                    const float d2 = icdf_u32(sobol_coord(sl, 2 * k + 1)) * a.sqrt_h;
+                    ps.stepH(a, d1, a.leg[0].L10 * d1 + a.leg[0].L11 * d2);
+                }
+            } else {
+                for (int k = 0; k < a.n_steps; k++)
+                    ps.step(a, icdf_u32(sobol_coord(sl, k)) * a.sqrt_h, k);
+            }
+        } else {  // NZ_SOBOL_BB (one-factor models)
+            const float4* bw = (const float4*)a.bbw;
+            const int G = a.G;
+            float* sk = skel + tid;
+            sk[0] = 0.0f;
+            sk[G * 256] = bw[0].z * icdf_u32(sobol_coord(sl, 0));
+            for (int j = 1; j < G; j++) {
+                const int lev = 31 - __builtin_clz(j);
+                const int q = j - (1 << lev);
+                const int st = G >> lev;
+                const int L = q * st, R = L + st, M = L + (st >> 1);
+                const float4 c = bw[j];
+                sk[M * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32(sobol_coord(sl, j));
+            }
+            if constexpr (LB == 0) {
+                float left = 0.0f;
+                for (int i = 0; i < G; i++) {
+                    const float right = sk[(i + 1) * 256];
+                    ps.step(a, right - left, i);
+                    left = right;
+                }
+            } else {
+                constexpr int B = 1 << LB;
+                float left = 0.0f;
+                for (int i = 0; i < G; i++) {
+                    float W[B + 1];
+                    W[0] = left;
+                    W[B] = sk[(i + 1) * 256];
+#pragma unroll
+                    for (int l = 0; l < LB; l++) {
+                        const int j0 = (G << l) + (i << l);
+#pragma unroll
+                        for (int q = 0; q < (1 << l); q++) {
+                            const int s = B >> l;
+                            const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
+                            const float4 c = bw[j0 + q];
+                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32(sobol_coord(sl, j0 + q));
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < B; k++) ps.step(a, W[k + 1] - W[k], i * B + k);
+                    left = W[B];
+                }
+            }
+        }
+
+        float ym, yc;
+        ps.finish(a, ym, yc);
+        if (valid) {
+            const double dm = (double)ym;
+            acc[0] += dm;
+            acc[1] += dm * dm;
+            if constexpr (CV != CV_NONE) {
+                const float ycv = ym - a.beta_cv * (yc - a.E_ctrl);
+                const double dv = (double)ycv, dc = (double)yc;
+                acc[2] += dv;
+                acc[3] += dv * dv;
+                acc[4] += dc;
+                acc[5] += dc * dc;
+                acc[6] += dm * dc;
+            }
+        }
+    }
+
+    __syncthreads();
+    block_sum<NACC, 4>(acc, red);
+    if (tid == 0) {
+        double* out = a.partials + ((int64_t)r * a.nwg_x + blockIdx.x) * NACC;
+#pragma unroll
+        for (int i = 0; i < NACC; i++) out[i] = acc[i];
+    }
+}
+
+}  // namespace fdmc

@@ -1,0 +1,85 @@
+// kernel_params.hpp -- POD launch parameters shared by host code and kernels.
+//
+// The reference flattens a model and a payoff into one global __constant__
+// KernelParams (methods_cuda.cu:218-246), rewritten before every launch.  Here
+// each launch carries its own parameters by value (kernel arguments), so
+// launches on one stream never race on shared constant memory.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fdmc {
+
+enum ModelKind : int { MK_GBM = 0, MK_HESTON = 1, MK_DUPIRE = 2, MK_MULTI_DUPIRE = 3, MK_LVQ = 4,
+                       MK_MULTI_GBM = 5 };
+enum PayoffKind : int { PK_EUROPEAN = 0, PK_ASIAN = 1, PK_GEOM_ASIAN = 2, PK_LOOKBACK = 3,
+                        PK_BARRIER = 4, PK_BASKET = 5 };
+// Noise sources of the fused path kernels.
+enum NoiseKind : int { NZ_SOBOL_RAW = 0, NZ_SOBOL_BB = 1, NZ_PRNG = 2 };
+// Control-variate leg: none, a second payoff on the same path, or a GBM leg.
+enum CtrlKind : int { CV_NONE = 0, CV_SHARED = 1, CV_GBM = 2 };
+
+// One (model, payoff) leg in FP32, as make_params builds it (methods_cuda.cu:986-1051).
+struct LegParams {
+    float S0, mu, sigma, discount, K, B;
+    float theta, xi, L10, L11, v0, kappa;
+    float sigma0, alpha, beta_m1, inv_S0;
+    float lv_a, lv_b, lv_c, inv_sref;
+};
+
+// Scrambled Sobol direction record of one (replica, dimension), 64 words:
+//  [0..7]   TA[k] = shift ^ XOR_{j<3, bit j of k} Lv[j]   (lane Gray bits 0..2)
+//  [8..15]  TB[k] = XOR_{j<3, bit j of k} Lv[j+3]         (lane Gray bits 3..5)
+//  [16..19] WV[w] = XOR_{j in gray(64 w)} Lv[j]           (wave index within 256)
+//  [32..63] Lv[0..31]
+// so that for point index i = base + 64 w + lane (base a multiple of 256):
+//  scrambled(i) = TA[g & 7] ^ TB[g >> 3] ^ WV[w] ^ XOR_{j in gray(base)} Lv[j],
+//  g = gray(lane).
+constexpr int SDIR_WORDS = 64;
+
+struct PathLaunch {
+    LegParams leg[2];       // main, control
+    float h, sqrt_h, em;    // step, sqrt(step), e^{-kappa h}
+    float bgk[2];           // exp(-0.5826 sigma_bgk sqrt(h)) per leg (1 when unused)
+    float beta_cv, E_ctrl;
+    int n_steps, nf, D;
+    int G, log2_G;          // BB: coarse blocks (N / 16, or N when N < 16)
+    uint32_t seed, tag;     // PRNG stream key
+    const uint32_t* sdir;   // [R][D][SDIR_WORDS]
+    const float* bbw;       // BB weights [N][4] = {wl, wr, sd, 0} in bb_precompute order
+    const float* L;         // basket: n x n lower Cholesky (row-major) or null
+    int64_t first;          // first point index (inclusive)
+    int64_t count;          // points per replica
+    int tiles_per_wg;       // 256-point tiles per workgroup
+    int nwg_x;              // workgroups per replica
+    double* partials;       // [R][nwg_x][NACC]
+};
+
+// Coupled multilevel launch: one work item = one 256-path tile of one level.
+struct LevelWork {
+    int level, n_fine, replica;
+    int64_t first;          // first sample index of the tile range
+    int64_t count;          // samples in this level/replica request
+    int tile0, ntiles;      // tile range of this entry within the queue
+    float h_f, h_c, sqrt_hf, em_f, em_c, bgk_f, bgk_c;
+    uint32_t seed, tag;
+    const uint32_t* sdir;   // per (level, replica) direction records or null (PRNG)
+    const float* bbw;       // BB weights for N = n_fine or null
+    int slot;               // output slot (level * R + replica)
+};
+
+// Sobol direction records are staged in LDS up to this dimension; above it the
+// path kernels read them from global memory (L2-resident).
+constexpr int SOBOL_LDS_MAX_D = 1024;
+
+// Dynamic LDS bytes of one k_path launch (layout documented in k_path.inl).
+inline size_t path_lds_bytes(int nz, int D, int G) {
+    size_t b = 0;
+    if (nz != NZ_PRNG && D <= SOBOL_LDS_MAX_D) b += (size_t)D * 20 * 4;
+    if (nz == NZ_SOBOL_BB) b += (size_t)(G + 1) * 256 * 4;
+    b = (b + 15) & ~(size_t)15;
+    return b + 4 * 7 * sizeof(double);
+}
+
+}  // namespace fdmc

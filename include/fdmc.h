@@ -1,0 +1,232 @@
+/*
+ * fdmc.h -- C ABI of the MI355X (gfx950) Monte Carlo path-simulation engine.
+ *
+ * Plain C types only (no HIP, torch or C++ types).  Every run_* entry point
+ * replaces one function of the reference's methods_cuda.cuh; the citation is
+ * given beside each declaration.  The C++ facade methods_hip.hpp re-exposes the
+ * reference's own signatures (std::variant models/payoffs, MCResult, ...) on top
+ * of this ABI.
+ *
+ * Conventions
+ *  - Every function returns an fdmc_status.  On failure a message is available
+ *    from fdmc_last_error() (thread-local, valid until the next call on the
+ *    same thread).  No C++ exception crosses this boundary.
+ *  - Inputs are validated before any device work is queued.
+ *  - run_* calls are synchronous: they return after the device work finished.
+ *  - A context owns one HIP stream (default priority), the device tables
+ *    (Sobol direction numbers, scrambled direction-number caches) and grow-only
+ *    scratch buffers.  Use one context per host thread.
+ *  - Pointers named d_* are device pointers; all others are host pointers.
+ */
+#ifndef FDMC_H
+#define FDMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDMC_ABI_VERSION 1
+
+typedef enum {
+    FDMC_OK = 0,
+    FDMC_E_INVALID = 1,     /* bad argument (reference: std::runtime_error from validation) */
+    FDMC_E_UNSUPPORTED = 2, /* model/payoff/mode combination the engine rejects */
+    FDMC_E_RANGE = 3,       /* size or offset beyond a documented limit */
+    FDMC_E_DEVICE = 4,      /* HIP runtime error */
+    FDMC_E_ALLOC = 5,       /* device or host allocation failed */
+    FDMC_E_INTERNAL = 6
+} fdmc_status;
+
+/* ---- models (reference models.hpp:15-71; kinds 4 and 5 are additive) ---- */
+typedef enum {
+    FDMC_MODEL_GBM = 0,          /* GBMParams          models.hpp:15-21 */
+    FDMC_MODEL_HESTON = 1,       /* HestonParams       models.hpp:23-43 */
+    FDMC_MODEL_DUPIRE = 2,       /* DupireLocalParams  models.hpp:45-54 */
+    FDMC_MODEL_MULTI_DUPIRE = 3, /* MultiDupireParams  models.hpp:56-68 */
+    FDMC_MODEL_LOCALVOL_QUAD = 4,/* sigma(S,t) = a + b exp(-c t) ln(S/s_ref)^2 */
+    FDMC_MODEL_MULTI_GBM = 5     /* n correlated GBM assets with per-asset sigma */
+} fdmc_model_kind;
+
+typedef struct fdmc_model {
+    int32_t kind;
+    int32_t n_assets;            /* MULTI_*: number of assets */
+    double S0, mu, T;
+    double sigma;                /* GBM */
+    double kappa, theta, xi, rho, v0; /* HESTON */
+    double L[4];                 /* HESTON: 2x2 Cholesky, row-major (compute_cholesky) */
+    double sigma0, alpha, beta_d;/* DUPIRE, MULTI_DUPIRE */
+    double lv_a, lv_b, lv_c, lv_sref; /* LOCALVOL_QUAD */
+    int32_t uncorrelated;        /* MULTI_*: nonzero -> identity correlation */
+    int32_t reserved0;
+    const double* S0_assets;     /* MULTI_*: n_assets values */
+    const double* sigma_assets;  /* MULTI_GBM: n_assets values */
+    const double* L_assets;      /* MULTI_*: n x n lower Cholesky, row-major; NULL -> identity */
+} fdmc_model;
+
+/* ---- payoffs (reference payoffs.hpp:15-22) ---- */
+typedef enum {
+    FDMC_PAYOFF_EUROPEAN = 0,    /* max(S_T-K,0) e^{-rT} */
+    FDMC_PAYOFF_ASIAN = 1,       /* max(mean S_k - K, 0), undiscounted */
+    FDMC_PAYOFF_GEOM_ASIAN = 2,  /* max(exp(mean log S_k) - K, 0), undiscounted */
+    FDMC_PAYOFF_LOOKBACK = 3,    /* S_T - min S_k (BGK-shifted), undiscounted */
+    FDMC_PAYOFF_BARRIER = 4,     /* up-and-out call with BGK shift, e^{-rT} */
+    FDMC_PAYOFF_BASKET = 5       /* max(sum w_a S_T^a - K, 0) e^{-rT} */
+} fdmc_payoff_kind;
+
+typedef struct fdmc_payoff {
+    int32_t kind;
+    int32_t n_assets;            /* BASKET */
+    double K, r, T;              /* r and T only set the discount */
+    double B;                    /* BARRIER level */
+    double sigma;                /* LOOKBACK/BARRIER: BGK volatility */
+    const double* weights;       /* BASKET: n_assets weights; NULL -> 1/n */
+} fdmc_payoff;
+
+/* ---- configurations (reference methods_cuda.cuh:10-32; trailing fields additive) ---- */
+typedef struct fdmc_mc_config {  /* MCConfig */
+    int64_t N_batch;             /* kept for API parity; paths per launch cap (0 = no cap) */
+    int32_t pilot_n;             /* default 10000 */
+    uint32_t seed;               /* default 123 */
+} fdmc_mc_config;
+
+typedef struct fdmc_qmc_config { /* QMCConfig */
+    int32_t R;                   /* replicas, default 32 */
+    int32_t max_doublings;       /* default 20 */
+    int64_t N0;                  /* first points per replica (power of 2), default 1<<14 */
+    int64_t N_max;               /* cap on points per replica (0 = 2^32) */
+    uint32_t salt_base;          /* replica salt = salt_base + r, default 42 */
+    int32_t reserved0;
+} fdmc_qmc_config;
+
+typedef struct fdmc_mlmc_config {/* MLMCConfig */
+    int32_t M;                   /* refinement factor, default 2 */
+    int32_t max_L;               /* default 10 */
+    int32_t pilot_n;             /* default 400 */
+    uint32_t seed;               /* PRNG key for run_mlmc, default 42 */
+} fdmc_mlmc_config;
+
+typedef enum { FDMC_NOISE_RAW = 0, FDMC_NOISE_BB = 1, FDMC_NOISE_PCA = 2 } fdmc_noise_mode;
+
+#define FDMC_FLAG_PRNG_FALLBACK 1u   /* QMC dimension > 20000: pseudo-random points were used */
+#define FDMC_FLAG_NOT_CONVERGED 2u   /* stopping rule not met within the configured limits */
+
+typedef struct fdmc_result {     /* MCResult (methods_cuda.cuh:41-46) + additive fields */
+    double price;
+    double std_error;
+    int64_t n_samples;
+    double time_s;
+    int64_t path_steps;          /* path-steps simulated (all levels, replicas) */
+    int32_t levels;              /* MLMC/MLQMC: finest level L + 1; QMC: doublings done */
+    uint32_t flags;
+} fdmc_result;
+
+typedef struct fdmc_cv_pilot_result { double beta, var_plain, var_cv; } fdmc_cv_pilot_result; /* CVPilot */
+
+typedef struct fdmc_context fdmc_context;
+typedef struct fdmc_bb fdmc_bb;    /* DeviceBBData  (methods_cuda.cuh:52) */
+typedef struct fdmc_pca fdmc_pca;  /* DevicePCAData (methods_cuda.cuh:53) */
+
+/* ---- library / context ---- */
+int32_t fdmc_abi_version(void);
+const char* fdmc_last_error(void);
+fdmc_status fdmc_context_create(int32_t device, fdmc_context** out);
+fdmc_status fdmc_context_destroy(fdmc_context* ctx);
+/* The context's HIP stream as an opaque handle (hipStream_t), for event timing. */
+fdmc_status fdmc_context_stream(fdmc_context* ctx, void** stream_out);
+fdmc_status fdmc_synchronize(fdmc_context* ctx);
+/* Kernel timing: when enabled, every fused path/level kernel launch is bracketed
+ * by HIP events on the context stream; read the count and summed milliseconds
+ * since the last reset (no counterpart in the reference; used by bench.py). */
+fdmc_status fdmc_timing_enable(fdmc_context* ctx, int32_t enable);
+fdmc_status fdmc_timing_read(fdmc_context* ctx, int64_t* launches, double* total_ms, int32_t reset);
+
+/* ---- Brownian bridge / PCA data (methods_cuda.cuh:56-63) ---- */
+/* bb_upload: tables as produced by bb_precompute (utils.cpp:54-101); N a power of 2
+ * with the dyadic coarse-to-fine schedule (checked). */
+fdmc_status fdmc_bb_upload(fdmc_context* ctx, int32_t N, double T,
+                           const int32_t* map_idx, const int32_t* left_idx,
+                           const int32_t* right_idx, const double* weight_left,
+                           const double* weight_right, const double* std_dev,
+                           fdmc_bb** out);
+fdmc_status fdmc_bb_free(fdmc_bb* bb);
+/* pca_upload: M (m x m, column-major, dW = M Z) as produced by pca_compute
+ * (utils.cpp:131-167).  Stored on the device in FP16 and BF16. */
+fdmc_status fdmc_pca_upload(fdmc_context* ctx, int32_t m, const double* M_colmajor,
+                            fdmc_pca** out);
+fdmc_status fdmc_pca_free(fdmc_pca* pca);
+
+/* ---- estimators ---- */
+/* run_mc_cuda (methods_cuda.cuh:71-72). */
+fdmc_status fdmc_run_mc(fdmc_context* ctx, const fdmc_model* model, const fdmc_payoff* payoff,
+                        double eps, int32_t n_steps, const fdmc_mc_config* cfg,
+                        fdmc_result* out);
+/* run_mc_fixed (methods_cuda.cuh:131-134): mean and variance of the mean. */
+fdmc_status fdmc_run_mc_fixed(fdmc_context* ctx, const fdmc_model* model,
+                              const fdmc_payoff* payoff, int32_t n_steps, int64_t n_paths,
+                              uint32_t seed, double* mean_out, double* var_of_mean_out);
+/* run_mlmc_cuda (methods_cuda.cuh:75-76). */
+fdmc_status fdmc_run_mlmc(fdmc_context* ctx, const fdmc_model* model, const fdmc_payoff* payoff,
+                          double eps, const fdmc_mlmc_config* cfg, fdmc_result* out);
+/* run_qmc_cuda (methods_cuda.cuh:82-87). */
+fdmc_status fdmc_run_qmc(fdmc_context* ctx, const fdmc_model* model, const fdmc_payoff* payoff,
+                         double eps, int32_t n_steps, const fdmc_qmc_config* cfg,
+                         int32_t mode, const fdmc_bb* bb, const fdmc_pca* pca,
+                         fdmc_result* out);
+/* run_mlqmc_cuda (methods_cuda.cuh:90-96). */
+fdmc_status fdmc_run_mlqmc(fdmc_context* ctx, const fdmc_model* model, const fdmc_payoff* payoff,
+                           double eps, const fdmc_mlmc_config* ml_cfg,
+                           const fdmc_qmc_config* qmc_cfg, int32_t mode,
+                           const fdmc_bb* const* bb_list, int32_t n_bb,
+                           const fdmc_pca* const* pca_list, int32_t n_pca,
+                           fdmc_result* out);
+/* cv_pilot (methods_cuda.cuh:103-108). */
+fdmc_status fdmc_cv_pilot(fdmc_context* ctx, const fdmc_model* main_model,
+                          const fdmc_model* ctrl_model, const fdmc_payoff* main_payoff,
+                          const fdmc_payoff* ctrl_payoff, double E_ctrl, int32_t n_steps,
+                          int32_t N_pilot, uint32_t seed, fdmc_cv_pilot_result* out);
+/* run_mc_cv_cuda (methods_cuda.cuh:110-116). */
+fdmc_status fdmc_run_mc_cv(fdmc_context* ctx, const fdmc_model* main_model,
+                           const fdmc_model* ctrl_model, const fdmc_payoff* main_payoff,
+                           const fdmc_payoff* ctrl_payoff, double E_ctrl, double beta,
+                           double eps, int32_t n_steps, const fdmc_mc_config* cfg,
+                           fdmc_result* out);
+/* run_qmc_cv_cuda (methods_cuda.cuh:118-124); mode/bb are additive (the reference
+ * is Raw-only, methods_cuda.cu:2241-2242). */
+fdmc_status fdmc_run_qmc_cv(fdmc_context* ctx, const fdmc_model* main_model,
+                            const fdmc_model* ctrl_model, const fdmc_payoff* main_payoff,
+                            const fdmc_payoff* ctrl_payoff, double E_ctrl, double beta,
+                            double eps, int32_t n_steps, const fdmc_qmc_config* cfg,
+                            int32_t mode, const fdmc_bb* bb, fdmc_result* out);
+
+/* One fixed QMC pass (the body of accumulate_range, methods_cuda.cu:1549-1622) for
+ * R replicas over points [offset, offset + n_points) of each replica's scrambled
+ * sequence (salt = salt_base + r).  sums_out: R x n_acc doubles per replica:
+ *   n_acc = 2 {sum Y, sum Y^2} without a control, or 7 with one
+ *   {Y, Y^2, Ycv, Ycv^2, Yc, Yc^2, Y*Yc} (methods_cuda.cu:628-639).
+ * ctrl_model/ctrl_payoff may be NULL (no control variate). */
+fdmc_status fdmc_qmc_accumulate(fdmc_context* ctx, const fdmc_model* model,
+                                const fdmc_payoff* payoff, const fdmc_model* ctrl_model,
+                                const fdmc_payoff* ctrl_payoff, double E_ctrl, double beta,
+                                int32_t n_steps, int32_t R, uint32_t salt_base,
+                                int64_t offset, int64_t n_points, int32_t mode,
+                                const fdmc_bb* bb, const fdmc_pca* pca, double* sums_out,
+                                int32_t* n_acc_out);
+
+/* ---- scrambled Sobol generator (methods_cuda.cu:753-844) ----
+ * d_out is [D][n] (dimension-major, as kernel_gen_hh_scrambled_sobol_normal writes).
+ * Point p of dimension d is the Hong-Hickernell-scrambled Gray-code Sobol point
+ * with index offset + p, replica salt `salt`.  offset + n must fit in 32 bits. */
+fdmc_status fdmc_sobol_scrambled_u32(fdmc_context* ctx, uint32_t salt, int32_t D,
+                                     int64_t n, int64_t offset, uint32_t* d_out);
+/* Same points mapped to N(0,1) (u = (s + 1/2) 2^-32, own FP32 inverse CDF). */
+fdmc_status fdmc_sobol_normal(fdmc_context* ctx, uint32_t salt, int32_t D, int64_t n,
+                              int64_t offset, float* d_out);
+/* Map n uint32 values through the device inverse CDF (test hook for the ICDF). */
+fdmc_status fdmc_icdf_u32(fdmc_context* ctx, const uint32_t* d_in, int64_t n, float* d_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDMC_H */

@@ -1,0 +1,87 @@
+"""C ABI surface (CPU only): the library loads, exports every symbol that
+include/fdmc.h declares, and rejects bad arguments on the host before any
+device work (no GPU compute is issued by these tests)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import fdmc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fdmc.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fdmc_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_reference_entry_points():
+    syms = declared_symbols()
+    for s in ("fdmc_run_mc", "fdmc_run_qmc", "fdmc_run_mlmc", "fdmc_run_mlqmc", "fdmc_bb_upload",
+              "fdmc_pca_upload", "fdmc_cv_pilot", "fdmc_run_mc_cv", "fdmc_run_qmc_cv", "fdmc_run_mc_fixed"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = fdmc.load_library()
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+    assert lib.fdmc_abi_version() == 1
+
+
+def test_ctypes_struct_layout_matches_header():
+    # fdmc_model: 2 ints + 15 doubles + L[4] + 4 doubles + 2 ints + 3 pointers
+    assert C.sizeof(fdmc.CModel) == 8 + 8 * (4 + 5 + 4 + 3 + 4) + 8 + 3 * 8
+    assert C.sizeof(fdmc.CPayoff) == 8 + 5 * 8 + 8
+    assert C.sizeof(fdmc.CResult) == 8 * 5 + 8
+    assert C.sizeof(fdmc.CQMCConfig) == 32
+
+
+def test_null_context_is_rejected_on_host():
+    lib = fdmc.load_library()
+    m, p, c, r = fdmc.GBMParams().to_c(), fdmc.European(100, 0.05, 1).to_c(), fdmc.QMCConfig().to_c(), fdmc.CResult()
+    rc = lib.fdmc_run_qmc(None, C.byref(m), C.byref(p), 1e-3, 64, C.byref(c), 0, None, None, C.byref(r))
+    assert rc == 1
+    assert b"null" in lib.fdmc_last_error()
+
+
+def test_context_create_without_device_reports_device_error():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    lib = fdmc.load_library()
+    h = C.c_void_p()
+    rc = lib.fdmc_context_create(0, C.byref(h))
+    assert rc == 4 and not h.value
+    assert lib.fdmc_last_error()
+
+
+def test_bb_upload_rejects_non_dyadic_tables():
+    lib = fdmc.load_library()
+    # null context is checked first; a bad table must also be rejected without a device
+    bb = fdmc.bb_precompute(8, 1.0)
+    N = 8
+    ia = lambda v: (C.c_int32 * N)(*v)
+    da = lambda v: (C.c_double * N)(*v)
+    h = C.c_void_p()
+    rc = lib.fdmc_bb_upload(None, N, 1.0, ia(bb.map_idx), ia(bb.left_idx), ia(bb.right_idx),
+                            da(bb.weight_left), da(bb.weight_right), da(bb.std_dev), C.byref(h))
+    assert rc == 1
+
+
+def test_python_bb_precompute_matches_oracle():
+    import numpy as np
+    import pyoracle as O
+    for N in (1, 2, 16, 256):
+        a = fdmc.bb_precompute(N, 1.3)
+        b = O.OracleBB(N, 1.3).arrays()
+        assert list(b["map_idx"]) == a.map_idx and list(b["left_idx"]) == a.left_idx
+        assert np.allclose(b["sd"], a.std_dev, rtol=0, atol=1e-15)
+        assert np.allclose(b["wl"], a.weight_left, rtol=0, atol=1e-15)
