@@ -31,7 +31,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(target_s=12.0):
+def cpu_baseline(target_s=10.0):
     """Oracle (FP32 paths, FP64 ICDF) on all host threads, same workload, fewer points."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import fdmc
@@ -41,11 +41,15 @@ def cpu_baseline(target_s=12.0):
     a, g = fdmc.Asian(100.0), fdmc.GeomAsian(100.0)
     E = fdmc.geom_asian_analytic(100, 100, 1, 0.05, 0.3, 256)
     bb = O.OracleBB(256, 1.0)
-    R, n = 32, 64
-    t = time.perf_counter()
-    O.qmc_accumulate(m, a, 256, R, 42, 0, n, fdmc.NOISE_BB, bb=bb, ctrl_model=m, ctrl_payoff=g, E_ctrl=E, beta=1.0)
-    dt = time.perf_counter() - t
-    n2 = max(64, int(n * target_s / max(dt, 1e-3)) // 64 * 64)
+    R = 32
+    dts = []
+    for n in (256, 1024):   # two sizes: the per-call direction-table set-up cancels in the slope
+        t = time.perf_counter()
+        O.qmc_accumulate(m, a, 256, R, 42, 0, n, fdmc.NOISE_BB, bb=bb, ctrl_model=m, ctrl_payoff=g, E_ctrl=E,
+                         beta=1.0)
+        dts.append(time.perf_counter() - t)
+    per_point = max(1e-9, (dts[1] - dts[0]) / 768)
+    n2 = max(1024, int(target_s / per_point) // 64 * 64)
     t = time.perf_counter()
     O.qmc_accumulate(m, a, 256, R, 42, 0, n2, fdmc.NOISE_BB, bb=bb, ctrl_model=m, ctrl_payoff=g, E_ctrl=E, beta=1.0)
     dt = time.perf_counter() - t

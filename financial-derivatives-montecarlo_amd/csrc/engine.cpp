@@ -268,10 +268,10 @@ const uint32_t* get_sdir(fdmc_context* c, const std::vector<uint32_t>& salts, in
 
 // ---------------------------------------------------------------- one fused pass
 struct PassSpec {
-    const fdmc_model* model;
-    const fdmc_payoff* payoff;
-    const fdmc_model* cmodel;     // control leg or null
-    const fdmc_payoff* cpayoff;
+    const fdmc_model* model = nullptr;
+    const fdmc_payoff* payoff = nullptr;
+    const fdmc_model* cmodel = nullptr;     // control leg or null
+    const fdmc_payoff* cpayoff = nullptr;
     double E_ctrl = 0.0, beta = 0.0;
     int n_steps = 0;
     int R = 1;

@@ -8,6 +8,16 @@ sys.path.insert(0, os.path.join(ROOT, "financial-derivatives-montecarlo_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
+# torch and libfdmc.so both link libamdhip64.so.7; load torch (and its HIP
+# runtime) first so that a process using both has a single runtime.
+try:
+    import torch
+
+    _HAS_GPU = torch.cuda.is_available()
+except ImportError:  # pragma: no cover
+    torch = None
+    _HAS_GPU = False
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libfdmc.so")
@@ -24,7 +34,5 @@ def engine():
 
 @pytest.fixture(scope="session")
 def torch_cuda():
-    import torch
-
-    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    assert torch is not None and _HAS_GPU, "GPU tests need torch with a HIP device"
     return torch
