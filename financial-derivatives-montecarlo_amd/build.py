@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "lib", "libfdmc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["engine.cpp", "k_sobol.hip", "k_path_raw.hip", "k_path_bb.hip", "k_path_prng.hip"]
+SOURCES = ["engine.cpp", "k_sobol.hip", "k_path_raw.hip", "k_path_bb.hip", "k_path_prng.hip",
+           "k_levels_raw.hip", "k_levels_bb.hip", "k_levels_prng.hip"]
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -455,6 +456,361 @@ void qmc_loop(fdmc_context* c, PassSpec s, double eps, const fdmc_qmc_config& cf
     out->time_s = now_s() - t0;
 }
 
+// ---------------------------------------------------------------- multilevel
+int level_steps(int M, int l) {
+    int64_t n = 1;
+    for (int i = 0; i < l; i++) {
+        n *= M;
+        if (n > (1LL << 30)) fail(FDMC_E_RANGE, "M^l exceeds 2^30 fine steps");
+    }
+    return (int)n;
+}
+
+// One (level, replica) request of a multilevel iteration.
+struct LevelReq {
+    int level = 0, replica = 0;
+    int64_t first = 0, count = 0;
+};
+
+struct MultiSpec {
+    const fdmc_model* model = nullptr;
+    const fdmc_payoff* payoff = nullptr;
+    int M = 2;
+    bool qmc = false;                 // Sobol (MLQMC) or Philox (MLMC)
+    int mode = FDMC_NOISE_RAW;        // MLQMC: RAW or BB
+    int R = 1;                        // replicas per level (MLQMC)
+    uint32_t salt_base = 42;          // MLQMC salt = salt_base + l R + r; MLMC: PRNG seed
+    const fdmc_bb* const* bb_list = nullptr;
+    int n_bb = 0;
+};
+
+// Scrambled records of all R replicas of level l (cached per context for the
+// lifetime of one multilevel call through `cache`).
+const uint32_t* level_sdir(fdmc_context* c, std::map<int, DevBuf>& cache, const MultiSpec& ms, int l, int D) {
+    auto it = cache.find(l);
+    if (it != cache.end()) return (const uint32_t*)it->second.p;
+    DevBuf& b = cache[l];
+    b.ensure((size_t)ms.R * D * SDIR_WORDS * sizeof(uint32_t));
+    std::vector<uint32_t> salts((size_t)ms.R);
+    for (int r = 0; r < ms.R; r++) salts[r] = ms.salt_base + (uint32_t)(l * ms.R + r);
+    c->salts.ensure(salts.size() * sizeof(uint32_t));
+    HIPC(hipMemcpyAsync(c->salts.p, salts.data(), salts.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                        c->stream));
+    HIPC(launch_build_sdir(c->d_dirv, (const uint32_t*)c->salts.p, ms.R, D, (uint32_t*)b.p, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));  // the salts buffer is reused by the next level
+    c->sdir_D = -1;
+    return (const uint32_t*)b.p;
+}
+
+// Runs every request in one persistent launch per noise kind; out[i] receives
+// {sum dY, sum dY^2, sum Yf, sum Yf^2} of request i.
+void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cache,
+                const std::vector<LevelReq>& reqs, std::vector<std::array<double, 4>>& out) {
+    const fdmc_model& m = *ms.model;
+    const fdmc_payoff& p = *ms.payoff;
+    const int nf = noise_dim(m);
+    const int mk = model_kind_dev(m), pk = payoff_kind_dev(p);
+    out.assign(reqs.size(), {0.0, 0.0, 0.0, 0.0});
+    // split by noise kind: Sobol (D <= 20000) and pseudo-random (MLMC or QMC fallback)
+    for (int pass = 0; pass < 2; pass++) {
+        std::vector<LevelEntry> ent;
+        std::vector<int> which;
+        for (size_t i = 0; i < reqs.size(); i++) {
+            const LevelReq& q = reqs[i];
+            const int nfine = level_steps(ms.M, q.level);
+            const int64_t D64 = (int64_t)nfine * nf;
+            const bool prng = !ms.qmc || D64 > D_MAX_SOBOL;
+            if ((pass == 0) == prng) continue;
+            if (q.count <= 0) continue;
+            LevelEntry e{};
+            e.level = q.level;
+            e.n_fine = nfine;
+            e.n_coarse = q.level == 0 ? 0 : nfine / ms.M;
+            e.D = (int)D64;
+            e.first = q.first;
+            e.count = q.count;
+            const double T = m.T;
+            e.h_f = (float)(T / nfine);
+            e.h_c = (float)(T / std::max(e.n_coarse, 1));
+            e.sqrt_hf = std::sqrt(e.h_f);
+            e.em_f = std::exp(-(float)m.kappa * e.h_f);
+            e.em_c = std::exp(-(float)m.kappa * e.h_c);
+            e.bgk_f = bgk_factor(p, e.h_f);
+            e.bgk_c = bgk_factor(p, e.h_c);
+            if (prng) {
+                e.seed = ms.qmc ? ms.salt_base + (uint32_t)(q.level * ms.R + q.replica) : ms.salt_base;
+                e.tag = ms.qmc ? TAG_QMC_FALLBACK : TAG_MLMC + (uint32_t)q.level;
+            } else {
+                if (q.first + q.count > (int64_t)4294967296LL)
+                    fail(FDMC_E_RANGE, "MLQMC Sobol offset beyond 32 bits (methods_cuda.cu:1847-1851)");
+                e.sdir = level_sdir(c, cache, ms, q.level, e.D) + (size_t)q.replica * e.D * SDIR_WORDS;
+                if (ms.mode == FDMC_NOISE_BB) {
+                    if (q.level >= ms.n_bb || !ms.bb_list[q.level])
+                        fail(FDMC_E_INVALID, "missing bb_list[" + std::to_string(q.level) + "]");
+                    const fdmc_bb* b = ms.bb_list[q.level];
+                    if (b->N != nfine)
+                        fail(FDMC_E_INVALID, "bb_list[" + std::to_string(q.level) + "] has N=" +
+                                                 std::to_string(b->N) + ", expected " + std::to_string(nfine));
+                    if (nfine > SOBOL_LDS_MAX_D) fail(FDMC_E_UNSUPPORTED, "BB supports at most 1024 steps");
+                    e.bbw = b->d_w;
+                    e.G = nfine >= 16 ? nfine / 16 : nfine;
+                }
+            }
+            ent.push_back(e);
+            which.push_back((int)i);
+        }
+        if (ent.empty()) continue;
+        // longest paths first
+        std::vector<int> order(ent.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ent[x].D > ent[y].D; });
+        std::vector<LevelEntry> sorted;
+        std::vector<int> sw;
+        int total = 0, max_D = 0, max_G = 0;
+        for (int o : order) {
+            LevelEntry e = ent[o];
+            const int64_t t0 = e.first >> 8, t1 = (e.first + e.count - 1) >> 8;
+            if (t1 - t0 + 1 > (1LL << 30) || total + (t1 - t0 + 1) > 0x7FFFFFFF)
+                fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
+            e.tile_begin = total;
+            e.ntiles = (int)(t1 - t0 + 1);
+            total += e.ntiles;
+            max_D = std::max(max_D, e.D);
+            max_G = std::max(max_G, e.G);
+            sorted.push_back(e);
+            sw.push_back(which[o]);
+        }
+        const int nz = pass == 1 ? NZ_PRNG : (ms.mode == FDMC_NOISE_BB ? NZ_SOBOL_BB : NZ_SOBOL_RAW);
+        if (nz == NZ_SOBOL_BB && nf != 1) fail(FDMC_E_UNSUPPORTED, "BB needs a one-factor model");
+        const void* fn = nz == NZ_PRNG ? levels_kernel_prng(mk, pk)
+                         : nz == NZ_SOBOL_BB ? levels_kernel_bb(mk, pk) : levels_kernel_raw(mk, pk);
+        if (!fn) fail(FDMC_E_UNSUPPORTED, "model/payoff combination not supported by the multilevel kernel");
+
+        const size_t ent_bytes = sorted.size() * sizeof(LevelEntry);
+        const size_t part_bytes = (size_t)total * 4 * sizeof(double);
+        const size_t sums_bytes = sorted.size() * 4 * sizeof(double);
+        char* scratch = (char*)c->partials.ensure(part_bytes + ent_bytes + sums_bytes + 256);
+        double* d_part = (double*)scratch;
+        LevelEntry* d_ent = (LevelEntry*)(scratch + ((part_bytes + 63) & ~(size_t)63));
+        double* d_out = (double*)((char*)d_ent + ((ent_bytes + 63) & ~(size_t)63));
+        unsigned int* d_queue = (unsigned int*)c->sums.ensure(64);
+        HIPC(hipMemcpyAsync(d_ent, sorted.data(), ent_bytes, hipMemcpyHostToDevice, c->stream));
+        HIPC(launch_zero_u32(d_queue, 1, c->stream));
+
+        LevelsLaunch a{};
+        a.leg = make_leg(m, p);
+        a.M = ms.M;
+        a.nf = nf;
+        a.n_entries = (int)sorted.size();
+        a.total_items = total;
+        a.entries = d_ent;
+        a.queue = d_queue;
+        a.partials = d_part;
+        const size_t lds = levels_lds_bytes(nz, max_D, max_G);
+        if (lds > 160 * 1024) fail(FDMC_E_UNSUPPORTED, "multilevel LDS footprint too large");
+        if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const int grid = (int)std::min<int64_t>(total, (int64_t)c->n_cu * 8);
+        void* args[] = {(void*)&a, (void*)&max_D, (void*)&max_G};
+        if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
+        HIPC(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, c->stream));
+        if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
+        HIPC(launch_reduce_entries(d_part, d_ent, a.n_entries, 4, d_out, c->stream));
+        ensure_host_sums(c, sorted.size() * 4);
+        HIPC(hipMemcpyAsync(c->h_sums, d_out, sums_bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        if (c->timing) {
+            float ms_ = 0.0f;
+            HIPC(hipEventElapsedTime(&ms_, c->ev0, c->ev1));
+            c->timed_ms += ms_;
+            c->timed_launches++;
+        }
+        for (size_t i = 0; i < sorted.size(); i++)
+            for (int k = 0; k < 4; k++) out[sw[i]][k] = c->h_sums[i * 4 + k];
+    }
+}
+
+// run_mlmc_cuda (methods_cuda.cu:1305-1487): pilot on levels 0..2, Giles
+// allocation N_l = ceil(2/eps^2 sqrt(V_l/C_l) sum sqrt(V_l C_l)), bias test on
+// the two finest levels, L++ with a pilot.  Each iteration's extra samples of
+// all levels run in one launch; level l's samples are Philox paths
+// (seed, 0x100 + l, index) continuing from N_l.
+void mlmc_loop(fdmc_context* c, const MultiSpec& ms, double eps, int max_L, int pilot_n, fdmc_result* out) {
+    const double t0 = now_s();
+    const int M = ms.M;
+    if (M < 2) fail(FDMC_E_INVALID, "MLMCConfig.M must be >= 2");
+    if (max_L < 2 || max_L > 30) fail(FDMC_E_INVALID, "MLMCConfig.max_L must be in [2, 30]");
+    if (pilot_n <= 1) fail(FDMC_E_INVALID, "pilot_n must be > 1");
+    if (!(eps > 0.0)) fail(FDMC_E_INVALID, "eps must be positive");
+    std::map<int, DevBuf> cache;
+    std::vector<double> E(max_L + 1, 0.0), V(max_L + 1, 0.0);
+    std::vector<int64_t> N(max_L + 1, 0);
+    *out = fdmc_result{};
+    std::vector<std::array<double, 4>> s;
+    auto steps = [&](int l, int64_t n) { out->path_steps += n * (int64_t)level_steps(M, l); };
+    int L = 2;
+    {
+        std::vector<LevelReq> rq;
+        for (int l = 0; l <= L; l++) rq.push_back({l, 0, 0, pilot_n});
+        run_levels(c, ms, cache, rq, s);
+        for (int l = 0; l <= L; l++) {
+            const double em = s[l][0] / pilot_n;
+            E[l] = em;
+            V[l] = std::max(0.0, s[l][1] / pilot_n - em * em);
+            N[l] = pilot_n;
+            steps(l, pilot_n);
+        }
+    }
+    bool conv = false;
+    int iter = 0;
+    while (!conv && L <= max_L && iter++ < 50) {
+        double st = 0.0;
+        for (int l = 0; l <= L; l++) st += std::sqrt(V[l] * std::pow((double)M, l));
+        std::vector<LevelReq> rq;
+        for (int l = 0; l <= L; l++) {
+            const double Cl = std::pow((double)M, l);
+            int64_t Nopt = (int64_t)std::ceil(2.0 / (eps * eps) * std::sqrt(V[l] / Cl) * st);
+            Nopt = std::max<int64_t>(Nopt, 100);
+            if (Nopt > N[l]) rq.push_back({l, 0, N[l], Nopt - N[l]});
+        }
+        if (!rq.empty()) {
+            run_levels(c, ms, cache, rq, s);
+            for (size_t i = 0; i < rq.size(); i++) {
+                const int l = rq[i].level;
+                const int64_t ex = rq[i].count, Nn = N[l] + ex;
+                const double ts = E[l] * N[l] + s[i][0];
+                const double t2 = (V[l] + E[l] * E[l]) * N[l] + s[i][1];
+                E[l] = ts / Nn;
+                V[l] = std::max(0.0, t2 / Nn - E[l] * E[l]);
+                N[l] = Nn;
+                steps(l, ex);
+            }
+        }
+        double bias = std::fabs(E[L]) / (M - 1);
+        if (L >= 1) bias = std::max(bias, std::fabs(E[L - 1]) * M / (double)(M * M - M));
+        conv = bias < eps / std::sqrt(2.0);
+        if (!conv && L < max_L) {
+            L++;
+            std::vector<LevelReq> one{{L, 0, 0, pilot_n}};
+            run_levels(c, ms, cache, one, s);
+            const double em = s[0][0] / pilot_n;
+            E[L] = em;
+            V[L] = std::max(0.0, s[0][1] / pilot_n - em * em);
+            N[L] = pilot_n;
+            steps(L, pilot_n);
+        }
+    }
+    double price = 0.0, vs = 0.0;
+    int64_t tot = 0;
+    for (int l = 0; l <= L; l++) {
+        price += E[l];
+        vs += N[l] > 0 ? V[l] / N[l] : 0.0;
+        tot += N[l];
+    }
+    for (auto& kv : cache) kv.second.release();
+    out->price = price;
+    out->std_error = std::sqrt(vs);
+    out->n_samples = tot;
+    out->levels = L + 1;
+    out->flags = conv ? 0u : FDMC_FLAG_NOT_CONVERGED;
+    out->time_s = now_s() - t0;
+}
+
+// run_mlqmc_cuda (methods_cuda.cu:1680-2019): per level the estimator is the
+// mean of R replica means (salt = salt_base + l R + r, Sobol offsets continuing
+// per (l, r)), sig2_l = S^2(replica means) N_l, N_opt carries 1/R.
+void mlqmc_loop(fdmc_context* c, const MultiSpec& ms, double eps, int max_L, int pilot_n, fdmc_result* out) {
+    const double t0 = now_s();
+    const int M = ms.M, R = ms.R;
+    if (M < 2) fail(FDMC_E_INVALID, "MLMCConfig.M must be >= 2");
+    if (max_L < 2 || max_L > 30) fail(FDMC_E_INVALID, "MLMCConfig.max_L must be in [2, 30]");
+    if (pilot_n <= 1) fail(FDMC_E_INVALID, "pilot_n must be > 1");
+    if (R <= 0) fail(FDMC_E_INVALID, "QMCConfig.R must be positive");
+    if (!(eps > 0.0)) fail(FDMC_E_INVALID, "eps must be positive");
+    std::map<int, DevBuf> cache;
+    std::vector<double> E(max_L + 1, 0.0), S2(max_L + 1, 0.0);
+    std::vector<int64_t> N(max_L + 1, 0), off(max_L + 1, 0);
+    *out = fdmc_result{};
+    std::vector<std::array<double, 4>> s;
+    // launch R requests for each (level, n) and return (mean, sample var) of replica means per level
+    auto run = [&](const std::vector<std::pair<int, int64_t>>& lv, std::vector<std::pair<double, double>>& mv) {
+        std::vector<LevelReq> rq;
+        for (auto& x : lv)
+            for (int r = 0; r < R; r++) rq.push_back({x.first, r, off[x.first], x.second});
+        run_levels(c, ms, cache, rq, s);
+        mv.clear();
+        for (size_t k = 0; k < lv.size(); k++) {
+            const int l = lv[k].first;
+            const int64_t n = lv[k].second;
+            std::vector<double> rm((size_t)R);
+            for (int r = 0; r < R; r++) rm[r] = s[k * R + r][0] / (double)n;
+            double m_, v_;
+            mean_var(rm, m_, v_);
+            mv.push_back({m_, v_});
+            off[l] += n;
+            out->path_steps += (int64_t)R * n * level_steps(M, l);
+        }
+    };
+    int L = 2;
+    std::vector<std::pair<double, double>> mv;
+    {
+        std::vector<std::pair<int, int64_t>> lv;
+        for (int l = 0; l <= L; l++) lv.push_back({l, pilot_n});
+        run(lv, mv);
+        for (int l = 0; l <= L; l++) {
+            E[l] = mv[l].first;
+            S2[l] = mv[l].second * (double)pilot_n;
+            N[l] = pilot_n;
+        }
+    }
+    bool conv = false;
+    int iter = 0;
+    while (!conv && L <= max_L && iter++ < 50) {
+        double st = 0.0;
+        for (int l = 0; l <= L; l++) st += std::sqrt(S2[l] * std::pow((double)M, l));
+        std::vector<std::pair<int, int64_t>> lv;
+        for (int l = 0; l <= L; l++) {
+            const double Cl = std::pow((double)M, l);
+            int64_t Nopt = (int64_t)std::ceil(2.0 / ((double)R * eps * eps) * std::sqrt(S2[l] / Cl) * st);
+            Nopt = std::max<int64_t>(Nopt, 100);
+            if (Nopt > N[l]) lv.push_back({l, Nopt - N[l]});
+        }
+        if (!lv.empty()) {
+            run(lv, mv);
+            for (size_t k = 0; k < lv.size(); k++) {
+                const int l = lv[k].first;
+                const double wo = (double)N[l], wn = (double)lv[k].second;
+                E[l] = (E[l] * wo + mv[k].first * wn) / (wo + wn);
+                S2[l] = (S2[l] * wo + mv[k].second * wn * wn) / (wo + wn);
+                N[l] += lv[k].second;
+            }
+        }
+        double bias = std::fabs(E[L]) / std::max(M - 1, 1);
+        if (L >= 1) bias = std::max(bias, std::fabs(E[L - 1]) * M / (double)std::max(M * (M - 1), 1));
+        conv = bias < eps / std::sqrt(2.0);
+        if (!conv && L < max_L) {
+            L++;
+            run({{L, pilot_n}}, mv);
+            E[L] = mv[0].first;
+            S2[L] = mv[0].second * (double)pilot_n;
+            N[L] = pilot_n;
+        }
+    }
+    double price = 0.0, vs = 0.0;
+    int64_t tot = 0;
+    for (int l = 0; l <= L; l++) {
+        price += E[l];
+        vs += S2[l] / ((double)N[l] * (double)R);
+        tot += N[l] * R;
+    }
+    for (auto& kv : cache) kv.second.release();
+    out->price = price;
+    out->std_error = std::sqrt(vs);
+    out->n_samples = tot;
+    out->levels = L + 1;
+    out->flags = conv ? 0u : FDMC_FLAG_NOT_CONVERGED;
+    out->time_s = now_s() - t0;
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -790,8 +1146,46 @@ fdmc_status fdmc_run_mc_cv(fdmc_context* c, const fdmc_model* mm, const fdmc_mod
 fdmc_status fdmc_run_mlmc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double eps,
                           const fdmc_mlmc_config* cfg, fdmc_result* out) {
     return guarded([&] {
-        (void)c; (void)model; (void)payoff; (void)eps; (void)cfg; (void)out;
-        fail(FDMC_E_UNSUPPORTED, "run_mlmc: multilevel kernel not built yet");
+        if (!c || !cfg || !out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (is_basket(*model))  // methods_cuda.cu:1318-1320
+            fail(FDMC_E_UNSUPPORTED, "run_mlmc: basket models are not supported (no multi-asset MLMC kernel)");
+        MultiSpec ms;
+        ms.model = model; ms.payoff = payoff; ms.M = cfg->M; ms.qmc = false; ms.salt_base = cfg->seed;
+        mlmc_loop(c, ms, eps, cfg->max_L, cfg->pilot_n, out);
+    });
+}
+
+// One coupled-level pass (test and bench hook): request i covers samples
+// [first_i, first_i + count_i) of level level_i (replica replica_i for Sobol).
+fdmc_status fdmc_levels_accumulate(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff,
+                                   int32_t M, int32_t qmc, int32_t mode, int32_t R, uint32_t salt_or_seed,
+                                   int32_t n_req, const int32_t* levels, const int32_t* replicas,
+                                   const int64_t* first, const int64_t* count,
+                                   const fdmc_bb* const* bb_list, int32_t n_bb, double* sums_out) {
+    return guarded([&] {
+        if (!c || !levels || !first || !count || !sums_out || n_req <= 0) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (is_basket(*model)) fail(FDMC_E_UNSUPPORTED, "basket models are not supported by the multilevel kernel");
+        MultiSpec ms;
+        ms.model = model; ms.payoff = payoff; ms.M = M; ms.qmc = qmc != 0; ms.mode = mode; ms.R = std::max(R, 1);
+        ms.salt_base = salt_or_seed; ms.bb_list = bb_list; ms.n_bb = n_bb;
+        if (M < 2) fail(FDMC_E_INVALID, "M must be >= 2");
+        std::vector<LevelReq> rq;
+        for (int i = 0; i < n_req; i++) {
+            if (levels[i] < 0 || count[i] <= 0 || first[i] < 0) fail(FDMC_E_INVALID, "bad level request");
+            const int r = replicas ? replicas[i] : 0;
+            if (r < 0 || r >= ms.R) fail(FDMC_E_INVALID, "replica out of range");
+            rq.push_back({levels[i], r, first[i], count[i]});
+        }
+        std::map<int, DevBuf> cache;
+        std::vector<std::array<double, 4>> s;
+        run_levels(c, ms, cache, rq, s);
+        for (auto& kv : cache) kv.second.release();
+        for (int i = 0; i < n_req; i++)
+            for (int k = 0; k < 4; k++) sums_out[i * 4 + k] = s[i][k];
     });
 }
 
@@ -800,9 +1194,21 @@ fdmc_status fdmc_run_mlqmc(fdmc_context* c, const fdmc_model* model, const fdmc_
                            const fdmc_bb* const* bbl, int32_t n_bb, const fdmc_pca* const* pcal, int32_t n_pca,
                            fdmc_result* out) {
     return guarded([&] {
-        (void)c; (void)model; (void)payoff; (void)eps; (void)ml; (void)qc; (void)mode; (void)bbl;
-        (void)n_bb; (void)pcal; (void)n_pca; (void)out;
-        fail(FDMC_E_UNSUPPORTED, "run_mlqmc: multilevel kernel not built yet");
+        if (!c || !ml || !qc || !out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (is_basket(*model))  // methods_cuda.cu:1695-1697
+            fail(FDMC_E_UNSUPPORTED, "run_mlqmc: basket models are not supported");
+        if (mode != FDMC_NOISE_RAW && noise_dim(*model) != 1)  // methods_cuda.cu:1702-1704
+            fail(FDMC_E_UNSUPPORTED, "run_mlqmc: Brownian bridge/PCA need a one-factor model");
+        if (mode == FDMC_NOISE_PCA) {
+            (void)pcal; (void)n_pca;
+            fail(FDMC_E_UNSUPPORTED, "run_mlqmc: PCA noise mode is not available yet");
+        }
+        MultiSpec ms;
+        ms.model = model; ms.payoff = payoff; ms.M = ml->M; ms.qmc = true; ms.mode = mode; ms.R = qc->R;
+        ms.salt_base = qc->salt_base; ms.bb_list = bbl; ms.n_bb = n_bb;
+        mlqmc_loop(c, ms, eps, ml->max_L, ml->pilot_n, out);
     });
 }
 

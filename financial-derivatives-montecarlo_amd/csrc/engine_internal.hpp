@@ -18,6 +18,15 @@ hipError_t launch_icdf(const uint32_t* in, int64_t n, float* out, hipStream_t st
 hipError_t launch_reduce_partials(const double* part, int n_slots, int nw, int nacc, double* out,
                                   hipStream_t st);
 
+hipError_t launch_reduce_entries(const double* part, const LevelEntry* ent, int n_entries, int nacc,
+                                 double* out, hipStream_t st);
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
+
+// k_levels_{raw,bb,prng}.hip: multilevel kernel for (model, payoff), or null.
+const void* levels_kernel_raw(int mk, int pk);
+const void* levels_kernel_bb(int mk, int pk);
+const void* levels_kernel_prng(int mk, int pk);
+
 // k_path_{raw,bb,prng}.hip: kernel for a (model, payoff, control) combination, or null.
 const void* path_kernel_raw(int mk, int pk, int cv, int cpk, int lb);
 const void* path_kernel_bb(int mk, int pk, int cv, int cpk, int lb);

@@ -1,0 +1,226 @@
+// k_levels.inl -- coupled fine/coarse multilevel kernel, one persistent launch
+// for every level (and replica) of an MLMC / MLQMC iteration.
+//
+// The reference launches one kernel per level on its own stream, each with its
+// own generator and buffers (methods_cuda.cu:1347-1395, 1762-1924).  Here a
+// grid sized to the device drains a queue of 256-sample tiles of all requested
+// (level, replica) entries, longest levels first; a workgroup takes the next
+// tile with one atomic.  The per-path arithmetic is kernel_mlmc
+// (methods_cuda.cu:445-518) with the fine path's running payoff updated for
+// every model (the reference skips it for Heston, :464-470).
+#pragma once
+
+#include "k_path.inl"
+
+namespace fdmc {
+
+template <int MK, int PK>
+struct Coupled {
+    float Sf, Vf, Sc, Vc, rf, rc, a1, a2;
+    int cnt, ck;
+    __device__ __forceinline__ void init(const LegParams& P) {
+        Sf = Sc = P.S0;
+        Vf = Vc = P.v0;
+        rf = rc = run_init<PK>(P);
+        a1 = a2 = 0.0f;
+        cnt = ck = 0;
+    }
+    // one-factor fine step with a scaled increment dw; coarse step every M.
+    __device__ __forceinline__ void fine1(const LegParams& P, const LevelEntry& E, float dw, int k, int M) {
+        a1 += dw;
+        Sf = step_1f<MK>(P, Sf, dw, (float)k * E.h_f, E.h_f);
+        if constexpr (is_path_dep<PK>()) run_update<PK>(rf, Sf);
+        if (++cnt == M) {
+            Sc = step_1f<MK>(P, Sc, a1, (float)ck * E.h_c, E.h_c);
+            if constexpr (is_path_dep<PK>()) run_update<PK>(rc, Sc);
+            a1 = 0.0f;
+            cnt = 0;
+            ++ck;
+        }
+    }
+    // Heston fine step from unscaled normals z1, z2.
+    __device__ __forceinline__ void fineH(const LegParams& P, const LevelEntry& E, float z1, float z2, int M) {
+        const float dw1 = z1 * E.sqrt_hf;
+        const float dw2 = (P.L10 * z1 + P.L11 * z2) * E.sqrt_hf;
+        a1 += dw1;
+        a2 += dw2;
+        step_heston(P, Sf, Vf, dw1, dw2, E.h_f, E.em_f);
+        if constexpr (is_path_dep<PK>()) run_update<PK>(rf, Sf);
+        if (++cnt == M) {
+            step_heston(P, Sc, Vc, a1, a2, E.h_c, E.em_c);
+            if constexpr (is_path_dep<PK>()) run_update<PK>(rc, Sc);
+            a1 = a2 = 0.0f;
+            cnt = 0;
+        }
+    }
+    __device__ __forceinline__ void finish(const LegParams& P, const LevelEntry& E, float& pf, float& pc) const {
+        pf = payoff<PK>(P, Sf, rf, E.n_fine, E.bgk_f);
+        pc = (E.n_coarse > 0) ? payoff<PK>(P, Sc, rc, E.n_coarse, E.bgk_c) : 0.0f;
+    }
+};
+
+template <int MK, int PK, int NZ, int LB>
+__global__ __launch_bounds__(256) void k_levels(LevelsLaunch a, int max_D, int max_G) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const bool lds_ok = (NZ != NZ_PRNG) && max_D <= SOBOL_LDS_MAX_D;
+    uint32_t* rec = (uint32_t*)smem;
+    const int rec_words = lds_ok ? max_D * 20 : 0;
+    float* skel = (float*)(smem + (size_t)rec_words * 4);
+    const int skel_words = (NZ == NZ_SOBOL_BB) ? (max_G + 1) * 256 : 0;
+    double* red = (double*)(smem + (((size_t)(rec_words + skel_words) * 4 + 15) & ~(size_t)15));
+    int* s_item = (int*)(red + 4 * 4);
+
+    SobolLane sl;
+    {
+        const uint32_t g = (uint32_t)(lane ^ (lane >> 1));
+        sl.rec = rec;
+        sl.grec = nullptr;
+        sl.ga = g & 7u;
+        sl.gb = g >> 3;
+        sl.w = (uint32_t)(tid >> 6);
+        sl.hi_gray = 0;
+        sl.lds_mode = lds_ok;
+    }
+    int last_e = -1;
+    const LegParams& P = a.leg;
+
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) *s_item = (int)atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const int item = *s_item;
+        if (item >= a.total_items) break;   // every workgroup leaves once the queue is empty
+        int lo = 0, hi = a.n_entries - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.entries[mid].tile_begin <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const LevelEntry E = a.entries[lo];
+        const int t = item - E.tile_begin;
+        const int64_t base = ((E.first >> 8) + t) << 8;
+        const int64_t idx = base + tid;
+        const bool valid = idx >= E.first && idx < E.first + E.count;
+        const int D = E.D;
+
+        if constexpr (NZ != NZ_PRNG) {
+            sl.grec = E.sdir;
+            const uint32_t b32 = (uint32_t)base;
+            const uint32_t gbase = b32 ^ (b32 >> 1);
+            if (lds_ok) {
+                if (lo != last_e) {
+                    for (int i = tid; i < D * 16; i += 256) {
+                        const int d = i >> 4, k = i & 15;
+                        rec[d * 20 + k] = E.sdir[(int64_t)d * SDIR_WORDS + k];
+                    }
+                }
+                for (int d = tid; d < D; d += 256) {
+                    const uint32_t* gr = E.sdir + (int64_t)d * SDIR_WORDS;
+                    uint32_t s = 0;
+                    for (uint32_t g = gbase; g; g &= g - 1u) s ^= gr[32 + __builtin_ctz(g)];
+                    rec[d * 20 + 16] = s ^ gr[16];
+                    rec[d * 20 + 17] = s ^ gr[17];
+                    rec[d * 20 + 18] = s ^ gr[18];
+                    rec[d * 20 + 19] = s ^ gr[19];
+                }
+                __syncthreads();
+            } else {
+                const uint32_t hi2 = b32 + (sl.w << 6);
+                sl.hi_gray = hi2 ^ (hi2 >> 1);
+            }
+        }
+        last_e = lo;
+
+        Coupled<MK, PK> cp;
+        cp.init(P);
+        const int M = a.M, nfine = E.n_fine;
+        if constexpr (NZ == NZ_PRNG) {
+            float z[4];
+            if constexpr (MK == MK_HESTON) {
+                for (int b = 0; 2 * b < nfine; b++) {
+                    prng_normals4(E.seed, E.tag, (uint64_t)idx, (uint32_t)b, z);
+                    cp.fineH(P, E, z[0], z[1], M);
+                    if (2 * b + 1 < nfine) cp.fineH(P, E, z[2], z[3], M);
+                }
+            } else {
+                for (int b = 0; 4 * b < nfine; b++) {
+                    prng_normals4(E.seed, E.tag, (uint64_t)idx, (uint32_t)b, z);
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        if (4 * b + c < nfine) cp.fine1(P, E, z[c] * E.sqrt_hf, 4 * b + c, M);
+                }
+            }
+        } else if constexpr (NZ == NZ_SOBOL_RAW) {
+            if constexpr (MK == MK_HESTON) {
+                for (int k = 0; k < nfine; k++)
+                    cp.fineH(P, E, icdf_u32(sobol_coord(sl, 2 * k)), icdf_u32(sobol_coord(sl, 2 * k + 1)), M);
+            } else {
+                for (int k = 0; k < nfine; k++) cp.fine1(P, E, icdf_u32(sobol_coord(sl, k)) * E.sqrt_hf, k, M);
+            }
+        } else {  // Brownian bridge over n_fine steps (one-factor models)
+            const float4* bw = (const float4*)E.bbw;
+            const int G = E.G;
+            float* sk = skel + tid;
+            sk[0] = 0.0f;
+            sk[G * 256] = bw[0].z * icdf_u32(sobol_coord(sl, 0));
+            for (int j = 1; j < G; j++) {
+                const int lev = 31 - __builtin_clz(j);
+                const int q = j - (1 << lev);
+                const int st = G >> lev;
+                const int L = q * st, R = L + st, Mi = L + (st >> 1);
+                const float4 c = bw[j];
+                sk[Mi * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32(sobol_coord(sl, j));
+            }
+            if (LB == 0 || nfine < 16) {
+                float left = 0.0f;
+                for (int i = 0; i < G; i++) {
+                    const float right = sk[(i + 1) * 256];
+                    cp.fine1(P, E, right - left, i, M);
+                    left = right;
+                }
+            } else {
+                constexpr int B = 1 << (LB > 0 ? LB : 1);
+                float left = 0.0f;
+                for (int i = 0; i < G; i++) {
+                    float W[B + 1];
+                    W[0] = left;
+                    W[B] = sk[(i + 1) * 256];
+#pragma unroll
+                    for (int l = 0; l < LB; l++) {
+                        const int j0 = (G << l) + (i << l);
+#pragma unroll
+                        for (int q = 0; q < (1 << l); q++) {
+                            const int s = B >> l;
+                            const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
+                            const float4 c = bw[j0 + q];
+                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32(sobol_coord(sl, j0 + q));
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < B; k++) cp.fine1(P, E, W[k + 1] - W[k], i * B + k, M);
+                    left = W[B];
+                }
+            }
+        }
+
+        float pf, pc;
+        cp.finish(P, E, pf, pc);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (valid) {
+            const double dy = (double)(pf - pc), yf = (double)pf;
+            acc[0] = dy;
+            acc[1] = dy * dy;
+            acc[2] = yf;
+            acc[3] = yf * yf;
+        }
+        block_sum<4, 4>(acc, red);
+        if (tid == 0) {
+            double* out = a.partials + (int64_t)item * 4;
+#pragma unroll
+            for (int i = 0; i < 4; i++) out[i] = acc[i];
+        }
+    }
+}
+
+}  // namespace fdmc

@@ -1,0 +1,17 @@
+// k_levels_prng.hip -- instantiations of the multilevel kernel for noise NZ_PRNG.
+#include "k_levels.inl"
+
+namespace fdmc {
+
+const void* levels_kernel_prng(int mk, int pk) {
+#define X(MKv, PKv) \
+    if (mk == MKv && pk == PKv) return (const void*)&k_levels<MKv, PKv, NZ_PRNG, 0>;
+#define XP(MKv) X(MKv, PK_EUROPEAN) X(MKv, PK_ASIAN) X(MKv, PK_GEOM_ASIAN) X(MKv, PK_LOOKBACK) X(MKv, PK_BARRIER)
+    XP(MK_GBM) XP(MK_DUPIRE) XP(MK_LVQ)
+    XP(MK_HESTON)
+#undef XP
+#undef X
+    return nullptr;
+}
+
+}  // namespace fdmc

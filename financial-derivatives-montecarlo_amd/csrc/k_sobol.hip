@@ -110,6 +110,28 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
     }
 }
 
+// Segmented variant: out[e][i] = sum_{t < len_e} part[begin_e + t][i], entries
+// taken from the multilevel entry table.
+__global__ __launch_bounds__(256) void k_reduce_entries(const double* __restrict__ part,
+                                                        const LevelEntry* __restrict__ ent, int nacc,
+                                                        double* __restrict__ out) {
+    __shared__ double lds[4 * 16];
+    const LevelEntry E = ent[blockIdx.x];
+    const double* p = part + (int64_t)E.tile_begin * nacc;
+    for (int i = 0; i < nacc; i++) {
+        double v[1] = {0.0};
+        for (int w = threadIdx.x; w < E.ntiles; w += 256) v[0] += p[(int64_t)w * nacc + i];
+        block_sum<1, 4>(v, lds);
+        if (threadIdx.x == 0) out[(int64_t)blockIdx.x * nacc + i] = v[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(unsigned int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 // ---- host launchers ----------------------------------------------------------
 hipError_t launch_build_sdir(const uint32_t* dirv, const uint32_t* d_salts, int n_slots, int D,
                              uint32_t* out, hipStream_t st) {
@@ -141,6 +163,17 @@ hipError_t launch_icdf(const uint32_t* in, int64_t n, float* out, hipStream_t st
 hipError_t launch_reduce_partials(const double* part, int n_slots, int nw, int nacc, double* out,
                                   hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_partials, dim3(n_slots), dim3(256), 0, st, part, nw, nacc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_entries(const double* part, const LevelEntry* ent, int n_entries, int nacc,
+                                 double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_entries, dim3(n_entries), dim3(256), 0, st, part, ent, nacc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

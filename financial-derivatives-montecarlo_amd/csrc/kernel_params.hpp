@@ -56,17 +56,26 @@ struct PathLaunch {
     double* partials;       // [R][nwg_x][NACC]
 };
 
-// Coupled multilevel launch: one work item = one 256-path tile of one level.
-struct LevelWork {
-    int level, n_fine, replica;
-    int64_t first;          // first sample index of the tile range
-    int64_t count;          // samples in this level/replica request
-    int tile0, ntiles;      // tile range of this entry within the queue
-    float h_f, h_c, sqrt_hf, em_f, em_c, bgk_f, bgk_c;
-    uint32_t seed, tag;
-    const uint32_t* sdir;   // per (level, replica) direction records or null (PRNG)
-    const float* bbw;       // BB weights for N = n_fine or null
-    int slot;               // output slot (level * R + replica)
+// Coupled multilevel launch (MLMC / MLQMC): one entry per (level, replica)
+// request; entries are cut into 256-sample tiles that a persistent grid drains
+// through an atomic counter.  Tile t of entry e writes its sums to
+// partials[tile_begin + t] so the result does not depend on scheduling.
+struct LevelEntry {
+    int level, n_fine, n_coarse, D, G;
+    int tile_begin, ntiles, pad0;
+    int64_t first, count;   // sample (PRNG path id / Sobol point index) range
+    float h_f, h_c, sqrt_hf, em_f, em_c, bgk_f, bgk_c, pad1;
+    uint32_t seed, tag;     // PRNG stream (MLMC)
+    const uint32_t* sdir;   // Sobol records of this (level, replica), or null
+    const float* bbw;       // BB weights for N = n_fine, or null
+};
+
+struct LevelsLaunch {
+    LegParams leg;
+    int M, nf, n_entries, total_items;
+    const LevelEntry* entries;  // device array, sorted by decreasing cost
+    unsigned int* queue;        // work counter (zeroed before the launch)
+    double* partials;           // [total_items][4]
 };
 
 // Sobol direction records are staged in LDS up to this dimension; above it the
@@ -80,6 +89,15 @@ inline size_t path_lds_bytes(int nz, int D, int G) {
     if (nz == NZ_SOBOL_BB) b += (size_t)(G + 1) * 256 * 4;
     b = (b + 15) & ~(size_t)15;
     return b + 4 * 7 * sizeof(double);
+}
+
+// Dynamic LDS bytes of one k_levels launch (max over its entries).
+inline size_t levels_lds_bytes(int nz, int max_D, int max_G) {
+    size_t b = 0;
+    if (nz != NZ_PRNG && max_D <= SOBOL_LDS_MAX_D) b += (size_t)max_D * 20 * 4;
+    if (nz == NZ_SOBOL_BB) b += (size_t)(max_G + 1) * 256 * 4;
+    b = (b + 15) & ~(size_t)15;
+    return b + 4 * 4 * sizeof(double) + 16;
 }
 
 }  // namespace fdmc

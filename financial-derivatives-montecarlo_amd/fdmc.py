@@ -324,6 +324,10 @@ _SIGS = {
                                       C.POINTER(CPayoff), C.c_double, C.c_double, C.c_int32, C.c_int32,
                                       C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P, _P,
                                       C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "fdmc_levels_accumulate": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_int32, C.c_int32,
+                                         C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                         C.POINTER(_P), C.c_int32, C.POINTER(C.c_double)]),
     "fdmc_sobol_scrambled_u32": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_sobol_normal": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_icdf_u32": (C.c_int, [_P, _P, C.c_int64, _P]),
@@ -517,6 +521,21 @@ class Engine:
                                                       dev_pca.handle if dev_pca else None, sums, C.byref(nacc)))
         na = nacc.value
         return [[sums[r * na + i] for i in range(na)] for r in range(R)]
+
+    def levels_accumulate(self, model, payoff, M, reqs, qmc=False, mode=NOISE_RAW, R=1, salt_or_seed=42,
+                          bb_list=()):
+        """reqs: list of (level, replica, first, count); returns n x 4 sums."""
+        n = len(reqs)
+        lv = (C.c_int32 * n)(*[q[0] for q in reqs])
+        rp = (C.c_int32 * n)(*[q[1] for q in reqs])
+        fi = (C.c_int64 * n)(*[q[2] for q in reqs])
+        co = (C.c_int64 * n)(*[q[3] for q in reqs])
+        bbs = (_P * max(1, len(bb_list)))(*[x.handle if x else None for x in bb_list])
+        out = (C.c_double * (4 * n))()
+        m, p = model.to_c(), payoff.to_c()
+        _check(self.lib, self.lib.fdmc_levels_accumulate(self.ctx, C.byref(m), C.byref(p), M, 1 if qmc else 0, mode,
+                                                         R, salt_or_seed, n, lv, rp, fi, co, bbs, len(bb_list), out))
+        return [[out[4 * i + k] for k in range(4)] for i in range(n)]
 
     # -- generator hooks (device pointers)
     def sobol_scrambled_u32(self, salt, D, n, offset, d_out):

@@ -214,6 +214,19 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                 const fdmc_bb* bb, const fdmc_pca* pca, double* sums_out,
                                 int32_t* n_acc_out);
 
+/* One coupled fine/coarse pass (the body of launch_level, methods_cuda.cu:1347-1395
+ * and 1800-1924) for n_req (level, replica, sample range) requests, run as one
+ * persistent launch.  qmc = 0: Philox paths (seed = salt_or_seed, stream
+ * 0x100 + level); qmc = 1: Sobol points of salt salt_or_seed + level*R + replica
+ * (mode RAW or BB with bb_list[level]).  sums_out: n_req x {sum dY, sum dY^2,
+ * sum Yf, sum Yf^2} (methods_cuda.cu:514-517). */
+fdmc_status fdmc_levels_accumulate(fdmc_context* ctx, const fdmc_model* model,
+                                   const fdmc_payoff* payoff, int32_t M, int32_t qmc,
+                                   int32_t mode, int32_t R, uint32_t salt_or_seed, int32_t n_req,
+                                   const int32_t* levels, const int32_t* replicas,
+                                   const int64_t* first, const int64_t* count,
+                                   const fdmc_bb* const* bb_list, int32_t n_bb, double* sums_out);
+
 /* ---- scrambled Sobol generator (methods_cuda.cu:753-844) ----
  * d_out is [D][n] (dimension-major, as kernel_gen_hh_scrambled_sobol_normal writes).
  * Point p of dimension d is the Hong-Hickernell-scrambled Gray-code Sobol point

@@ -1,0 +1,127 @@
+"""GPU parity of the persistent multilevel kernel (MLMC / MLQMC).
+
+The per-level sums are compared with the oracle's restatement of kernel_mlmc
+(methods_cuda.cu:445-518, with the Heston running-payoff fix) on the same Philox
+paths or scrambled Sobol points; the estimator loops (run_mlmc_cuda, :1305-1487;
+run_mlqmc_cuda, :1680-2019) are compared with the oracle's loops and with
+closed forms (Black-Scholes, Heston's semi-closed form).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fdmc
+import pyoracle as O
+from closed_forms import heston_call
+
+pytestmark = pytest.mark.gpu
+
+GBM = fdmc.GBMParams(100.0, 0.05, 0.2, 1.0)
+HES = fdmc.HestonParams(100.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04, 1.0).compute_cholesky()
+DUP = fdmc.DupireLocalParams(100.0, 0.05, 0.2, 0.5, 0.7, 1.0)
+LVQ = fdmc.LocalVolQuadParams(100.0, 0.05, 0.2, 0.1, 1.0, 100.0, 1.0)
+EURO = fdmc.European(100.0, 0.05, 1.0)
+ASIAN = fdmc.Asian(100.0)
+LOOK = fdmc.Lookback(0.2)
+BAR = fdmc.Barrier(100.0, 130.0, 0.2, 0.05, 1.0)
+
+
+def _check_sums(got, ref, n, barrier):
+    got, ref = np.asarray(got) / n, np.asarray(ref) / n
+    tol0 = 2e-5 * np.maximum(1.0, np.abs(ref[2])) + 1e-6 + (4 * 40.0 / n if barrier else 0.0)
+    assert abs(got[0] - ref[0]) <= tol0, (got, ref)
+    assert abs(got[2] - ref[2]) <= tol0, (got, ref)
+    assert abs(got[3] - ref[3]) <= 1e-4 * abs(ref[3]) + (4 * 1600.0 / n if barrier else 1e-6), (got, ref)
+
+
+@pytest.mark.parametrize("model,payoff,M,levels", [
+    (HES, EURO, 4, (0, 1, 2, 3, 4)),
+    (HES, BAR, 4, (0, 1, 3)),
+    (GBM, ASIAN, 2, (0, 1, 2, 5, 7)),
+    (DUP, LOOK, 2, (1, 4)),
+    (LVQ, EURO, 4, (2, 3)),
+], ids=["heston-euro-M4", "heston-barrier-M4", "gbm-asian-M2", "dupire-look-M2", "lvq-euro-M4"])
+def test_mlmc_level_sums_parity(engine, model, payoff, M, levels):
+    reqs = [(l, 0, 1000 * l, 700 + 37 * l) for l in levels]   # unaligned ranges
+    got = engine.levels_accumulate(model, payoff, M, reqs, qmc=False, salt_or_seed=42)
+    for (l, _, first, n), g in zip(reqs, got):
+        ref = O.mlmc_level_sums(model, payoff, M, l, 42, first, n)
+        _check_sums(g, ref, n, isinstance(payoff, fdmc.Barrier))
+
+
+@pytest.mark.parametrize("model,payoff,M,mode", [
+    (GBM, ASIAN, 2, fdmc.NOISE_BB),
+    (DUP, EURO, 2, fdmc.NOISE_RAW),
+    (HES, EURO, 4, fdmc.NOISE_RAW),
+    (LVQ, LOOK, 2, fdmc.NOISE_BB),
+], ids=["gbm-asian-bb", "dupire-euro-raw", "heston-euro-raw", "lvq-look-bb"])
+def test_mlqmc_level_sums_parity(engine, model, payoff, M, mode):
+    R, L = 3, 5
+    bbl = [engine.bb_upload(fdmc.bb_precompute(M ** l, 1.0)) for l in range(L + 1)] if mode == fdmc.NOISE_BB else []
+    reqs = [(l, r, 100 * r, 513) for l in (0, 2, L) for r in range(R)]
+    got = engine.levels_accumulate(model, payoff, M, reqs, qmc=True, mode=mode, R=R, salt_or_seed=42, bb_list=bbl)
+    for b in bbl:
+        b.free()
+    obbs = {l: O.OracleBB(M ** l, 1.0) for l in range(L + 1)} if mode == fdmc.NOISE_BB else {}
+    for l in (0, 2, L):
+        ref = O.mlqmc_level_sums(model, payoff, M, l, R, 42, 0, 513, mode, bb=obbs.get(l))
+        ref2 = O.mlqmc_level_sums(model, payoff, M, l, R, 42, 100, 513, mode, bb=obbs.get(l))
+        ref3 = O.mlqmc_level_sums(model, payoff, M, l, R, 42, 200, 513, mode, bb=obbs.get(l))
+        rows = {0: ref[0], 1: ref2[1], 2: ref3[2]}
+        for r in range(R):
+            g = got[[i for i, q in enumerate(reqs) if q[0] == l and q[1] == r][0]]
+            _check_sums(g, rows[r], 513, False)
+
+
+def test_run_mlmc_matches_oracle_loop(engine):
+    cfg = fdmc.MLMCConfig(M=2, max_L=6, pilot_n=2000, seed=7)
+    g = engine.run_mlmc_cuda(GBM, EURO, 2e-2, cfg)
+    r = O.run_mlmc(GBM, EURO, 2e-2, cfg)
+    assert g.levels == r.levels
+    assert abs(g.n_samples - r.n_samples) <= 0.01 * r.n_samples
+    assert abs(g.price - r.price) < 0.2 * r.std_error + 1e-4
+
+
+def test_run_mlqmc_matches_oracle_loop(engine):
+    ml = fdmc.MLMCConfig(M=2, max_L=6, pilot_n=256)
+    qc = fdmc.QMCConfig(R=8)
+    bbl = [engine.bb_upload(fdmc.bb_precompute(2 ** l, 1.0)) for l in range(7)]
+    g = engine.run_mlqmc_cuda(GBM, ASIAN, 5e-3, ml, qc, fdmc.NOISE_BB, bbl)
+    for b in bbl:
+        b.free()
+    r = O.run_mlqmc(GBM, ASIAN, 5e-3, ml, qc, fdmc.NOISE_BB, [O.OracleBB(2 ** l, 1.0) for l in range(7)])
+    # the allocation rounds N_opt = ceil(...) of FP64 variance estimates that differ in
+    # the last digits (FP32 paths), so sample counts may differ slightly
+    assert g.levels == r.levels
+    assert abs(g.n_samples - r.n_samples) <= 0.02 * r.n_samples
+    assert abs(g.price - r.price) < 1.0 * r.std_error
+
+
+def test_mlmc_gbm_european_vs_black_scholes(engine):
+    r = engine.run_mlmc_cuda(GBM, EURO, 5e-3, fdmc.MLMCConfig(M=2, max_L=10, pilot_n=2000))
+    bs = fdmc.bs_call(100, 100, 1, 0.05, 0.2)
+    assert abs(r.price - bs) < 3 * r.std_error + 5e-3 / math.sqrt(2)
+
+
+def test_c4_heston_mlmc_european_vs_closed_form(engine):
+    """Config 4: Heston MLMC, M=4, L <= 8, eps = 5e-4, pilot 1e4 per level."""
+    cfg = fdmc.MLMCConfig(M=4, max_L=8, pilot_n=10000, seed=42)
+    r = engine.run_mlmc_cuda(HES, EURO, 5e-4, cfg)
+    ref = heston_call(100, 100, 1, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04)
+    assert r.std_error < 5e-4 / math.sqrt(2) * 1.01
+    assert abs(r.price - ref) < 3 * r.std_error + 5e-4 / math.sqrt(2), (r.price, ref, r.std_error, r.levels)
+
+
+def test_c4_heston_mlmc_barrier_vs_single_level(engine):
+    """Config 4 barrier: the coupled kernel applies the running maximum for Heston
+    (the reference never updates it there, methods_cuda.cu:464-470, so its
+    barrier never knocks out).  With the finest level capped at L = 4 (4^4 = 256
+    steps) the telescoping sum estimates the 256-step BGK-corrected price, which
+    single-level MC at 256 steps estimates independently."""
+    euro = heston_call(100, 100, 1, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04)
+    r = engine.run_mlmc_cuda(HES, BAR, 2e-3, fdmc.MLMCConfig(M=4, max_L=4, pilot_n=10000, seed=42))
+    assert r.levels == 5
+    m, v = engine.run_mc_fixed(HES, BAR, 256, 4_000_000, 5)
+    assert r.price < euro - 1.0          # knock-out is active
+    assert abs(r.price - m) < 3 * math.sqrt(r.std_error ** 2 + v), (r.price, m)

@@ -1,0 +1,329 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each group checks the oracle against something it was not derived from:
+scipy's independent Sobol generator and ndtri, the Random123 known-answer
+vectors, closed-form eigenpairs and covariances, numerical integration of the
+closed forms, hand-computed paths, exact special-case reductions of the models,
+net properties that an ordering or masking mistake in the scramble would break,
+and the reference's own report prices.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import integrate, stats
+from scipy.special import ndtri
+
+import fdmc
+import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ---------------------------------------------------------------- Sobol
+def test_sobol_raw_matches_scipy_first_dims():
+    D, m = 64, 10
+    X = (stats.qmc.Sobol(D, scramble=False, bits=32).random_base2(m) * 2.0 ** 32).astype(np.uint64)
+    for d in range(D):
+        for i in range(0, 1 << m, 7):
+            assert O.sobol_raw(d, i) == X[i, d], (d, i)
+
+
+def test_sobol_raw_matches_scipy_high_dims():
+    D = 20000
+    X = (stats.qmc.Sobol(D, scramble=False, bits=32).random(33) * 2.0 ** 32).astype(np.uint64)
+    for d in (100, 1023, 5000, 12345, 19990, 19999):
+        for i in range(33):
+            assert O.sobol_raw(d, i) == X[i, d], (d, i)
+
+
+def test_dimension_zero_is_van_der_corput():
+    v = O.dirvectors(0)
+    assert [int(x) for x in v] == [1 << (31 - j) for j in range(32)]
+    # Gray-code order: 0, 1/2, 3/4, 1/4, ...
+    assert [O.sobol_raw(0, i) for i in range(4)] == [0, 1 << 31, 3 << 30, 1 << 30]
+    assert [O.sobol_raw(1, i) for i in range(4)] == [0, 1 << 31, 1 << 30, 3 << 30]
+
+
+def test_scramble_is_lower_triangular_with_unit_diagonal():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        base = int(rng.integers(0, 2 ** 32))
+        raw = int(rng.integers(0, 2 ** 32))
+        b = int(rng.integers(0, 32))
+        y0 = O.lib().orc_hh_scramble(raw, base)
+        y1 = O.lib().orc_hh_scramble(raw ^ (1 << b), base)
+        diff = y0 ^ y1
+        # flipping raw bit b flips output bit b and possibly lower bits only
+        assert (diff >> b) == 1, (base, raw, b, hex(diff))
+
+
+def test_scramble_is_f2_affine():
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        base = int(rng.integers(0, 2 ** 32))
+        a, b = (int(x) for x in rng.integers(0, 2 ** 32, 2))
+        f = lambda x: O.lib().orc_hh_scramble(x, base)
+        assert f(a ^ b) == f(a) ^ f(b) ^ f(0)
+
+
+def test_scrambled_points_keep_the_net_property():
+    """Dims 0 and 1 form a (0, m, 2)-net in base 2; the lower-triangular scramble
+    with a digital shift keeps every elementary box of area 2^-m holding one point."""
+    m = 8
+    for salt in (42, 43, 1234567):
+        pts = O.sobol_scrambled_block(salt, 2, 0, 1 << m)
+        for a in range(m + 1):
+            bx = (pts[0] >> np.uint32(32 - a)) if a else np.zeros(1 << m, np.uint32)
+            by = (pts[1] >> np.uint32(32 - (m - a))) if m - a else np.zeros(1 << m, np.uint32)
+            cells = bx.astype(np.uint64) * (1 << (m - a)) + by
+            assert len(np.unique(cells)) == 1 << m, (salt, a)
+
+
+def test_linear_table_generator_equals_per_point_scramble():
+    rng = np.random.default_rng(3)
+    for salt, D, off, n in ((42, 16, 0, 300), (99, 40, 123456, 200), (7, 3, 0xFFFFFFFF - 100, 101)):
+        blk = O.sobol_scrambled_block(salt, D, off, n)
+        for _ in range(150):
+            d, p = int(rng.integers(D)), int(rng.integers(n))
+            assert blk[d, p] == O.sobol_scrambled(salt, d, off + p)
+
+
+def test_golden_scrambled_vectors():
+    for v in json.load(open(os.path.join(HERE, "golden", "sobol_scrambled.json"))):
+        assert O.sobol_scrambled(v["salt"], v["dim"], v["index"]) == v["value"], v
+
+
+# ---------------------------------------------------------------- uniform map / ICDF
+def test_reference_uniform_map_hits_one():
+    # methods_cuda.cu:823: ((float)s + 1) * 2^-32 rounds to 1.0f near the top
+    assert O.lib().orc_u_reference(0xFFFFFFFF) == 1.0
+    assert O.lib().orc_u_reference(0xFFFFFF80) == 1.0
+    assert O.lib().orc_u_open(0xFFFFFFFF) < 1.0 and O.lib().orc_u_open(0) > 0.0
+
+
+def test_ndtri_matches_scipy():
+    us = np.concatenate([np.logspace(-300, -1, 300), np.linspace(0.01, 0.99, 199), 1 - np.logspace(-16, -1, 80)])
+    ref = ndtri(us)
+    got = np.array([O.lib().orc_ndtri(float(u)) for u in us])
+    assert np.max(np.abs(got - ref) / np.maximum(1, np.abs(ref))) < 1e-13
+
+
+def test_normal_from_u32_is_odd_and_exact():
+    for s in (0, 1, 12345, 2 ** 31 - 1, 2 ** 24 + 3):
+        x = O.lib().orc_normal_u32(s)
+        assert x == -O.lib().orc_normal_u32(2 ** 32 - 1 - s)
+        assert abs(x - ndtri((s + 0.5) / 2 ** 32)) < 1e-12 * max(1, abs(x))
+
+
+# ---------------------------------------------------------------- Philox / Box-Muller
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32_10
+    assert O.philox([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert O.philox([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert O.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_box_muller_normals_are_standard():
+    z = np.concatenate([O.prng_normals(123, 0, p, 64) for p in range(4000)]).astype(np.float64)
+    assert abs(z.mean()) < 4 / math.sqrt(z.size)
+    assert abs(z.var() - 1) < 4 * math.sqrt(2 / z.size)
+    assert stats.kstest(z, "norm").pvalue > 1e-3
+
+
+# ---------------------------------------------------------------- Brownian bridge
+@pytest.mark.parametrize("N,T", [(1, 1.0), (2, 0.5), (8, 1.0), (64, 2.0)])
+def test_bridge_covariance_is_min_t(N, T):
+    bb = O.OracleBB(N, T)
+    B = np.stack([bb.apply_f64(np.eye(N)[j]) for j in range(N)], axis=1)   # dW = B z
+    W = np.cumsum(B, axis=0)
+    t = np.arange(1, N + 1) * T / N
+    assert np.allclose(W @ W.T, np.minimum.outer(t, t), atol=1e-12)
+
+
+def test_bridge_f32_matches_f64_and_raw_for_one_step():
+    bb = O.OracleBB(256, 1.0)
+    z = np.random.default_rng(5).standard_normal(256)
+    assert np.allclose(bb.apply_f32(z.astype(np.float32)), bb.apply_f64(z), atol=2e-6)
+    one = O.OracleBB(1, 0.7)
+    assert abs(one.apply_f64(np.array([1.3]))[0] - 1.3 * math.sqrt(0.7)) < 1e-15
+
+
+# ---------------------------------------------------------------- PCA
+@pytest.mark.parametrize("m", [4, 33, 128])
+def test_pca_matches_closed_form_eigenpairs(m):
+    T = 1.0
+    h = T / m
+    C = h * np.minimum.outer(np.arange(1, m + 1), np.arange(1, m + 1)).astype(float)
+    w, V = O.eigh_jacobi(C)
+    k = np.arange(1, m + 1)
+    lam = h / (4 * np.sin((2 * k - 1) * np.pi / (2 * (2 * m + 1))) ** 2)
+    assert np.allclose(w, lam, rtol=1e-12)
+    i = np.arange(1, m + 1)[:, None]
+    Vc = 2 / np.sqrt(2 * m + 1) * np.sin((2 * k[None, :] - 1) * i * np.pi / (2 * m + 1))
+    assert np.allclose(np.abs(V), np.abs(Vc), atol=1e-10)
+    M = O.pca_compute(m, T)
+    assert np.allclose(M @ M.T, h * np.eye(m), atol=1e-13)
+
+
+def test_jacobi_matches_numpy_on_random_symmetric():
+    A = np.random.default_rng(7).standard_normal((40, 40))
+    A = A + A.T
+    w, V = O.eigh_jacobi(A)
+    assert np.allclose(w, np.sort(np.linalg.eigvalsh(A))[::-1], atol=1e-11)
+    assert np.allclose(V @ np.diag(w) @ V.T, A, atol=1e-10)
+
+
+def test_kronecker_pca_reproduces_increment_covariance():
+    n, steps, T = 4, 8, 1.0
+    B = np.random.default_rng(9).uniform(0.3, 0.6, (n, 2))
+    corr = B @ B.T
+    np.fill_diagonal(corr, 1.0)
+    M = O.pca_kron_compute(steps, T, corr)
+    h = T / steps
+    assert np.allclose(M @ M.T, h * np.kron(np.eye(steps), corr), atol=1e-13)
+
+
+# ---------------------------------------------------------------- closed forms
+def test_black_scholes_report_value_and_integral():
+    bs = O.bs_call(100, 100, 1, 0.05, 0.2)
+    assert round(bs, 6) == 10.450584          # informes_finales/informe_completo.txt:8
+    f = lambda z: max(100 * math.exp((0.05 - 0.02) + 0.2 * z) - 100, 0) * stats.norm.pdf(z)
+    assert abs(bs - math.exp(-0.05) * integrate.quad(f, -10, 10, points=[0.0])[0]) < 1e-8
+
+
+@pytest.mark.parametrize("n", [1, 12, 64, 256])
+def test_geometric_asian_closed_form_vs_integral(n):
+    S0, K, T, mu, sig = 100.0, 95.0, 1.0, 0.05, 0.3
+    t = np.arange(1, n + 1) * T / n
+    m = math.log(S0) + (mu - 0.5 * sig * sig) * t.mean()
+    v = sig * sig * np.minimum.outer(t, t).sum() / n ** 2
+    f = lambda x: max(math.exp(m + math.sqrt(v) * x) - K, 0) * stats.norm.pdf(x)
+    ref = integrate.quad(f, -12, 12, points=[(math.log(K) - m) / math.sqrt(v)])[0]
+    assert abs(O.geom_asian_analytic(S0, K, T, mu, sig, n) - ref) < 1e-9
+
+
+# ---------------------------------------------------------------- path kernels
+def test_gbm_two_steps_by_hand():
+    P = O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.European(90, 0.05, 1.0), 2)
+    dW = np.array([0.1, -0.05], np.float32)
+    S = 100.0
+    for d in dW:
+        S = S + 0.05 * S * 0.5 + 0.2 * S * float(d)
+    assert abs(O.path_single(P, dW, 2) - (S - 90) * math.exp(-0.05)) < 1e-4
+
+
+def test_asian_is_undiscounted_and_excludes_S0():
+    P = O.make_params(fdmc.GBMParams(100, 0.0, 0.0, 1.0), fdmc.Asian(90), 4)
+    assert abs(O.path_single(P, np.zeros(4, np.float32), 4) - 10.0) < 1e-5
+    P2 = O.make_params(fdmc.GBMParams(100, 0.1, 0.0, 1.0), fdmc.Asian(100), 2)
+    S1, S2 = 100 * 1.05, 100 * 1.05 * 1.05
+    assert abs(O.path_single(P2, np.zeros(2, np.float32), 2) - ((S1 + S2) / 2 - 100)) < 1e-4
+
+
+def test_lookback_and_barrier_bgk_shift():
+    h = 1.0 / 4
+    corr = math.exp(-0.5826 * 0.2 * math.sqrt(h))
+    dW = np.array([-0.1, 0.05, 0.2, -0.02], np.float32)
+    S, path = 100.0, []
+    for d in dW:
+        S = S + 0.05 * S * h + 0.2 * S * float(d)
+        path.append(S)
+    P = O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Lookback(0.2), 4)
+    assert abs(O.path_single(P, dW, 4) - (path[-1] - min(100.0, min(path)) * corr)) < 1e-4
+    hi = max(path)
+    Pk = O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Barrier(90, hi / corr * 0.999, 0.2, 0.05, 1.0), 4)
+    assert O.path_single(Pk, dW, 4) == 0.0           # max / corr >= B: knocked out
+    Pa = O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Barrier(90, hi / corr * 1.001, 0.2, 0.05, 1.0), 4)
+    assert abs(O.path_single(Pa, dW, 4) - (path[-1] - 90) * math.exp(-0.05)) < 1e-4
+
+
+def test_model_reductions_are_exact():
+    rng = np.random.default_rng(11)
+    dW = (rng.standard_normal(64) * math.sqrt(1 / 64)).astype(np.float32)
+    euro = fdmc.European(100, 0.05, 1.0)
+    g = O.path_single(O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), euro, 64), dW, 64)
+    d = O.path_single(O.make_params(fdmc.DupireLocalParams(100, 0.05, 0.2, 0.0, 1.0, 1.0), euro, 64), dW, 64)
+    assert g == d                                    # alpha = 0, beta = 1 -> GBM(sigma0), bit-exact
+    lv = O.path_single(O.make_params(fdmc.LocalVolQuadParams(100, 0.05, 0.2, 0.0, 1.0, 100.0, 1.0), euro, 64), dW, 64)
+    assert abs(g - lv) < 1e-4                        # b = 0 -> GBM(a)
+    hz = np.zeros(128, np.float32)
+    hz[0::2] = dW
+    hz[1::2] = rng.standard_normal(64).astype(np.float32)
+    hes = fdmc.HestonParams(100, 0.05, 2.0, 0.04, 0.0, -0.5, 0.04, 1.0).compute_cholesky()
+    hv = O.path_single(O.make_params(hes, euro, 64), hz, 64)
+    g2 = O.path_single(O.make_params(fdmc.GBMParams(100, 0.05, float(np.sqrt(np.float32(0.04))), 1.0), euro, 64), dW, 64)
+    assert abs(hv - g2) < 1e-5                       # xi = 0, v0 = theta -> GBM(sqrt(theta))
+    bk = fdmc.MultiGBMParams(1, [100.0], [0.2], 0.05, 1.0)
+    b = O.path_basket(O.make_params(bk, fdmc.Basket(100, 0.05, 1.0, 1), 64), dW, 64)
+    assert abs(b - g) < 1e-5                         # one-asset basket -> European
+
+
+def test_coupled_path_consistency():
+    rng = np.random.default_rng(13)
+    M, nf = 4, 16
+    dW = (rng.standard_normal(nf) * math.sqrt(1 / nf)).astype(np.float32)
+    P = O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Asian(100), 1)
+    pf, pc = O.path_coupled(P, dW, nf, M)
+    Pf = O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Asian(100), nf)
+    Pc = O.make_params(fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Asian(100), nf // M)
+    assert abs(pf - O.path_single(Pf, dW, nf)) < 1e-5
+    dWc = dW.reshape(-1, M).sum(axis=1).astype(np.float32)
+    assert abs(pc - O.path_single(Pc, dWc, nf // M)) < 1e-4
+    pf0, pc0 = O.path_coupled(P, dW[:1], 1, M)
+    assert pc0 == 0.0
+
+
+# ---------------------------------------------------------------- estimators
+def test_mc_european_within_three_se_of_black_scholes():
+    r = O.run_mc(fdmc.GBMParams(), fdmc.European(100, 0.05, 1), 5e-2, 64, fdmc.MCConfig())
+    assert abs(r.price - (10.450584 - 0.002)) < 3 * r.std_error
+
+
+def test_qmc_bb_european_within_eps():
+    bb = O.OracleBB(64, 1.0)
+    r = O.run_qmc(fdmc.GBMParams(), fdmc.European(100, 0.05, 1), 5e-3, 64, fdmc.QMCConfig(R=16, N0=512), 1, bb=bb)
+    assert r.flags == 0
+    assert abs(r.price - 10.450584) < 3 * r.std_error + 0.004
+
+
+def test_mlmc_gbm_european():
+    r = O.run_mlmc(fdmc.GBMParams(), fdmc.European(100, 0.05, 1), 5e-2, fdmc.MLMCConfig(M=2, max_L=8, pilot_n=1000))
+    assert abs(r.price - 10.450584) < 3 * r.std_error + 5e-2
+
+
+def test_cv_pilot_beta_for_asian_geometric():
+    g = fdmc.GBMParams(100, 0.05, 0.2, 1.0)
+    E = fdmc.geom_asian_analytic(100, 100, 1, 0.05, 0.2, 64)
+    p = O.cv_pilot(g, g, fdmc.Asian(100), fdmc.GeomAsian(100), E, 64, 20000, 0)
+    assert 0.9 < p.beta < 1.2 and p.var_cv < 0.02 * p.var_plain
+
+
+REPORT = json.load(open(os.path.join(HERE, "golden", "report_prices.json")))
+
+
+def _build(entry):
+    md, pd = entry["model"], entry["payoff"]
+    if md["kind"] == "gbm":
+        m = fdmc.GBMParams(md["S0"], md["mu"], md["sigma"], md["T"])
+    elif md["kind"] == "heston":
+        m = fdmc.HestonParams(md["S0"], md["mu"], md["kappa"], md["theta"], md["xi"], md["rho"], md["v0"],
+                              md["T"]).compute_cholesky()
+    else:
+        m = fdmc.DupireLocalParams(md["S0"], md["mu"], md["sigma0"], md["alpha"], md["beta_d"], md["T"])
+    k = pd["kind"]
+    p = {"european": lambda: fdmc.European(pd["K"], pd["r"], pd["T"]), "asian": lambda: fdmc.Asian(pd["K"]),
+         "lookback": lambda: fdmc.Lookback(pd["sigma"]),
+         "barrier": lambda: fdmc.Barrier(pd["K"], pd["B"], pd["sigma"], pd["r"], pd["T"])}[k]()
+    return m, p
+
+
+@pytest.mark.parametrize("entry", [e for e in REPORT if e["kind"] == "mc500k"], ids=lambda e: e["example"])
+def test_report_reference_prices(entry):
+    """The reference's own printed prices (fixed-seed 500k-path runs, +-0.01-0.02)."""
+    m, p = _build(entry)
+    mean, var = O.run_mc_fixed(m, p, entry["n_steps"], 200000, 2024)
+    assert abs(mean - entry["price"]) < 4 * math.sqrt(var + var * 200000 / 500000), (mean, entry["price"])
