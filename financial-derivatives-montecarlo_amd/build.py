@@ -19,7 +19,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["engine.cpp", "k_sobol.hip", "k_path_raw.hip", "k_path_bb.hip", "k_path_prng.hip",
-           "k_levels_raw.hip", "k_levels_bb.hip", "k_levels_prng.hip"]
+           "k_levels_raw.hip", "k_levels_bb.hip", "k_levels_prng.hip",
+           "k_greeks_raw.hip", "k_greeks_bb.hip", "k_greeks_prng.hip",
+           "../cpp/utils.cpp", "../cpp/methods_hip.cpp"]
+EXAMPLES = ["asian_table"]
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
@@ -27,9 +30,10 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-ta
 
 
 def compile_one(src, verbose):
-    s = os.path.join(CSRC, src)
-    o = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", s, "-o", o]
+    s = os.path.normpath(os.path.join(CSRC, src))
+    o = os.path.join(OBJ, os.path.splitext(os.path.basename(src))[0] + ".o")
+    lang = ["-x", "hip"] if src.endswith(".hip") or src == "engine.cpp" else []
+    cmd = [HIPCC] + FLAGS + lang + ["-c", s, "-o", o]
     t = time.time()
     p = subprocess.run(cmd, capture_output=True, text=True)
     if verbose or p.returncode != 0:
@@ -55,6 +59,16 @@ def build(jobs=None, verbose=False):
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
     print("  linked", LIB, flush=True)
+    for ex in EXAMPLES:
+        src = os.path.join(HERE, "cpp", "examples", ex + ".cpp")
+        exe = os.path.join(HERE, "lib", ex)
+        cmd = [HIPCC, "-O2", "-std=c++17", src, "-o", exe, "-L" + os.path.dirname(LIB), "-lfdmc",
+               "-Wl,-rpath,$ORIGIN"]
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            sys.stderr.write(p.stdout + p.stderr)
+            raise RuntimeError("example build failed: " + ex)
+        print("  built", exe, flush=True)
     return LIB
 
 

@@ -282,6 +282,7 @@ struct PassSpec {
     bool prng = false;            // pseudo-random noise (MC, CV, QMC fallback)
     uint32_t tag = 0;
     const fdmc_bb* bb = nullptr;
+    bool greeks = false;          // price + adjoint Greeks kernel (k_greeks)
 };
 
 // Runs one pass; writes R x nacc sums into out, returns nacc.
@@ -321,7 +322,14 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     a.n_steps = s.n_steps;
     a.nf = nf;
     a.D = D;
+    a.T = T;
     const int mk = model_kind_dev(m), pk = payoff_kind_dev(p);
+    if (s.greeks) {
+        if (mk != MK_GBM && mk != MK_DUPIRE && mk != MK_LVQ)
+            fail(FDMC_E_UNSUPPORTED, "adjoint Greeks need a one-factor GBM, Dupire or local-vol model");
+        if (pk == PK_BARRIER)
+            fail(FDMC_E_UNSUPPORTED, "adjoint Greeks need a Lipschitz payoff (the barrier indicator is not)");
+    }
 
     int nz, lb = 0;
     const void* fn = nullptr;
@@ -331,7 +339,7 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
         nz = NZ_PRNG;
         a.seed = s.salt_base;
         a.tag = s.prng ? s.tag : TAG_QMC_FALLBACK;
-        fn = path_kernel_prng(mk, pk, cv, cpk, 0);
+        fn = s.greeks ? greeks_kernel_prng(mk, pk, cv, cpk, 0) : path_kernel_prng(mk, pk, cv, cpk, 0);
     } else if (s.mode == FDMC_NOISE_BB) {
         if (nf != 1) fail(FDMC_E_UNSUPPORTED, "Brownian bridge needs a one-factor model");
         if (!s.bb) fail(FDMC_E_INVALID, "BB mode needs a DeviceBBData");
@@ -342,12 +350,12 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
         lb = s.n_steps >= 16 ? 4 : 0;
         a.G = s.n_steps >> lb;
         a.bbw = s.bb->d_w;
-        fn = path_kernel_bb(mk, pk, cv, cpk, lb);
+        fn = s.greeks ? greeks_kernel_bb(mk, pk, cv, cpk, lb) : path_kernel_bb(mk, pk, cv, cpk, lb);
     } else if (s.mode == FDMC_NOISE_PCA) {
         fail(FDMC_E_UNSUPPORTED, "PCA noise mode is not available for this estimator yet");
     } else {
         nz = NZ_SOBOL_RAW;
-        fn = path_kernel_raw(mk, pk, cv, cpk, 0);
+        fn = s.greeks ? greeks_kernel_raw(mk, pk, cv, cpk, 0) : path_kernel_raw(mk, pk, cv, cpk, 0);
     }
     if (!fn) fail(FDMC_E_UNSUPPORTED, "model/payoff/control combination not supported");
 
@@ -370,11 +378,12 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     a.count = s.count;
     a.tiles_per_wg = (int)tpw;
     a.nwg_x = (int)nwg;
-    const int nacc = (cv == CV_NONE) ? 2 : 7;
+    const int nacc = ((cv == CV_NONE) ? 2 : 7) + (s.greeks ? 6 : 0);
     a.partials = (double*)c->partials.ensure((size_t)s.R * nwg * nacc * sizeof(double));
     double* d_sums = (double*)c->sums.ensure((size_t)s.R * nacc * sizeof(double));
 
-    const size_t lds = path_lds_bytes(nz, D, a.G);
+    const size_t lds = s.greeks ? greeks_lds_bytes(nz, D, a.G, s.n_steps) : path_lds_bytes(nz, D, a.G);
+    if (lds > 160 * 1024) fail(FDMC_E_UNSUPPORTED, "LDS footprint of this pass exceeds 160 KiB");
     if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     void* args[] = {(void*)&a};
     if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
@@ -409,7 +418,8 @@ void mean_var(const std::vector<double>& v, double& m, double& var) {
 // Doubling loop of run_qmc_cuda / run_qmc_cv_cuda (methods_cuda.cu:1624-1665,
 // 2203-2273): accumulate only the new points of each replica, stop when
 // var_of_means < eps^2/2.
-void qmc_loop(fdmc_context* c, PassSpec s, double eps, const fdmc_qmc_config& cfg, fdmc_result* out) {
+void qmc_loop(fdmc_context* c, PassSpec s, double eps, const fdmc_qmc_config& cfg, fdmc_result* out,
+              fdmc_greeks* greeks = nullptr, const double* dE = nullptr) {
     const double t0 = now_s();
     if (cfg.R <= 0) fail(FDMC_E_INVALID, "QMCConfig.R must be positive");
     if (cfg.max_doublings <= 0) fail(FDMC_E_INVALID, "QMCConfig.max_doublings must be positive");
@@ -447,6 +457,24 @@ void qmc_loop(fdmc_context* c, PassSpec s, double eps, const fdmc_qmc_config& cf
         if (vom < eps * eps / 2.0) { conv = true; break; }
         if (2 * N > cap) break;
         N *= 2;
+    }
+    if (greeks && !tot.empty()) {
+        // per replica Greek means, then mean and standard error across replicas
+        const int nacc = (int)(tot.size() / cfg.R), np = nacc - 6;
+        double g[3], se[3];
+        for (int k = 0; k < 3; k++) {
+            for (int r = 0; r < cfg.R; r++) means[r] = tot[(size_t)r * nacc + np + 2 * k] / (double)done;
+            double v;
+            mean_var(means, g[k], v);
+            se[k] = std::sqrt(v / cfg.R);
+        }
+        const double beta = s.beta;
+        greeks->delta = g[0] + (dE ? beta * dE[0] : 0.0);
+        greeks->vega = g[1] + (dE ? beta * dE[1] : 0.0);
+        greeks->rho = g[2] + (dE ? beta * dE[2] : 0.0);
+        greeks->delta_se = se[0];
+        greeks->vega_se = se[1];
+        greeks->rho_se = se[2];
     }
     out->price = gm;
     out->std_error = std::sqrt(vom);
@@ -1001,6 +1029,51 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* c, const fdmc_model* model, const 
         const int nacc = run_pass(c, s, v);
         std::memcpy(sums_out, v.data(), v.size() * sizeof(double));
         if (n_acc_out) *n_acc_out = nacc;
+    });
+}
+
+fdmc_status fdmc_greeks_accumulate(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff,
+                                   const fdmc_model* cmodel, const fdmc_payoff* cpayoff, double E_ctrl,
+                                   double beta, int32_t n_steps, int32_t R, uint32_t salt_or_seed,
+                                   int64_t offset, int64_t n_points, int32_t mode, const fdmc_bb* bb,
+                                   double* sums_out, int32_t* n_acc_out) {
+    return guarded([&] {
+        if (!c || !sums_out) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if ((cmodel == nullptr) != (cpayoff == nullptr))
+            fail(FDMC_E_INVALID, "control model and payoff must both be given");
+        if (cmodel) { validate_model(cmodel); validate_payoff(cpayoff); }
+        if (offset < 0) fail(FDMC_E_INVALID, "offset must be non-negative");
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.cmodel = cmodel; s.cpayoff = cpayoff;
+        s.E_ctrl = E_ctrl; s.beta = beta; s.n_steps = n_steps; s.R = R; s.salt_base = salt_or_seed;
+        s.first = offset; s.count = n_points; s.bb = bb; s.greeks = true;
+        if (mode == FDMC_NOISE_PRNG) { s.prng = true; s.tag = 0; s.mode = FDMC_NOISE_RAW; }
+        else s.mode = mode;
+        std::vector<double> v;
+        const int nacc = run_pass(c, s, v);
+        std::memcpy(sums_out, v.data(), v.size() * sizeof(double));
+        if (n_acc_out) *n_acc_out = nacc;
+    });
+}
+
+fdmc_status fdmc_run_qmc_greeks(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff,
+                                const fdmc_model* cmodel, const fdmc_payoff* cpayoff, double E_ctrl,
+                                const double* dE_ctrl, double beta, double eps, int32_t n_steps,
+                                const fdmc_qmc_config* cfg, int32_t mode, const fdmc_bb* bb,
+                                fdmc_result* out, fdmc_greeks* greeks) {
+    return guarded([&] {
+        if (!c || !cfg || !out || !greeks) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if ((cmodel == nullptr) != (cpayoff == nullptr))
+            fail(FDMC_E_INVALID, "control model and payoff must both be given");
+        if (cmodel) { validate_model(cmodel); validate_payoff(cpayoff); }
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.cmodel = cmodel; s.cpayoff = cpayoff;
+        s.E_ctrl = E_ctrl; s.beta = beta; s.n_steps = n_steps; s.mode = mode; s.bb = bb; s.greeks = true;
+        qmc_loop(c, s, eps, *cfg, out, greeks, cmodel ? dE_ctrl : nullptr);
     });
 }
 

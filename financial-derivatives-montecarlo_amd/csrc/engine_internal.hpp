@@ -27,6 +27,11 @@ const void* levels_kernel_raw(int mk, int pk);
 const void* levels_kernel_bb(int mk, int pk);
 const void* levels_kernel_prng(int mk, int pk);
 
+// k_greeks_{raw,bb,prng}.hip: price + adjoint Greeks kernel, or null.
+const void* greeks_kernel_raw(int mk, int pk, int cv, int cpk, int lb);
+const void* greeks_kernel_bb(int mk, int pk, int cv, int cpk, int lb);
+const void* greeks_kernel_prng(int mk, int pk, int cv, int cpk, int lb);
+
 // k_path_{raw,bb,prng}.hip: kernel for a (model, payoff, control) combination, or null.
 const void* path_kernel_raw(int mk, int pk, int cv, int cpk, int lb);
 const void* path_kernel_bb(int mk, int pk, int cv, int cpk, int lb);

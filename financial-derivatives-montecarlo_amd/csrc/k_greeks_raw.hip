@@ -1,0 +1,23 @@
+// k_greeks_raw.hip -- instantiations of the price + adjoint-Greeks kernel for noise NZ_SOBOL_RAW.
+#include "k_greeks.inl"
+
+namespace fdmc {
+
+template <int LB>
+static const void* pick_greeks_raw(int mk, int pk, int cv, int cpk) {
+#define X(MKv, PKv, CVv, CPKv) \
+    if (mk == MKv && pk == PKv && cv == CVv && cpk == CPKv) return (const void*)&k_greeks<MKv, PKv, CVv, CPKv, NZ_SOBOL_RAW, LB>;
+#define XM(MKv) X(MKv, PK_EUROPEAN, CV_NONE, 0) X(MKv, PK_ASIAN, CV_NONE, 0) X(MKv, PK_GEOM_ASIAN, CV_NONE, 0) \
+    X(MKv, PK_LOOKBACK, CV_NONE, 0) X(MKv, PK_ASIAN, CV_SHARED, PK_GEOM_ASIAN)
+    XM(MK_GBM) XM(MK_DUPIRE) XM(MK_LVQ)
+#undef XM
+#undef X
+    return nullptr;
+}
+
+const void* greeks_kernel_raw(int mk, int pk, int cv, int cpk, int lb) {
+    (void)lb;
+    return pick_greeks_raw<4>(mk, pk, cv, cpk);
+}
+
+}  // namespace fdmc

@@ -43,6 +43,7 @@ struct PathLaunch {
     float h, sqrt_h, em;    // step, sqrt(step), e^{-kappa h}
     float bgk[2];           // exp(-0.5826 sigma_bgk sqrt(h)) per leg (1 when unused)
     float beta_cv, E_ctrl;
+    float T;                // maturity (rho of the discount)
     int n_steps, nf, D;
     int G, log2_G;          // BB: coarse blocks (N / 16, or N when N < 16)
     uint32_t seed, tag;     // PRNG stream key
@@ -89,6 +90,18 @@ inline size_t path_lds_bytes(int nz, int D, int G) {
     if (nz == NZ_SOBOL_BB) b += (size_t)(G + 1) * 256 * 4;
     b = (b + 15) & ~(size_t)15;
     return b + 4 * 7 * sizeof(double);
+}
+
+// Dynamic LDS bytes of one k_greeks launch: k_path's layout plus one S
+// checkpoint per lane and 16-step block.
+inline size_t greeks_lds_bytes(int nz, int D, int G, int n_steps) {
+    const int Gk = (nz == NZ_SOBOL_BB) ? G : (n_steps + 15) / 16;
+    size_t b = 0;
+    if (nz != NZ_PRNG && D <= SOBOL_LDS_MAX_D) b += (size_t)D * 20 * 4;
+    if (nz == NZ_SOBOL_BB) b += (size_t)(G + 1) * 256 * 4;
+    b += (size_t)Gk * 256 * 4;
+    b = (b + 15) & ~(size_t)15;
+    return b + 4 * 13 * sizeof(double);
 }
 
 // Dynamic LDS bytes of one k_levels launch (max over its entries).

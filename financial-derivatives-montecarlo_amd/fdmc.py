@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfdmc.so")
 # ---------------------------------------------------------------- enums
 MODEL_GBM, MODEL_HESTON, MODEL_DUPIRE, MODEL_MULTI_DUPIRE, MODEL_LOCALVOL_QUAD, MODEL_MULTI_GBM = range(6)
 PAYOFF_EUROPEAN, PAYOFF_ASIAN, PAYOFF_GEOM_ASIAN, PAYOFF_LOOKBACK, PAYOFF_BARRIER, PAYOFF_BASKET = range(6)
-NOISE_RAW, NOISE_BB, NOISE_PCA = 0, 1, 2
+NOISE_RAW, NOISE_BB, NOISE_PCA, NOISE_PRNG = 0, 1, 2, 3
 FLAG_PRNG_FALLBACK, FLAG_NOT_CONVERGED = 1, 2
 BGK_BETA = 0.5826  # payoffs.hpp:7
 D_MAX_SOBOL = 20000  # models.hpp:9
@@ -66,12 +66,18 @@ class CResult(C.Structure):
                 ("flags", C.c_uint32)]
 
 
+class CGreeks(C.Structure):
+    _fields_ = [("delta", C.c_double), ("vega", C.c_double), ("rho", C.c_double),
+                ("delta_se", C.c_double), ("vega_se", C.c_double), ("rho_se", C.c_double)]
+
+
 class CCVPilot(C.Structure):
     _fields_ = [("beta", C.c_double), ("var_plain", C.c_double), ("var_cv", C.c_double)]
 
 
 MCResult = namedtuple("MCResult", "price std_error n_samples time_s path_steps levels flags")
 CVPilot = namedtuple("CVPilot", "beta var_plain var_cv")
+Greeks = namedtuple("Greeks", "delta vega rho delta_se vega_se rho_se")
 
 
 def _result(r):
@@ -328,6 +334,14 @@ _SIGS = {
                                          C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                          C.POINTER(_P), C.c_int32, C.POINTER(C.c_double)]),
+    "fdmc_greeks_accumulate": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.POINTER(CModel),
+                                         C.POINTER(CPayoff), C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                         C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "fdmc_run_qmc_greeks": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.POINTER(CModel),
+                                      C.POINTER(CPayoff), C.c_double, C.POINTER(C.c_double), C.c_double,
+                                      C.c_double, C.c_int32, C.POINTER(CQMCConfig), C.c_int32, _P,
+                                      C.POINTER(CResult), C.POINTER(CGreeks)]),
     "fdmc_sobol_scrambled_u32": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_sobol_normal": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_icdf_u32": (C.c_int, [_P, _P, C.c_int64, _P]),
@@ -537,6 +551,33 @@ class Engine:
                                                          R, salt_or_seed, n, lv, rp, fi, co, bbs, len(bb_list), out))
         return [[out[4 * i + k] for k in range(4)] for i in range(n)]
 
+    def greeks_accumulate(self, model, payoff, n_steps, R, salt_or_seed, offset, n_points, mode=NOISE_RAW,
+                          dev_bb=None, ctrl_model=None, ctrl_payoff=None, E_ctrl=0.0, beta=0.0):
+        """One fused price + adjoint-Greeks pass; R rows of [price sums..., sum/sum^2 of delta, vega, rho]."""
+        m, p = model.to_c(), payoff.to_c()
+        cm = C.byref(ctrl_model.to_c()) if ctrl_model is not None else None
+        cp = C.byref(ctrl_payoff.to_c()) if ctrl_payoff is not None else None
+        sums = (C.c_double * (R * 13))()
+        nacc = C.c_int32()
+        _check(self.lib, self.lib.fdmc_greeks_accumulate(self.ctx, C.byref(m), C.byref(p), cm, cp, E_ctrl, beta,
+                                                         n_steps, R, salt_or_seed, offset, n_points, mode,
+                                                         dev_bb.handle if dev_bb else None, sums, C.byref(nacc)))
+        na = nacc.value
+        return [[sums[r * na + i] for i in range(na)] for r in range(R)]
+
+    def run_qmc_greeks(self, model, payoff, eps, n_steps, cfg=None, mode=NOISE_RAW, dev_bb=None, ctrl_model=None,
+                       ctrl_payoff=None, E_ctrl=0.0, dE_ctrl=None, beta=0.0):
+        cfg = cfg or QMCConfig()
+        r, g = CResult(), CGreeks()
+        m, p, c = model.to_c(), payoff.to_c(), cfg.to_c()
+        cm = C.byref(ctrl_model.to_c()) if ctrl_model is not None else None
+        cp = C.byref(ctrl_payoff.to_c()) if ctrl_payoff is not None else None
+        dE = (C.c_double * 3)(*(dE_ctrl or (0.0, 0.0, 0.0)))
+        _check(self.lib, self.lib.fdmc_run_qmc_greeks(self.ctx, C.byref(m), C.byref(p), cm, cp, E_ctrl, dE, beta, eps,
+                                                      n_steps, C.byref(c), mode, dev_bb.handle if dev_bb else None,
+                                                      C.byref(r), C.byref(g)))
+        return _result(r), Greeks(g.delta, g.vega, g.rho, g.delta_se, g.vega_se, g.rho_se)
+
     # -- generator hooks (device pointers)
     def sobol_scrambled_u32(self, salt, D, n, offset, d_out):
         _check(self.lib, self.lib.fdmc_sobol_scrambled_u32(self.ctx, salt, D, n, offset, d_out))
@@ -597,3 +638,12 @@ def geom_asian_analytic(S0, K, T, mu, sigma, n):  # utils.cpp:34-47 (undiscounte
         return max(math.exp(m_G) - K, 0.0)
     d1 = (m_G + s2 - math.log(K)) / s
     return math.exp(m_G + 0.5 * s2) * norm_cdf(d1) - K * norm_cdf(d1 - s)
+
+
+def geom_asian_greeks(S0, K, T, mu, sigma, n):
+    """(delta, vega, rho) of geom_asian_analytic by central differences (FP64)."""
+    f = geom_asian_analytic
+    hS, hs, hm = 1e-4 * S0, 1e-5, 1e-5
+    return ((f(S0 + hS, K, T, mu, sigma, n) - f(S0 - hS, K, T, mu, sigma, n)) / (2 * hS),
+            (f(S0, K, T, mu, sigma + hs, n) - f(S0, K, T, mu, sigma - hs, n)) / (2 * hs),
+            (f(S0, K, T, mu + hm, sigma, n) - f(S0, K, T, mu - hm, sigma, n)) / (2 * hm))

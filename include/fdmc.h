@@ -107,7 +107,8 @@ typedef struct fdmc_mlmc_config {/* MLMCConfig */
     uint32_t seed;               /* PRNG key for run_mlmc, default 42 */
 } fdmc_mlmc_config;
 
-typedef enum { FDMC_NOISE_RAW = 0, FDMC_NOISE_BB = 1, FDMC_NOISE_PCA = 2 } fdmc_noise_mode;
+typedef enum { FDMC_NOISE_RAW = 0, FDMC_NOISE_BB = 1, FDMC_NOISE_PCA = 2,
+               FDMC_NOISE_PRNG = 3 /* accumulate hooks only: Philox paths */ } fdmc_noise_mode;
 
 #define FDMC_FLAG_PRNG_FALLBACK 1u   /* QMC dimension > 20000: pseudo-random points were used */
 #define FDMC_FLAG_NOT_CONVERGED 2u   /* stopping rule not met within the configured limits */
@@ -121,6 +122,14 @@ typedef struct fdmc_result {     /* MCResult (methods_cuda.cuh:41-46) + additive
     int32_t levels;              /* MLMC/MLQMC: finest level L + 1; QMC: doublings done */
     uint32_t flags;
 } fdmc_result;
+
+/* Pathwise adjoint Greeks (no counterpart in the reference): delta = dV/dS0,
+ * vega = dV/d(volatility parameter: sigma, Dupire sigma0, local-vol level a),
+ * rho = dV/d(mu = r); each with its standard error across replicas. */
+typedef struct fdmc_greeks {
+    double delta, vega, rho;
+    double delta_se, vega_se, rho_se;
+} fdmc_greeks;
 
 typedef struct fdmc_cv_pilot_result { double beta, var_plain, var_cv; } fdmc_cv_pilot_result; /* CVPilot */
 
@@ -226,6 +235,31 @@ fdmc_status fdmc_levels_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                    const int32_t* levels, const int32_t* replicas,
                                    const int64_t* first, const int64_t* count,
                                    const fdmc_bb* const* bb_list, int32_t n_bb, double* sums_out);
+
+/* ---- adjoint Greeks (additive; the reference prices only) ----
+ * Fused price + reverse-mode pathwise Greeks for one-factor GBM / Dupire /
+ * local-vol models and Lipschitz payoffs (European, Asian, geometric Asian,
+ * lookback), optionally with the geometric-Asian control on the same path.
+ * Path states are recomputed in reverse from the Sobol index (or Philox
+ * counter) with S checkpoints every 16 steps; no path is stored.
+ * greeks_accumulate: one pass; sums_out R x n_acc with n_acc = 2 (+5 with a
+ * control) price sums followed by {sum g, sum g^2} for delta, vega, rho.
+ * mode: RAW, BB or FDMC_NOISE_PRNG (Philox stream seed salt_or_seed + r). */
+fdmc_status fdmc_greeks_accumulate(fdmc_context* ctx, const fdmc_model* model,
+                                   const fdmc_payoff* payoff, const fdmc_model* ctrl_model,
+                                   const fdmc_payoff* ctrl_payoff, double E_ctrl, double beta,
+                                   int32_t n_steps, int32_t R, uint32_t salt_or_seed,
+                                   int64_t offset, int64_t n_points, int32_t mode,
+                                   const fdmc_bb* bb, double* sums_out, int32_t* n_acc_out);
+/* run_qmc_cuda / run_qmc_cv_cuda with Greeks: same doubling loop and stopping
+ * rule on the price; the control's expectation E_ctrl enters the Greeks through
+ * dE_ctrl[3] = its delta, vega, rho (NULL: no control). */
+fdmc_status fdmc_run_qmc_greeks(fdmc_context* ctx, const fdmc_model* model,
+                                const fdmc_payoff* payoff, const fdmc_model* ctrl_model,
+                                const fdmc_payoff* ctrl_payoff, double E_ctrl,
+                                const double* dE_ctrl, double beta, double eps, int32_t n_steps,
+                                const fdmc_qmc_config* cfg, int32_t mode, const fdmc_bb* bb,
+                                fdmc_result* out, fdmc_greeks* greeks);
 
 /* ---- scrambled Sobol generator (methods_cuda.cu:753-844) ----
  * d_out is [D][n] (dimension-major, as kernel_gen_hh_scrambled_sobol_normal writes).
