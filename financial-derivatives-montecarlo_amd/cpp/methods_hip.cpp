@@ -108,6 +108,7 @@ CPayoff to_c(const PayoffVariant& pv) {
 fdmc_qmc_config to_c(const QMCConfig& q) {
     fdmc_qmc_config c{};
     c.R = q.R; c.max_doublings = q.max_doublings; c.N0 = q.N0; c.N_max = q.N_max; c.salt_base = q.salt_base;
+    c.pca_precision = q.pca_precision;
     return c;
 }
 fdmc_mc_config to_c(const MCConfig& m) { return fdmc_mc_config{m.N_batch, m.pilot_n, m.seed}; }

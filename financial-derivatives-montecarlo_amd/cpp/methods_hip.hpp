@@ -26,6 +26,7 @@ struct QMCConfig {
     long long N0 = 1LL << 14;        // additive: first points per replica (power of two)
     long long N_max = 0;             // additive: cap on points per replica (0 = 2^32)
     unsigned salt_base = 42u;        // additive: replica salt = salt_base + r (+ l R in MLQMC)
+    int pca_precision = 0;           // additive: PCA GEMM operands 0 = FP16, 1 = BF16
 };
 
 struct MLMCConfig {

@@ -98,7 +98,7 @@ struct fdmc_context {
     int n_cu = 256;
     hipStream_t stream = nullptr;
     uint32_t* d_dirv = nullptr;   // [FDMC_JK_NDIM][32]
-    DevBuf partials, sums, salts, sdir, gen_sdir;
+    DevBuf partials, sums, salts, sdir, gen_sdir, pca_z, pca_dw, basket_L;
     std::vector<uint32_t> sdir_key;  // salts + D of the cached sdir table
     int sdir_D = -1;
     double* h_sums = nullptr;     // pinned
@@ -119,6 +119,7 @@ struct fdmc_bb {
 struct fdmc_pca {
     fdmc_context* ctx;
     int m;
+    int mp;                 // m rounded up to a multiple of 128 (zero padded on the device)
     void* d_f16;
     void* d_bf16;
     std::vector<double> M;  // column-major FP64 copy
@@ -282,8 +283,14 @@ struct PassSpec {
     bool prng = false;            // pseudo-random noise (MC, CV, QMC fallback)
     uint32_t tag = 0;
     const fdmc_bb* bb = nullptr;
+    const fdmc_pca* pca = nullptr;
+    int pca_bf16 = 0;             // PCA GEMM operands: 0 FP16 (as the reference), 1 BF16
     bool greeks = false;          // price + adjoint Greeks kernel (k_greeks)
 };
+
+int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out);
+int pca_rows(fdmc_context* c, const fdmc_pca* pca, const uint32_t* sdir, int D, int R, int64_t first,
+             int64_t count, bool bf16);
 
 // Runs one pass; writes R x nacc sums into out, returns nacc.
 int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
@@ -292,7 +299,7 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     if (s.n_steps <= 0) fail(FDMC_E_INVALID, "n_steps must be positive");
     if (s.R <= 0 || s.R > 65535) fail(FDMC_E_INVALID, "R out of range");
     if (s.count <= 0) fail(FDMC_E_INVALID, "number of points must be positive");
-    if (is_basket(m)) fail(FDMC_E_UNSUPPORTED, "basket models are not supported by this pass yet");
+    if (is_basket(m)) return run_basket_pass(c, s, out);
     const int nf = noise_dim(m);
     const int64_t D64 = (int64_t)s.n_steps * nf;
     const bool use_prng = s.prng || D64 > D_MAX_SOBOL;
@@ -352,7 +359,10 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
         a.bbw = s.bb->d_w;
         fn = s.greeks ? greeks_kernel_bb(mk, pk, cv, cpk, lb) : path_kernel_bb(mk, pk, cv, cpk, lb);
     } else if (s.mode == FDMC_NOISE_PCA) {
-        fail(FDMC_E_UNSUPPORTED, "PCA noise mode is not available for this estimator yet");
+        if (nf != 1) fail(FDMC_E_UNSUPPORTED, "PCA noise needs a one-factor model (methods_cuda.cu:1702-1704)");
+        if (s.greeks) fail(FDMC_E_UNSUPPORTED, "adjoint Greeks are available with Raw and BB noise");
+        nz = NZ_ROWS;
+        fn = path_kernel_rows(mk, pk, cv, cpk, 0);
     } else {
         nz = NZ_SOBOL_RAW;
         fn = s.greeks ? greeks_kernel_raw(mk, pk, cv, cpk, 0) : path_kernel_raw(mk, pk, cv, cpk, 0);
@@ -363,6 +373,12 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
         std::vector<uint32_t> salts((size_t)s.R);
         for (int r = 0; r < s.R; r++) salts[r] = s.salt_base + (uint32_t)r;
         a.sdir = get_sdir(c, salts, D);
+    }
+    if (nz == NZ_ROWS) {
+        const int64_t bytes = (int64_t)s.R * s.count * ((D + 127) / 128 * 128) * 6;
+        if (bytes > (int64_t)16 << 30) fail(FDMC_E_RANGE, "PCA pass above 16 GiB of buffers: lower N_max or R");
+        a.ld = pca_rows(c, s.pca, a.sdir, D, s.R, s.first, s.count, s.pca_bf16 != 0);
+        a.rows = (const float*)c->pca_dw.p;
     }
 
     // Tiles of 256 points in absolute index space; a few tiles per workgroup.
@@ -386,9 +402,10 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     if (lds > 160 * 1024) fail(FDMC_E_UNSUPPORTED, "LDS footprint of this pass exceeds 160 KiB");
     if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     void* args[] = {(void*)&a};
-    if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
+    const bool time_path = c->timing && nz != NZ_ROWS;   // PCA passes time their GEMM
+    if (time_path) HIPC(hipEventRecord(c->ev0, c->stream));
     HIPC(hipLaunchKernel(fn, dim3((unsigned)nwg, (unsigned)s.R), dim3(256), args, lds, c->stream));
-    if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
+    if (time_path) HIPC(hipEventRecord(c->ev1, c->stream));
     HIPC(launch_reduce_partials(a.partials, s.R, (int)nwg, nacc, d_sums, c->stream));
     ensure_host_sums(c, (size_t)s.R * nacc);
     HIPC(hipMemcpyAsync(c->h_sums, d_sums, (size_t)s.R * nacc * sizeof(double), hipMemcpyDeviceToHost,
@@ -402,6 +419,142 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     }
     out.assign(c->h_sums, c->h_sums + (size_t)s.R * nacc);
     return nacc;
+}
+
+// PCA noise for R replicas, points [first, first + count): Z (FP16/BF16,
+// point-major) then dW = Z M^T on MFMA into c->pca_dw rows r * count + p.
+// Returns the row stride (D rounded up to 128).
+int pca_rows(fdmc_context* c, const fdmc_pca* pca, const uint32_t* sdir, int D, int R, int64_t first,
+             int64_t count, bool bf16) {
+    if (!pca || pca->m != D) fail(FDMC_E_INVALID, "PCA data must have m == n_steps * noise dimension");
+    const int ld = pca->mp;
+    const int64_t rows = (int64_t)R * count, mrows = (rows + 127) / 128 * 128;
+    if (mrows > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many PCA rows in one pass");
+    void* z = c->pca_z.ensure((size_t)mrows * ld * 2);
+    float* dw = (float*)c->pca_dw.ensure((size_t)mrows * ld * sizeof(float));
+    HIPC(launch_gen_z(sdir, D, ld, R, first, count, bf16, z, c->stream));
+    if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
+    HIPC(launch_gemm_nt(z, bf16 ? pca->d_bf16 : pca->d_f16, dw, (int)mrows, ld, ld, bf16, c->stream));
+    if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
+    return ld;
+}
+
+int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
+    const fdmc_model& m = *s.model;
+    const fdmc_payoff& p = *s.payoff;
+    const int n = m.n_assets;
+    if (n > BASKET_MAX) fail(FDMC_E_UNSUPPORTED, "baskets of more than 32 assets are not supported");
+    if (p.kind != FDMC_PAYOFF_BASKET && p.kind != FDMC_PAYOFF_EUROPEAN)
+        fail(FDMC_E_UNSUPPORTED, "basket models take a Basket (or European) payoff");
+    if (s.cmodel) fail(FDMC_E_UNSUPPORTED, "control variates are not available for baskets");
+    if (s.greeks && m.kind != FDMC_MODEL_MULTI_GBM)
+        fail(FDMC_E_UNSUPPORTED, "basket deltas need GBM assets");
+    int na = 4;
+    while (na < n) na <<= 1;
+    const int64_t D64 = (int64_t)s.n_steps * n;
+    const bool prng = s.prng || D64 > D_MAX_SOBOL;
+    const int D = (int)D64;
+    BasketLaunch b{};
+    b.n = n;
+    b.n_steps = s.n_steps;
+    b.D = D;
+    b.greeks = s.greeks ? 1 : 0;
+    const float T = (float)m.T;
+    b.h = T / (float)s.n_steps;
+    b.sqrt_h = std::sqrt(b.h);
+    b.mu = (float)m.mu;
+    b.K = (float)p.K;
+    b.discount = (float)std::exp(-p.r * p.T);
+    b.sigma0 = (float)m.sigma0;
+    b.alpha = (float)m.alpha;
+    b.beta_m1 = (float)m.beta_d - 1.0f;
+    b.inv_S0 = 1.0f / (float)m.S0_assets[0];
+    for (int a = 0; a < n; a++) {
+        b.S0[a] = (float)m.S0_assets[a];
+        b.sig[a] = m.sigma_assets ? (float)m.sigma_assets[a] : 0.0f;
+        b.w[a] = p.weights ? (float)p.weights[a] : 1.0f / (float)n;
+    }
+    const int mk = m.kind == FDMC_MODEL_MULTI_GBM ? MK_MULTI_GBM : MK_MULTI_DUPIRE;
+    int src;
+    if (prng) {
+        if (n % 4) fail(FDMC_E_UNSUPPORTED, "pseudo-random baskets need a multiple of 4 assets");
+        src = SRC_PRNG;
+        b.seed = s.salt_base;
+        b.tag = s.prng ? s.tag : TAG_QMC_FALLBACK;
+    } else if (s.mode == FDMC_NOISE_PCA) {
+        src = SRC_ROWS;
+    } else if (s.mode == FDMC_NOISE_RAW) {
+        src = SRC_SOBOL;
+    } else {
+        fail(FDMC_E_UNSUPPORTED, "baskets support Raw and PCA noise (the bridge is one-factor)");
+    }
+    if (src != SRC_ROWS && !m.uncorrelated && m.L_assets) {
+        std::vector<float> L((size_t)n * n);
+        for (int i = 0; i < n * n; i++) L[i] = (float)m.L_assets[i];
+        c->basket_L.ensure(L.size() * sizeof(float));
+        HIPC(hipMemcpyAsync(c->basket_L.p, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        b.L = (const float*)c->basket_L.p;
+    }
+    const void* fn = basket_kernel(mk, src, na);
+    if (!fn) fail(FDMC_E_UNSUPPORTED, "basket kernel not available");
+    const uint32_t* sdir = nullptr;
+    if (src != SRC_PRNG) {
+        std::vector<uint32_t> salts((size_t)s.R);
+        for (int r = 0; r < s.R; r++) salts[r] = s.salt_base + (uint32_t)r;
+        sdir = get_sdir(c, salts, D);
+    }
+    const int64_t t_first = s.first >> 8, t_last = (s.first + s.count - 1) >> 8;
+    const int64_t ntiles = t_last - t_first + 1;
+    const int64_t per_rep = std::max<int64_t>(1, (int64_t)c->n_cu * 32 / s.R);
+    const int64_t tpw = std::max<int64_t>(1, (ntiles + per_rep - 1) / per_rep);
+    const int64_t nwg = (ntiles + tpw - 1) / tpw;
+    const int nacc = 2 + 2 * na;
+    b.first = s.first;
+    b.count = s.count;
+    b.tiles_per_wg = (int)tpw;
+    b.nwg_x = (int)nwg;
+    double* part = (double*)c->partials.ensure((size_t)s.R * nwg * nacc * sizeof(double));
+    double* d_sums = (double*)c->sums.ensure((size_t)s.R * nacc * sizeof(double));
+    const size_t lds = basket_lds_bytes(src, D, na);
+    if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // replicas in groups so that the PCA buffers stay below ~8 GiB
+    int group = s.R;
+    if (src == SRC_ROWS) {
+        const double per = (double)s.count * (D + 128) * 6.0;
+        group = (int)std::max(1.0, std::min((double)s.R, 8.0 * (1 << 30) / per));
+    }
+    for (int r0 = 0; r0 < s.R; r0 += group) {
+        const int g = std::min(group, s.R - r0);
+        BasketLaunch bg = b;
+        bg.partials = part + (size_t)r0 * nwg * nacc;
+        if (src == SRC_ROWS) {
+            bg.ld = pca_rows(c, s.pca, sdir + (size_t)r0 * D * SDIR_WORDS, D, g, s.first, s.count, s.pca_bf16 != 0);
+            bg.dW = (const float*)c->pca_dw.p;
+        } else {
+            bg.sdir = sdir ? sdir + (size_t)r0 * D * SDIR_WORDS : nullptr;
+            bg.seed = b.seed + (uint32_t)r0;
+        }
+        void* args[] = {(void*)&bg};
+        HIPC(hipLaunchKernel(fn, dim3((unsigned)nwg, (unsigned)g), dim3(256), args, lds, c->stream));
+        if (src == SRC_ROWS && c->timing) {
+            HIPC(hipStreamSynchronize(c->stream));
+            float ms = 0.0f;
+            HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->timed_ms += ms;
+            c->timed_launches++;
+        }
+    }
+    HIPC(launch_reduce_partials(part, s.R, (int)nwg, nacc, d_sums, c->stream));
+    ensure_host_sums(c, (size_t)s.R * nacc);
+    HIPC(hipMemcpyAsync(c->h_sums, d_sums, (size_t)s.R * nacc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    // compact to 2 + 2 n accumulators per replica
+    const int nout = 2 + 2 * n;
+    out.assign((size_t)s.R * nout, 0.0);
+    for (int r = 0; r < s.R; r++)
+        for (int i = 0; i < nout; i++) out[(size_t)r * nout + i] = c->h_sums[(size_t)r * nacc + i];
+    return nout;
 }
 
 void mean_var(const std::vector<double>& v, double& m, double& var) {
@@ -884,6 +1037,9 @@ fdmc_status fdmc_context_destroy(fdmc_context* c) {
     c->salts.release();
     c->sdir.release();
     c->gen_sdir.release();
+    c->pca_z.release();
+    c->pca_dw.release();
+    c->basket_L.release();
     if (c->d_dirv) (void)hipFree(c->d_dirv);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -975,18 +1131,24 @@ fdmc_status fdmc_pca_upload(fdmc_context* c, int32_t m, const double* M, fdmc_pc
     return guarded([&] {
         if (!c || !M || !out) fail(FDMC_E_INVALID, "null argument");
         if (m <= 0) fail(FDMC_E_INVALID, "PCA m must be positive");
-        const size_t n = (size_t)m * m;
-        std::vector<uint16_t> f16(n), bf16(n);
-        for (size_t i = 0; i < n; i++) {
-            const _Float16 h = (_Float16)M[i];
-            std::memcpy(&f16[i], &h, 2);
-            const float f = (float)M[i];
-            uint32_t u;
-            std::memcpy(&u, &f, 4);
-            u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-            bf16[i] = (uint16_t)(u >> 16);
-        }
-        auto* p = new fdmc_pca{c, m, nullptr, nullptr, std::vector<double>(M, M + n)};
+        const int mp = (m + 127) / 128 * 128;
+        const size_t n = (size_t)mp * mp;
+        std::vector<uint16_t> f16(n, 0), bf16(n, 0);
+        // device copies are row-major (M[i][j] at i*mp + j), zero padded to mp:
+        // the GEMM's Bt operand
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+                const double v = M[(size_t)j * m + i];
+                const size_t o = (size_t)i * mp + j;
+                const _Float16 h = (_Float16)v;
+                std::memcpy(&f16[o], &h, 2);
+                const float f = (float)v;
+                uint32_t u;
+                std::memcpy(&u, &f, 4);
+                u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+                bf16[o] = (uint16_t)(u >> 16);
+            }
+        auto* p = new fdmc_pca{c, m, mp, nullptr, nullptr, std::vector<double>(M, M + (size_t)m * m)};
         if (hipMalloc(&p->d_f16, n * 2) != hipSuccess || hipMalloc(&p->d_bf16, n * 2) != hipSuccess) {
             if (p->d_f16) (void)hipFree(p->d_f16);
             delete p;
@@ -1012,6 +1174,8 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* c, const fdmc_model* model, const 
                                 double beta, int32_t n_steps, int32_t R, uint32_t salt_base,
                                 int64_t offset, int64_t n_points, int32_t mode, const fdmc_bb* bb,
                                 const fdmc_pca* pca, double* sums_out, int32_t* n_acc_out) {
+    const bool pca_precision_bf16 = (mode & FDMC_PCA_BF16) != 0;
+    mode &= ~FDMC_PCA_BF16;
     return guarded([&] {
         if (!c || !sums_out) fail(FDMC_E_INVALID, "null argument");
         validate_model(model);
@@ -1020,11 +1184,11 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* c, const fdmc_model* model, const 
             fail(FDMC_E_INVALID, "control model and payoff must both be given");
         if (cmodel) { validate_model(cmodel); validate_payoff(cpayoff); }
         if (offset < 0) fail(FDMC_E_INVALID, "offset must be non-negative");
-        (void)pca;
         PassSpec s;
         s.model = model; s.payoff = payoff; s.cmodel = cmodel; s.cpayoff = cpayoff;
         s.E_ctrl = E_ctrl; s.beta = beta; s.n_steps = n_steps; s.R = R; s.salt_base = salt_base;
-        s.first = offset; s.count = n_points; s.mode = mode; s.bb = bb;
+        s.first = offset; s.count = n_points; s.mode = mode; s.bb = bb; s.pca = pca;
+        s.pca_bf16 = (mode == FDMC_NOISE_PCA && pca_precision_bf16) ? 1 : 0;
         std::vector<double> v;
         const int nacc = run_pass(c, s, v);
         std::memcpy(sums_out, v.data(), v.size() * sizeof(double));
@@ -1077,6 +1241,29 @@ fdmc_status fdmc_run_qmc_greeks(fdmc_context* c, const fdmc_model* model, const 
     });
 }
 
+fdmc_status fdmc_pca_transform(fdmc_context* c, const fdmc_pca* pca, uint32_t salt, int64_t offset,
+                               int64_t n_points, int32_t bf16, float* d_out) {
+    return guarded([&] {
+        if (!c || !pca || !d_out) fail(FDMC_E_INVALID, "null argument");
+        if (n_points <= 0 || offset < 0 || offset + n_points > (int64_t)4294967296LL)
+            fail(FDMC_E_RANGE, "point range outside the 32-bit Sobol index");
+        const int D = pca->m;
+        if (D > D_MAX_SOBOL) fail(FDMC_E_RANGE, "PCA dimension above 20000");
+        std::vector<uint32_t> salts{salt};
+        const uint32_t* sdir = get_sdir(c, salts, D);
+        const int ld = pca_rows(c, pca, sdir, D, 1, offset, n_points, bf16 != 0);
+        HIPC(hipMemcpy2DAsync(d_out, (size_t)D * sizeof(float), c->pca_dw.p, (size_t)ld * sizeof(float),
+                              (size_t)D * sizeof(float), (size_t)n_points, hipMemcpyDeviceToDevice, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        if (c->timing) {
+            float ms = 0.0f;
+            HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->timed_ms += ms;
+            c->timed_launches++;
+        }
+    });
+}
+
 fdmc_status fdmc_run_qmc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double eps,
                          int32_t n_steps, const fdmc_qmc_config* cfg, int32_t mode, const fdmc_bb* bb,
                          const fdmc_pca* pca, fdmc_result* out) {
@@ -1086,9 +1273,9 @@ fdmc_status fdmc_run_qmc(fdmc_context* c, const fdmc_model* model, const fdmc_pa
         validate_payoff(payoff);
         if (mode == FDMC_NOISE_BB && noise_dim(*model) != 1)
             fail(FDMC_E_UNSUPPORTED, "Brownian bridge needs a one-factor model");
-        (void)pca;
         PassSpec s;
-        s.model = model; s.payoff = payoff; s.n_steps = n_steps; s.mode = mode; s.bb = bb;
+        s.model = model; s.payoff = payoff; s.n_steps = n_steps; s.mode = mode; s.bb = bb; s.pca = pca;
+        s.pca_bf16 = cfg->pca_precision;
         qmc_loop(c, s, eps, *cfg, out);
     });
 }

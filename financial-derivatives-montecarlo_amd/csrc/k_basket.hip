@@ -1,0 +1,175 @@
+// k_basket.hip -- multi-asset basket paths (reference kernel_multi_dupire,
+// methods_cuda.cu:526-551, generalised to per-asset GBM and weights).
+//
+// One lane = one path with all NA asset prices in registers, stepped in time
+// order (the reference loops assets outside, steps inside, re-reading dW with a
+// stride of N paths).  Increments come from
+//   SRC_ROWS : dW rows [row][k * n + a] produced by the PCA GEMM (FP32), or
+//   SRC_SOBOL/SRC_PRNG : normals generated in the lane, scaled by sqrt(h) and
+//              correlated with the lower Cholesky factor L (dW_k = L z_k sqrt h,
+//              the batched GEMM of correlate_multi_dupire, :1133-1148).
+// Payoff max(sum_a w_a S_T^a - K, 0) e^{-rT}.  For GBM assets the pathwise
+// adjoint of the Euler scheme gives dS_T^a / dS0^a = S_T^a / S0^a, hence the
+// per-asset deltas disc 1{B > K} w_a S_T^a / S0^a accumulated beside the price.
+#include "k_path.inl"
+
+namespace fdmc {
+
+template <int NA, int MK, int SRC>
+__global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
+    constexpr int NACC = 2 + 2 * NA;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = blockIdx.y;
+    const int n = b.n, D = b.D;
+    const bool lds_mode = (SRC == SRC_SOBOL) && D <= SOBOL_LDS_MAX_D;
+    uint32_t* rec = (uint32_t*)smem;
+    double* red = (double*)(smem + ((size_t)(lds_mode ? D * 20 : 0) * 4 + 15) / 16 * 16);
+    const uint32_t* grec = (SRC == SRC_SOBOL) ? b.sdir + (int64_t)r * D * SDIR_WORDS : nullptr;
+    SobolLane sl;
+    {
+        const uint32_t g = (uint32_t)(lane ^ (lane >> 1));
+        sl.rec = rec; sl.grec = grec; sl.ga = g & 7u; sl.gb = g >> 3; sl.w = (uint32_t)(tid >> 6);
+        sl.hi_gray = 0; sl.lds_mode = lds_mode;
+    }
+    if (SRC == SRC_SOBOL && lds_mode) {
+        for (int i = tid; i < D * 16; i += 256) {
+            const int d = i >> 4, k = i & 15;
+            rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
+        }
+    }
+    double acc[NACC];
+#pragma unroll
+    for (int i = 0; i < NACC; i++) acc[i] = 0.0;
+    const int64_t last = b.first + b.count;
+    const int64_t tile0 = (b.first >> 8) + (int64_t)blockIdx.x * b.tiles_per_wg;
+    for (int it = 0; it < b.tiles_per_wg; it++) {
+        const int64_t base = (tile0 + it) << 8;
+        if (base >= last) break;
+        if constexpr (SRC == SRC_SOBOL) {
+            const uint32_t b32 = (uint32_t)base, gbase = b32 ^ (b32 >> 1);
+            if (lds_mode) {
+                __syncthreads();
+                for (int d = tid; d < D; d += 256) {
+                    const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+                    uint32_t s = 0;
+                    for (uint32_t x = gbase; x; x &= x - 1u) s ^= gr[32 + __builtin_ctz(x)];
+                    rec[d * 20 + 16] = s ^ gr[16];
+                    rec[d * 20 + 17] = s ^ gr[17];
+                    rec[d * 20 + 18] = s ^ gr[18];
+                    rec[d * 20 + 19] = s ^ gr[19];
+                }
+                __syncthreads();
+            } else {
+                const uint32_t hi = b32 + (sl.w << 6);
+                sl.hi_gray = hi ^ (hi >> 1);
+            }
+        }
+        const int64_t idx = base + tid;
+        const bool valid = idx >= b.first && idx < last;
+        const int64_t row = (int64_t)r * b.count + (valid ? idx - b.first : 0);
+        float S[NA];
+#pragma unroll
+        for (int a = 0; a < NA; a++) S[a] = b.S0[a];
+        for (int k = 0; k < b.n_steps; k++) {
+            float dw[NA];
+            if constexpr (SRC == SRC_ROWS) {
+                const float* src = b.dW + row * b.ld + (int64_t)k * n;
+#pragma unroll
+                for (int a = 0; a < NA; a++) dw[a] = (a < n) ? src[a] : 0.0f;
+            } else {
+                float z[NA];
+                if constexpr (SRC == SRC_SOBOL) {
+#pragma unroll
+                    for (int a = 0; a < NA; a++) z[a] = (a < n) ? icdf_u32(sobol_coord(sl, k * n + a)) * b.sqrt_h : 0.0f;
+                } else {
+#pragma unroll
+                    for (int a0 = 0; a0 < NA; a0 += 4) {
+                        float q[4];
+                        const int j = k * n + a0;   // normal index; blocks of 4 (n % 4 == 0 checked)
+                        prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j >> 2), q);
+#pragma unroll
+                        for (int c = 0; c < 4; c++) z[a0 + c] = (a0 + c < n) ? q[c] * b.sqrt_h : 0.0f;
+                    }
+                }
+                if (b.L) {
+#pragma unroll
+                    for (int a = 0; a < NA; a++) {
+                        float s = 0.0f;
+#pragma unroll
+                        for (int c = 0; c <= a; c++) s += b.L[a * n + c] * z[c];
+                        dw[a] = (a < n) ? s : 0.0f;
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < NA; a++) dw[a] = z[a];
+                }
+            }
+            const float t = (float)k * b.h;
+#pragma unroll
+            for (int a = 0; a < NA; a++) {
+                if constexpr (MK == MK_MULTI_GBM) {
+                    S[a] = S[a] + b.mu * S[a] * b.h + b.sig[a] * S[a] * dw[a];
+                } else {
+                    const float sl_ = b.sigma0 * fast_exp(-b.alpha * t) * fast_exp2(b.beta_m1 * fast_log2(S[a] * b.inv_S0));
+                    S[a] = S[a] + b.mu * S[a] * b.h + sl_ * S[a] * dw[a];
+                }
+            }
+        }
+        float B = 0.0f;
+#pragma unroll
+        for (int a = 0; a < NA; a++) B += (a < n) ? b.w[a] * S[a] : 0.0f;
+        const float y = fmaxf(B - b.K, 0.0f) * b.discount;
+        if (valid) {
+            acc[0] += (double)y;
+            acc[1] += (double)y * (double)y;
+            if (b.greeks && MK == MK_MULTI_GBM) {
+                const float ind = (B > b.K) ? b.discount : 0.0f;
+#pragma unroll
+                for (int a = 0; a < NA; a++) {
+                    const double g = (double)(ind * b.w[a] * S[a] / b.S0[a]);
+                    acc[2 + 2 * a] += g;
+                    acc[3 + 2 * a] += g * g;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    block_sum<NACC, 4>(acc, red);
+    if (tid == 0) {
+        double* out = b.partials + ((int64_t)r * b.nwg_x + blockIdx.x) * NACC;
+#pragma unroll
+        for (int i = 0; i < NACC; i++) out[i] = acc[i];
+    }
+}
+
+template <int MK, int SRC>
+static const void* pick_na(int na) {
+    switch (na) {
+    case 4: return (const void*)&k_basket<4, MK, SRC>;
+    case 8: return (const void*)&k_basket<8, MK, SRC>;
+    case 16: return (const void*)&k_basket<16, MK, SRC>;
+    case 32: return (const void*)&k_basket<32, MK, SRC>;
+    default: return nullptr;
+    }
+}
+
+const void* basket_kernel(int mk, int src, int na) {
+    if (mk == MK_MULTI_GBM) {
+        if (src == SRC_ROWS) return pick_na<MK_MULTI_GBM, SRC_ROWS>(na);
+        if (src == SRC_SOBOL) return pick_na<MK_MULTI_GBM, SRC_SOBOL>(na);
+        return pick_na<MK_MULTI_GBM, SRC_PRNG>(na);
+    }
+    if (src == SRC_ROWS) return pick_na<MK_MULTI_DUPIRE, SRC_ROWS>(na);
+    if (src == SRC_SOBOL) return pick_na<MK_MULTI_DUPIRE, SRC_SOBOL>(na);
+    return pick_na<MK_MULTI_DUPIRE, SRC_PRNG>(na);
+}
+
+size_t basket_lds_bytes(int src, int D, int na) {
+    return ((size_t)((src == SRC_SOBOL && D <= SOBOL_LDS_MAX_D) ? D * 20 : 0) * 4 + 15) / 16 * 16 +
+           4 * (2 + 2 * na) * sizeof(double);
+}
+
+size_t basket_launch_size() { return sizeof(BasketLaunch); }
+
+}  // namespace fdmc

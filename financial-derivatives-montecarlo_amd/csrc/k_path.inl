@@ -100,14 +100,15 @@ __global__ __launch_bounds__(256) void k_path(PathLaunch a) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = blockIdx.y;
     const int D = a.D;
-    const bool lds_mode = (NZ != NZ_PRNG) && D <= SOBOL_LDS_MAX_D;
+    constexpr bool SOBOL = (NZ == NZ_SOBOL_RAW || NZ == NZ_SOBOL_BB);
+    const bool lds_mode = SOBOL && D <= SOBOL_LDS_MAX_D;
     uint32_t* rec = (uint32_t*)smem;
-    const int rec_words = (NZ != NZ_PRNG && lds_mode) ? D * 20 : 0;
+    const int rec_words = lds_mode ? D * 20 : 0;
     float* skel = (float*)(smem + (size_t)rec_words * 4);
     const int skel_words = (NZ == NZ_SOBOL_BB) ? (a.G + 1) * 256 : 0;
     double* red = (double*)(smem + (((size_t)(rec_words + skel_words) * 4 + 15) & ~(size_t)15));
 
-    const uint32_t* grec = (NZ != NZ_PRNG) ? a.sdir + (int64_t)r * D * SDIR_WORDS : nullptr;
+    const uint32_t* grec = SOBOL ? a.sdir + (int64_t)r * D * SDIR_WORDS : nullptr;
     SobolLane sl;
     {
         const uint32_t g = (uint32_t)(lane ^ (lane >> 1));
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(256) void k_path(PathLaunch a) {
         sl.hi_gray = 0;
         sl.lds_mode = lds_mode;
     }
-    if (NZ != NZ_PRNG && lds_mode) {
+    if (SOBOL && lds_mode) {
         for (int i = tid; i < D * 16; i += 256) {
             const int d = i >> 4, k = i & 15;
             rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(256) void k_path(PathLaunch a) {
     for (int it = 0; it < a.tiles_per_wg; it++) {
         const int64_t base = (tile0 + it) << 8;
         if (base >= last) break;
-        if constexpr (NZ != NZ_PRNG) {
+        if constexpr (SOBOL) {
             const uint32_t b32 = (uint32_t)base;
             const uint32_t gbase = b32 ^ (b32 >> 1);
             if (lds_mode) {
@@ -161,7 +162,11 @@ __global__ __launch_bounds__(256) void k_path(PathLaunch a) {
         PathState<MK, PK, CV, CPK> ps;
         ps.init(a);
 
-        if constexpr (NZ == NZ_PRNG) {
+        if constexpr (NZ == NZ_ROWS) {
+            // increments from the PCA GEMM: row r * count + (idx - first)
+            const float* src = a.rows + ((int64_t)r * a.count + (valid ? idx - a.first : 0)) * a.ld;
+            for (int k = 0; k < a.n_steps; k++) ps.step(a, src[k], k);
+        } else if constexpr (NZ == NZ_PRNG) {
             // Philox normals, 4 per call; normal n = k * nf + c.
             const int nn = a.n_steps * a.nf;
             float z[4];

@@ -1,0 +1,222 @@
+// k_pca.hip -- PCA noise transform on gfx950 matrix cores.
+//
+// The reference casts Z to FP16 in one kernel and calls cublasGemmEx (FP16 x
+// FP16 -> FP32, tensor cores) for dW = Z M^T (methods_cuda.cu:721-724,
+// 1151-1166).  Here:
+//   k_gen_z   : scrambled-Sobol normals of R replicas written point-major
+//               (row = replica * n + point, D contiguous coordinates) directly in
+//               BF16 or FP16 -- the GEMM's A operand, no FP32 round trip;
+//   k_gemm_nt : C[m][n] = sum_k A[m][k] Bt[n][k] with v_mfma_f32_32x32x16_{bf16,f16},
+//               FP32 accumulation.  Bt is M itself stored row-major ([D][D],
+//               K contiguous), so both operands stream along K.
+//               128 x 128 workgroup tiles, 4 waves of 64 x 64, BK = 32, LDS
+//               double buffering; tiles are assigned so the 8 column tiles of
+//               one row block run on the same XCD (its L2 keeps the A rows).
+#include "k_path.inl"
+
+namespace fdmc {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <typename T>
+__device__ __forceinline__ T to_half(float x);
+template <>
+__device__ __forceinline__ __bf16 to_half<__bf16>(float x) { return (__bf16)x; }
+template <>
+__device__ __forceinline__ _Float16 to_half<_Float16>(float x) { return (_Float16)x; }
+
+// ---- normals, point-major ----------------------------------------------------
+// Grid (tiles, R); 256 lanes = 256 consecutive points; dims in chunks of 64
+// staged through LDS so that global stores are 16-byte and row-contiguous.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir, int D, int ld, int64_t first,
+                                               int64_t count, T* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = blockIdx.y;
+    const bool lds_mode = D <= SOBOL_LDS_MAX_D;
+    uint32_t* rec = (uint32_t*)smem;
+    T* tile = (T*)(smem + (size_t)(lds_mode ? D * 20 : 0) * 4);   // [256][64 + 8]
+    constexpr int TW = 72;
+    const uint32_t* grec = sdir + (int64_t)r * D * SDIR_WORDS;
+    SobolLane sl;
+    const uint32_t g = (uint32_t)(lane ^ (lane >> 1));
+    sl.rec = rec; sl.grec = grec; sl.ga = g & 7u; sl.gb = g >> 3; sl.w = (uint32_t)(tid >> 6);
+    sl.hi_gray = 0; sl.lds_mode = lds_mode;
+    const int64_t base = ((first >> 8) + blockIdx.x) << 8;
+    const uint32_t b32 = (uint32_t)base, gbase = b32 ^ (b32 >> 1);
+    if (lds_mode) {
+        for (int i = tid; i < D * 16; i += 256) {
+            const int d = i >> 4, k = i & 15;
+            rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
+        }
+        for (int d = tid; d < D; d += 256) {
+            const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+            uint32_t s = 0;
+            for (uint32_t x = gbase; x; x &= x - 1u) s ^= gr[32 + __builtin_ctz(x)];
+            rec[d * 20 + 16] = s ^ gr[16];
+            rec[d * 20 + 17] = s ^ gr[17];
+            rec[d * 20 + 18] = s ^ gr[18];
+            rec[d * 20 + 19] = s ^ gr[19];
+        }
+        __syncthreads();
+    } else {
+        const uint32_t hi = b32 + (sl.w << 6);
+        sl.hi_gray = hi ^ (hi >> 1);
+    }
+    for (int d0 = 0; d0 < ld; d0 += 64) {
+        for (int j = 0; j < 64; j++) {
+            const float z = (d0 + j < D) ? icdf_u32(sobol_coord(sl, d0 + j)) : 0.0f;
+            tile[tid * TW + j] = to_half<T>(z);
+        }
+        __syncthreads();
+        // 256 rows x 64 values = 2048 chunks of 8 values
+        for (int c = tid; c < 2048; c += 256) {
+            const int row = c >> 3, q = (c & 7) * 8;
+            const int64_t p = base + row;
+            if (p >= first && p < first + count && d0 + q < ld) {
+                const int64_t orow = (int64_t)r * count + (p - first);
+                const uint4 v = *(const uint4*)(tile + row * TW + q);
+                *(uint4*)(out + orow * ld + d0 + q) = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- NT GEMM on MFMA ----------------------------------------------------------
+constexpr int GB_M = 128, GB_N = 128, GB_K = 32, G_PAD = 8, G_LD = GB_K + G_PAD;
+
+template <typename T>
+__device__ __forceinline__ f32x16 mfma32(const T* a, const T* b, f32x16 c);
+template <>
+__device__ __forceinline__ f32x16 mfma32<__bf16>(const __bf16* a, const __bf16* b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*(const bf16x8*)a, *(const bf16x8*)b, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ f32x16 mfma32<_Float16>(const _Float16* a, const _Float16* b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(*(const f16x8*)a, *(const f16x8*)b, c, 0, 0, 0);
+}
+
+// Requires Mrows % 128 == 0, N % 128 == 0, K % 32 == 0 (checked on the host).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_nt(const T* __restrict__ A, const T* __restrict__ Bt,
+                                                 float* __restrict__ C, int Mrows, int N, int K) {
+    __shared__ __align__(16) T As[2][GB_M * G_LD];
+    __shared__ __align__(16) T Bs[2][GB_N * G_LD];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // XCD-aware tile order: workgroup i runs on XCD i % 8; give each XCD a
+    // contiguous range of tiles, column tiles fastest.
+    const int ntn = N / GB_N, ntm = Mrows / GB_M, ntiles = ntn * ntm;
+    const int id = blockIdx.x;
+    const int per = (ntiles + 7) / 8;
+    const int t = (id % 8) * per + id / 8;
+    if (t >= ntiles) return;
+    const int m0 = (t / ntn) * GB_M, n0 = (t % ntn) * GB_N;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+
+    // global -> LDS: each operand tile is 128 rows x 32 k = 512 chunks of 8
+    uint4 ra[2], rb[2];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int c = tid + 256 * s, row = c >> 2, q = (c & 3) * 8;
+            ra[s] = *(const uint4*)(A + (int64_t)(m0 + row) * K + k0 + q);
+            rb[s] = *(const uint4*)(Bt + (int64_t)(n0 + row) * K + k0 + q);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int c = tid + 256 * s, row = c >> 2, q = (c & 3) * 8;
+            *(uint4*)(&As[buf][row * G_LD + q]) = ra[s];
+            *(uint4*)(&Bs[buf][row * G_LD + q]) = rb[s];
+        }
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int nk = K / GB_K;
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    for (int kt = 0; kt < nk; kt++) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * GB_K);
+#pragma unroll
+        for (int kk = 0; kk < GB_K; kk += 16) {
+            const T* a0 = &As[buf][(wm + lr) * G_LD + kk + lk];
+            const T* a1 = &As[buf][(wm + 32 + lr) * G_LD + kk + lk];
+            const T* b0 = &Bs[buf][(wn + lr) * G_LD + kk + lk];
+            const T* b1 = &Bs[buf][(wn + 32 + lr) * G_LD + kk + lk];
+            acc[0][0] = mfma32<T>(a0, b0, acc[0][0]);
+            acc[0][1] = mfma32<T>(a0, b1, acc[0][1]);
+            acc[1][0] = mfma32<T>(a1, b0, acc[1][0]);
+            acc[1][1] = mfma32<T>(a1, b1, acc[1][1]);
+        }
+        if (kt + 1 < nk) {
+            lstore(buf ^ 1);
+        }
+        __syncthreads();
+    }
+    // D layout of 32x32 MFMA: lane l holds column l % 32, element e at row
+    // (e / 4) * 8 + (l / 32) * 4 + e % 4.
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                const int col = n0 + wn + j * 32 + lr;
+                C[(int64_t)row * N + col] = acc[i][j][e];
+            }
+}
+
+// ---- host launchers ------------------------------------------------------------
+size_t gen_z_lds_bytes(int D) { return (size_t)(D <= SOBOL_LDS_MAX_D ? D * 20 * 4 : 0) + 256 * 72 * 2; }
+
+hipError_t launch_gen_z(const uint32_t* sdir, int D, int ld, int R, int64_t first, int64_t count, bool bf16,
+                        void* out, hipStream_t st) {
+    if (ld % 64 || ld < D) return hipErrorInvalidValue;
+    const int64_t tiles = ((first + count - 1) >> 8) - (first >> 8) + 1;
+    if (tiles > 0x7FFFFFFF || R > 65535) return hipErrorInvalidValue;
+    const size_t lds = gen_z_lds_bytes(D);
+    const void* fn = bf16 ? (const void*)&k_gen_z<__bf16> : (const void*)&k_gen_z<_Float16>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (bf16)
+        hipLaunchKernelGGL(k_gen_z<__bf16>, dim3((unsigned)tiles, R), dim3(256), lds, st, sdir, D, ld, first,
+                           count, (__bf16*)out);
+    else
+        hipLaunchKernelGGL(k_gen_z<_Float16>, dim3((unsigned)tiles, R), dim3(256), lds, st, sdir, D, ld, first,
+                           count, (_Float16*)out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_nt(const void* A, const void* Bt, float* C, int Mrows, int N, int K, bool bf16,
+                          hipStream_t st) {
+    if (Mrows % GB_M || N % GB_N || K % GB_K) return hipErrorInvalidValue;
+    const int64_t tiles = (int64_t)(Mrows / GB_M) * (N / GB_N);
+    const int64_t grid = (tiles + 7) / 8 * 8;
+    if (grid > 0x7FFFFFFF) return hipErrorInvalidValue;
+    if (bf16)
+        hipLaunchKernelGGL(k_gemm_nt<__bf16>, dim3((unsigned)grid), dim3(256), 0, st, (const __bf16*)A,
+                           (const __bf16*)Bt, C, Mrows, N, K);
+    else
+        hipLaunchKernelGGL(k_gemm_nt<_Float16>, dim3((unsigned)grid), dim3(256), 0, st, (const _Float16*)A,
+                           (const _Float16*)Bt, C, Mrows, N, K);
+    return hipGetLastError();
+}
+
+}  // namespace fdmc

@@ -16,7 +16,7 @@ enum ModelKind : int { MK_GBM = 0, MK_HESTON = 1, MK_DUPIRE = 2, MK_MULTI_DUPIRE
 enum PayoffKind : int { PK_EUROPEAN = 0, PK_ASIAN = 1, PK_GEOM_ASIAN = 2, PK_LOOKBACK = 3,
                         PK_BARRIER = 4, PK_BASKET = 5 };
 // Noise sources of the fused path kernels.
-enum NoiseKind : int { NZ_SOBOL_RAW = 0, NZ_SOBOL_BB = 1, NZ_PRNG = 2 };
+enum NoiseKind : int { NZ_SOBOL_RAW = 0, NZ_SOBOL_BB = 1, NZ_PRNG = 2, NZ_ROWS = 3 };
 // Control-variate leg: none, a second payoff on the same path, or a GBM leg.
 enum CtrlKind : int { CV_NONE = 0, CV_SHARED = 1, CV_GBM = 2 };
 
@@ -50,6 +50,8 @@ struct PathLaunch {
     const uint32_t* sdir;   // [R][D][SDIR_WORDS]
     const float* bbw;       // BB weights [N][4] = {wl, wr, sd, 0} in bb_precompute order
     const float* L;         // basket: n x n lower Cholesky (row-major) or null
+    const float* rows;      // NZ_ROWS: precomputed increments [R * count][ld] (PCA)
+    int ld;
     int64_t first;          // first point index (inclusive)
     int64_t count;          // points per replica
     int tiles_per_wg;       // 256-point tiles per workgroup
@@ -79,6 +81,25 @@ struct LevelsLaunch {
     double* partials;           // [total_items][4]
 };
 
+// Multi-asset basket launch (k_basket.hip).
+enum BasketSrc : int { SRC_ROWS = 0, SRC_SOBOL = 1, SRC_PRNG = 2 };
+constexpr int BASKET_MAX = 32;
+
+struct BasketLaunch {
+    int n, n_steps, D, greeks;
+    float h, sqrt_h, mu, K, discount;
+    float sigma0, alpha, beta_m1, inv_S0;     // MULTI_DUPIRE (vol normalised by the first S0)
+    float S0[BASKET_MAX], sig[BASKET_MAX], w[BASKET_MAX];
+    const float* L;           // n x n lower Cholesky (row-major) or null
+    const float* dW;          // SRC_ROWS: [R * count][ld]
+    int ld;                   // row stride of dW (D rounded up to 128)
+    const uint32_t* sdir;     // SRC_SOBOL: [R][D][SDIR_WORDS]
+    uint32_t seed, tag;       // SRC_PRNG
+    int64_t first, count;
+    int tiles_per_wg, nwg_x;
+    double* partials;         // [R][nwg_x][2 + 2 n]
+};
+
 // Sobol direction records are staged in LDS up to this dimension; above it the
 // path kernels read them from global memory (L2-resident).
 constexpr int SOBOL_LDS_MAX_D = 1024;
@@ -86,7 +107,7 @@ constexpr int SOBOL_LDS_MAX_D = 1024;
 // Dynamic LDS bytes of one k_path launch (layout documented in k_path.inl).
 inline size_t path_lds_bytes(int nz, int D, int G) {
     size_t b = 0;
-    if (nz != NZ_PRNG && D <= SOBOL_LDS_MAX_D) b += (size_t)D * 20 * 4;
+    if (nz != NZ_PRNG && nz != NZ_ROWS && D <= SOBOL_LDS_MAX_D) b += (size_t)D * 20 * 4;
     if (nz == NZ_SOBOL_BB) b += (size_t)(G + 1) * 256 * 4;
     b = (b + 15) & ~(size_t)15;
     return b + 4 * 7 * sizeof(double);

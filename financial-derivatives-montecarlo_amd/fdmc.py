@@ -20,6 +20,7 @@ MODEL_GBM, MODEL_HESTON, MODEL_DUPIRE, MODEL_MULTI_DUPIRE, MODEL_LOCALVOL_QUAD, 
 PAYOFF_EUROPEAN, PAYOFF_ASIAN, PAYOFF_GEOM_ASIAN, PAYOFF_LOOKBACK, PAYOFF_BARRIER, PAYOFF_BASKET = range(6)
 NOISE_RAW, NOISE_BB, NOISE_PCA, NOISE_PRNG = 0, 1, 2, 3
 FLAG_PRNG_FALLBACK, FLAG_NOT_CONVERGED = 1, 2
+PCA_BF16 = 0x100  # OR into the mode of qmc_accumulate
 BGK_BETA = 0.5826  # payoffs.hpp:7
 D_MAX_SOBOL = 20000  # models.hpp:9
 
@@ -53,7 +54,7 @@ class CMCConfig(C.Structure):
 
 class CQMCConfig(C.Structure):
     _fields_ = [("R", C.c_int32), ("max_doublings", C.c_int32), ("N0", C.c_int64),
-                ("N_max", C.c_int64), ("salt_base", C.c_uint32), ("reserved0", C.c_int32)]
+                ("N_max", C.c_int64), ("salt_base", C.c_uint32), ("pca_precision", C.c_int32)]
 
 
 class CMLMCConfig(C.Structure):
@@ -274,11 +275,12 @@ class MCConfig:
 
 
 class QMCConfig:
-    def __init__(self, R=32, max_doublings=20, N0=1 << 14, N_max=0, salt_base=42):
+    def __init__(self, R=32, max_doublings=20, N0=1 << 14, N_max=0, salt_base=42, pca_bf16=False):
         self.R, self.max_doublings, self.N0, self.N_max, self.salt_base = R, max_doublings, N0, N_max, salt_base
+        self.pca_bf16 = pca_bf16
 
     def to_c(self):
-        return CQMCConfig(self.R, self.max_doublings, self.N0, self.N_max, self.salt_base, 0)
+        return CQMCConfig(self.R, self.max_doublings, self.N0, self.N_max, self.salt_base, 1 if self.pca_bf16 else 0)
 
 
 class MLMCConfig:
@@ -342,6 +344,7 @@ _SIGS = {
                                       C.POINTER(CPayoff), C.c_double, C.POINTER(C.c_double), C.c_double,
                                       C.c_double, C.c_int32, C.POINTER(CQMCConfig), C.c_int32, _P,
                                       C.POINTER(CResult), C.POINTER(CGreeks)]),
+    "fdmc_pca_transform": (C.c_int, [_P, _P, C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P]),
     "fdmc_sobol_scrambled_u32": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_sobol_normal": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_icdf_u32": (C.c_int, [_P, _P, C.c_int64, _P]),
@@ -527,7 +530,7 @@ class Engine:
         m, p = model.to_c(), payoff.to_c()
         cm = C.byref(ctrl_model.to_c()) if ctrl_model is not None else None
         cp = C.byref(ctrl_payoff.to_c()) if ctrl_payoff is not None else None
-        sums = (C.c_double * (R * 7))()
+        sums = (C.c_double * (R * 66))()   # baskets with deltas: 2 + 2 n
         nacc = C.c_int32()
         _check(self.lib, self.lib.fdmc_qmc_accumulate(self.ctx, C.byref(m), C.byref(p), cm, cp, E_ctrl, beta,
                                                       n_steps, R, salt_base, offset, n_points, mode,
@@ -557,7 +560,7 @@ class Engine:
         m, p = model.to_c(), payoff.to_c()
         cm = C.byref(ctrl_model.to_c()) if ctrl_model is not None else None
         cp = C.byref(ctrl_payoff.to_c()) if ctrl_payoff is not None else None
-        sums = (C.c_double * (R * 13))()
+        sums = (C.c_double * (R * 66))()   # baskets: 2 + 2 n (n <= 32)
         nacc = C.c_int32()
         _check(self.lib, self.lib.fdmc_greeks_accumulate(self.ctx, C.byref(m), C.byref(p), cm, cp, E_ctrl, beta,
                                                          n_steps, R, salt_or_seed, offset, n_points, mode,
@@ -577,6 +580,10 @@ class Engine:
                                                       n_steps, C.byref(c), mode, dev_bb.handle if dev_bb else None,
                                                       C.byref(r), C.byref(g)))
         return _result(r), Greeks(g.delta, g.vega, g.rho, g.delta_se, g.vega_se, g.rho_se)
+
+    def pca_transform(self, dev_pca, salt, offset, n_points, bf16, d_out):
+        _check(self.lib, self.lib.fdmc_pca_transform(self.ctx, dev_pca.handle, salt, offset, n_points,
+                                                     1 if bf16 else 0, d_out))
 
     # -- generator hooks (device pointers)
     def sobol_scrambled_u32(self, salt, D, n, offset, d_out):

@@ -97,7 +97,7 @@ typedef struct fdmc_qmc_config { /* QMCConfig */
     int64_t N0;                  /* first points per replica (power of 2), default 1<<14 */
     int64_t N_max;               /* cap on points per replica (0 = 2^32) */
     uint32_t salt_base;          /* replica salt = salt_base + r, default 42 */
-    int32_t reserved0;
+    int32_t pca_precision;       /* PCA GEMM operands: 0 = FP16 (default), 1 = BF16 */
 } fdmc_qmc_config;
 
 typedef struct fdmc_mlmc_config {/* MLMCConfig */
@@ -109,6 +109,8 @@ typedef struct fdmc_mlmc_config {/* MLMCConfig */
 
 typedef enum { FDMC_NOISE_RAW = 0, FDMC_NOISE_BB = 1, FDMC_NOISE_PCA = 2,
                FDMC_NOISE_PRNG = 3 /* accumulate hooks only: Philox paths */ } fdmc_noise_mode;
+/* qmc_accumulate: OR into mode to run the PCA GEMM on BF16 instead of FP16 operands. */
+#define FDMC_PCA_BF16 0x100
 
 #define FDMC_FLAG_PRNG_FALLBACK 1u   /* QMC dimension > 20000: pseudo-random points were used */
 #define FDMC_FLAG_NOT_CONVERGED 2u   /* stopping rule not met within the configured limits */
@@ -243,7 +245,8 @@ fdmc_status fdmc_levels_accumulate(fdmc_context* ctx, const fdmc_model* model,
  * Path states are recomputed in reverse from the Sobol index (or Philox
  * counter) with S checkpoints every 16 steps; no path is stored.
  * greeks_accumulate: one pass; sums_out R x n_acc with n_acc = 2 (+5 with a
- * control) price sums followed by {sum g, sum g^2} for delta, vega, rho.
+ * control) price sums followed by {sum g, sum g^2} for delta, vega, rho;
+ * for GBM baskets n_acc = 2 + 2 n with {sum, sum^2} of each asset's delta.
  * mode: RAW, BB or FDMC_NOISE_PRNG (Philox stream seed salt_or_seed + r). */
 fdmc_status fdmc_greeks_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                    const fdmc_payoff* payoff, const fdmc_model* ctrl_model,
@@ -260,6 +263,13 @@ fdmc_status fdmc_run_qmc_greeks(fdmc_context* ctx, const fdmc_model* model,
                                 const double* dE_ctrl, double beta, double eps, int32_t n_steps,
                                 const fdmc_qmc_config* cfg, int32_t mode, const fdmc_bb* bb,
                                 fdmc_result* out, fdmc_greeks* greeks);
+
+/* PCA noise of one replica as the path kernels see it: scrambled-Sobol normals
+ * rounded to FP16 (bf16 = 0) or BF16 (bf16 = 1), times M^T on MFMA with FP32
+ * accumulation (phase1_pca, methods_cuda.cu:1151-1166).  d_out: [n_points][m]
+ * FP32 device rows; m must be a multiple of 128. */
+fdmc_status fdmc_pca_transform(fdmc_context* ctx, const fdmc_pca* pca, uint32_t salt, int64_t offset,
+                               int64_t n_points, int32_t bf16, float* d_out);
 
 /* ---- scrambled Sobol generator (methods_cuda.cu:753-844) ----
  * d_out is [D][n] (dimension-major, as kernel_gen_hh_scrambled_sobol_normal writes).

@@ -1,0 +1,134 @@
+"""PCA on matrix cores and the multi-asset basket (config 3).
+
+The MFMA GEMM is checked against an FP64 GEMM of the same FP16/BF16-rounded
+operands (only FP32 accumulation error remains, so any lane-layout mistake
+fails), and against an FP64 GEMM of the unrounded normals (the quantisation
+error the precision choice costs).  Basket prices are compared with the
+oracle's restatement of kernel_multi_dupire (methods_cuda.cu:526-551) at the
+same points; deltas with common-random-number differences.
+"""
+import numpy as np
+import pytest
+
+import fdmc
+import pyoracle as O
+from c3_setup import c3_parameters, cholesky_lower
+
+pytestmark = pytest.mark.gpu
+
+
+def _round_bf16(x):
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.fixture(scope="module")
+def c3(engine):
+    sig, C = c3_parameters()
+    M = O.pca_kron_compute(64, 1.0, C)          # D x D, dW = M z, index k * 16 + a
+    pca = engine.pca_upload(1024, M.T.ravel())   # column-major
+    yield sig, C, M, pca
+    pca.free()
+
+
+@pytest.mark.parametrize("bf16", [False, True], ids=["fp16", "bf16"])
+def test_gemm_matches_fp64_gemm(engine, torch_cuda, c3, bf16):
+    torch = torch_cuda
+    sig, C, M, pca = c3
+    n, salt, off = 4096, 42, 512
+    out = torch.empty((n, 1024), dtype=torch.float32, device="cuda")
+    engine.pca_transform(pca, salt, off, n, bf16, out.data_ptr())
+    got = out.cpu().numpy().astype(np.float64)
+    z = O.sobol_normal_block(salt, 1024, off, n).T.astype(np.float32)     # [n][D]
+    q = (lambda a: _round_bf16(a)) if bf16 else (lambda a: a.astype(np.float16).astype(np.float32))
+    exact_q = q(z).astype(np.float64) @ q(M.astype(np.float32)).astype(np.float64).T
+    err_acc = np.abs(got - exact_q).max() / np.abs(exact_q).max()
+    assert err_acc < 2e-6, err_acc                        # FP32 accumulation only
+    ref = z.astype(np.float64) @ M.T
+    rel_fro = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert rel_fro < (6e-3 if bf16 else 8e-4), rel_fro    # operand quantisation
+
+
+def _basket(sig, C, corr=True):
+    L = cholesky_lower(C).ravel() if corr else None
+    return fdmc.MultiGBMParams(16, [100.0] * 16, list(sig), 0.05, 1.0, L=L, uncorrelated=not corr)
+
+
+PAY = fdmc.Basket(100.0, 0.05, 1.0, 16)
+
+
+def test_basket_raw_cholesky_matches_oracle(engine, c3):
+    sig, C, M, pca = c3
+    m = _basket(sig, C)
+    R, n = 2, 1500
+    got = np.array(engine.qmc_accumulate(m, PAY, 64, R, 42, 100, n, fdmc.NOISE_RAW))
+    ref = O.qmc_accumulate(m, PAY, 64, R, 42, 100, n, fdmc.NOISE_RAW)
+    assert np.allclose(got[:, 0] / n, ref[:, 0] / n, rtol=2e-5, atol=1e-6)
+
+
+def test_basket_pca_matches_oracle_fp64_gemm(engine, c3):
+    sig, C, M, pca = c3
+    m = _basket(sig, C)
+    R, n = 2, 2048
+    for bf16, tol in ((False, 2e-3), (True, 1e-2)):
+        mode = fdmc.NOISE_PCA | (fdmc.PCA_BF16 if bf16 else 0)
+        got = np.array(engine.qmc_accumulate(m, PAY, 64, R, 42, 0, n, mode, dev_pca=pca))
+        ref = O.qmc_accumulate(m, PAY, 64, R, 42, 0, n, fdmc.NOISE_PCA, pca_M=M)
+        assert np.allclose(got[:, 0] / n, ref[:, 0] / n, rtol=tol), (bf16, got[:, 0] / n, ref[:, 0] / n)
+
+
+def test_basket_prng_matches_oracle(engine, c3):
+    sig, C, M, pca = c3
+    m = _basket(sig, C)
+    gm, gv = engine.run_mc_fixed(m, PAY, 64, 3000, 7)
+    rm, rv = O.run_mc_fixed(m, PAY, 64, 3000, 7)
+    assert abs(gm - rm) < 2e-5 * abs(rm) + 1e-6 and abs(gv - rv) < 1e-3 * rv
+
+
+def test_single_asset_qmc_pca_mode(engine):
+    """run_qmc_cuda PCA mode for a one-factor model (D = 64 padded to 128)."""
+    g = fdmc.GBMParams(100, 0.05, 0.2, 1.0)
+    A = fdmc.Asian(100.0)
+    M = O.pca_compute(64, 1.0)
+    pca = engine.pca_upload(64, M.T.ravel())
+    R, n = 3, 2048
+    got = np.array(engine.qmc_accumulate(g, A, 64, R, 42, 0, n, fdmc.NOISE_PCA, dev_pca=pca))
+    ref = O.qmc_accumulate(g, A, 64, R, 42, 0, n, fdmc.NOISE_PCA, pca_M=M)
+    assert np.allclose(got[:, 0] / n, ref[:, 0] / n, rtol=2e-3)
+    r = engine.run_qmc_cuda(g, A, 2e-3, 64, fdmc.QMCConfig(R=16, N0=1 << 12), fdmc.NOISE_PCA, dev_pca=pca)
+    bb = engine.bb_upload(fdmc.bb_precompute(64, 1.0))
+    rb = engine.run_qmc_cuda(g, A, 2e-3, 64, fdmc.QMCConfig(R=16, N0=1 << 12), fdmc.NOISE_BB, dev_bb=bb)
+    bb.free()
+    pca.free()
+    assert abs(r.price - rb.price) < 3 * np.hypot(r.std_error, rb.std_error) + 1e-3
+
+
+def test_c3_price_and_per_asset_deltas(engine, c3):
+    """Config 3: 2^18 points x R = 16 through the FP16 MFMA PCA; the price agrees
+    with the Cholesky/raw estimator and the per-asset deltas with CRN differences."""
+    sig, C, M, pca = c3
+    m = _basket(sig, C)
+    R, n = 16, 1 << 18
+    s = np.array(engine.qmc_accumulate(m, PAY, 64, R, 42, 0, n, fdmc.NOISE_PCA, dev_pca=pca))
+    means = s[:, 0] / n
+    p_pca, se_pca = means.mean(), means.std(ddof=1) / np.sqrt(R)
+    s2 = np.array(engine.qmc_accumulate(m, PAY, 64, R, 4242, 0, n, fdmc.NOISE_RAW))
+    m2 = s2[:, 0] / n
+    assert abs(p_pca - m2.mean()) < 4 * np.hypot(se_pca, m2.std(ddof=1) / np.sqrt(R)) + 2e-3
+    assert se_pca < 1e-3
+    # per-asset deltas (raw noise, one replica) vs CRN central differences
+    g = np.array(engine.greeks_accumulate(m, PAY, 64, 1, 42, 0, 1 << 16, fdmc.NOISE_RAW))[0]
+    for a in (0, 7, 15):
+        h = 0.05
+        up, dn = [100.0] * 16, [100.0] * 16
+        up[a] += h
+        dn[a] -= h
+        L = cholesky_lower(C).ravel()
+        pu = engine.qmc_accumulate(fdmc.MultiGBMParams(16, up, list(sig), 0.05, 1.0, L=L), PAY, 64, 1, 42, 0,
+                                   1 << 16, fdmc.NOISE_RAW)[0][0]
+        pd = engine.qmc_accumulate(fdmc.MultiGBMParams(16, dn, list(sig), 0.05, 1.0, L=L), PAY, 64, 1, 42, 0,
+                                   1 << 16, fdmc.NOISE_RAW)[0][0]
+        fd = (pu - pd) / (2 * h) / (1 << 16)
+        adj = g[2 + 2 * a] / (1 << 16)
+        assert abs(adj - fd) < 0.01 * abs(fd) + 1e-4, (a, adj, fd)
