@@ -751,10 +751,15 @@ void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cac
         for (int o : order) {
             LevelEntry e = ent[o];
             const int64_t t0 = e.first >> 8, t1 = (e.first + e.count - 1) >> 8;
-            if (t1 - t0 + 1 > (1LL << 30) || total + (t1 - t0 + 1) > 0x7FFFFFFF)
-                fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
+            const int64_t nt = t1 - t0 + 1;
+            if (nt > (1LL << 30)) fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
+            // about 512 noise dimensions of work per lane and item, and at least
+            // 64 items per entry where it has that many tiles
+            e.tpi = (int)std::max<int64_t>(1, std::min<int64_t>(512 / std::max(e.D, 1), nt / 64));
+            const int64_t items = (nt + e.tpi - 1) / e.tpi;
+            if (total + items > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
             e.tile_begin = total;
-            e.ntiles = (int)(t1 - t0 + 1);
+            e.ntiles = (int)items;
             total += e.ntiles;
             max_D = std::max(max_D, e.D);
             max_G = std::max(max_G, e.G);

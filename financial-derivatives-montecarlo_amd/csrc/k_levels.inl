@@ -3,9 +3,12 @@
 //
 // The reference launches one kernel per level on its own stream, each with its
 // own generator and buffers (methods_cuda.cu:1347-1395, 1762-1924).  Here a
-// grid sized to the device drains a queue of 256-sample tiles of all requested
+// grid sized to the device drains a queue of work items of all requested
 // (level, replica) entries, longest levels first; a workgroup takes the next
-// tile with one atomic.  The per-path arithmetic is kernel_mlmc
+// item with one atomic.  An item is tpi consecutive 256-sample tiles of one
+// entry, tpi chosen on the host so that items of short levels cost about as
+// much as one tile of a long level (a 1-step level would otherwise be millions
+// of 256-path items).  The per-path arithmetic is kernel_mlmc
 // (methods_cuda.cu:445-518) with the fine path's running payoff updated for
 // every model (the reference skips it for Heston, :464-470).
 #pragma once
@@ -98,17 +101,24 @@ __global__ __launch_bounds__(256) void k_levels(LevelsLaunch a, int max_D, int m
             else hi = mid - 1;
         }
         const LevelEntry E = a.entries[lo];
-        const int t = item - E.tile_begin;
-        const int64_t base = ((E.first >> 8) + t) << 8;
+        // this item covers tiles [t0, t1) of the entry
+        const int64_t tfirst = E.first >> 8;
+        const int nt_all = (int)(((E.first + E.count - 1) >> 8) - tfirst + 1);
+        const int t0 = (item - E.tile_begin) * E.tpi;
+        const int t1 = min(t0 + E.tpi, nt_all);
+        const int D = E.D;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int t = t0; t < t1; t++) {
+        const int64_t base = (tfirst + t) << 8;
         const int64_t idx = base + tid;
         const bool valid = idx >= E.first && idx < E.first + E.count;
-        const int D = E.D;
 
         if constexpr (NZ != NZ_PRNG) {
             sl.grec = E.sdir;
             const uint32_t b32 = (uint32_t)base;
             const uint32_t gbase = b32 ^ (b32 >> 1);
             if (lds_ok) {
+                if (t > t0) __syncthreads();   // every lane is done with the previous tile's table
                 if (lo != last_e) {
                     for (int i = tid; i < D * 16; i += 256) {
                         const int d = i >> 4, k = i & 15;
@@ -206,14 +216,14 @@ __global__ __launch_bounds__(256) void k_levels(LevelsLaunch a, int max_D, int m
 
         float pf, pc;
         cp.finish(P, E, pf, pc);
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
         if (valid) {
             const double dy = (double)(pf - pc), yf = (double)pf;
-            acc[0] = dy;
-            acc[1] = dy * dy;
-            acc[2] = yf;
-            acc[3] = yf * yf;
+            acc[0] += dy;
+            acc[1] += dy * dy;
+            acc[2] += yf;
+            acc[3] += yf * yf;
         }
+        }  // tiles of the item
         block_sum<4, 4>(acc, red);
         if (tid == 0) {
             double* out = a.partials + (int64_t)item * 4;

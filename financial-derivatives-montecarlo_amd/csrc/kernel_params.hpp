@@ -60,12 +60,13 @@ struct PathLaunch {
 };
 
 // Coupled multilevel launch (MLMC / MLQMC): one entry per (level, replica)
-// request; entries are cut into 256-sample tiles that a persistent grid drains
-// through an atomic counter.  Tile t of entry e writes its sums to
-// partials[tile_begin + t] so the result does not depend on scheduling.
+// request; entries are cut into work items of tpi 256-sample tiles that a
+// persistent grid drains through an atomic counter.  Item t of entry e writes
+// its sums to partials[tile_begin + t] so the result does not depend on
+// scheduling.
 struct LevelEntry {
     int level, n_fine, n_coarse, D, G;
-    int tile_begin, ntiles, pad0;
+    int tile_begin, ntiles, tpi;   // first work item, work items, 256-sample tiles per item
     int64_t first, count;   // sample (PRNG path id / Sobol point index) range
     float h_f, h_c, sqrt_hf, em_f, em_c, bgk_f, bgk_c, pad1;
     uint32_t seed, tag;     // PRNG stream (MLMC)
@@ -78,7 +79,7 @@ struct LevelsLaunch {
     int M, nf, n_entries, total_items;
     const LevelEntry* entries;  // device array, sorted by decreasing cost
     unsigned int* queue;        // work counter (zeroed before the launch)
-    double* partials;           // [total_items][4]
+    double* partials;           // [total_items][4], one row per work item
 };
 
 // Multi-asset basket launch (k_basket.hip).

@@ -43,10 +43,15 @@ def test_gemm_matches_fp64_gemm(engine, torch_cuda, c3, bf16):
     z = O.sobol_normal_block(salt, 1024, off, n).T.astype(np.float32)     # [n][D]
     q = (lambda a: _round_bf16(a)) if bf16 else (lambda a: a.astype(np.float16).astype(np.float32))
     exact_q = q(z).astype(np.float64) @ q(M.astype(np.float32)).astype(np.float64).T
-    err_acc = np.abs(got - exact_q).max() / np.abs(exact_q).max()
-    assert err_acc < 2e-6, err_acc                        # FP32 accumulation only
+    # The device rounds its own FP32 normals (within ~1e-7 of the FP64 oracle), so
+    # a small fraction of operands round to the neighbouring half-precision value;
+    # beyond that only FP32 accumulation error remains (a layout error is O(1)).
+    err_acc = np.linalg.norm(got - exact_q) / np.linalg.norm(exact_q)
+    assert err_acc < (1e-4 if bf16 else 3e-5), err_acc
     ref = z.astype(np.float64) @ M.T
     rel_fro = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print("GEMM %s: rel Frobenius error vs FP64 GEMM of the exact normals %.3e, vs rounded operands %.3e"
+          % ("bf16" if bf16 else "fp16", rel_fro, err_acc))
     assert rel_fro < (6e-3 if bf16 else 8e-4), rel_fro    # operand quantisation
 
 
