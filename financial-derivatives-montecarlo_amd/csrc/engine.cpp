@@ -18,96 +18,21 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fdmc.h"
-#include "engine_internal.hpp"
-#include "kernel_params.hpp"
+#include "engine_host.hpp"
 
 #include "joe_kuo6_table.inc"
 
 using namespace fdmc;
 
-// ---------------------------------------------------------------- errors
+using namespace fdmc_host;
+
+thread_local std::string fdmc_host::g_last_error;
+
 namespace {
-thread_local std::string g_last_error;
-
-struct Fail {
-    fdmc_status code;
-    std::string msg;
-};
-
-[[noreturn]] void fail(fdmc_status c, const std::string& m) { throw Fail{c, m}; }
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) fail(FDMC_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPC(call) hip_check((call), #call)
-
-template <class F>
-fdmc_status guarded(F&& f) {
-    try {
-        f();
-        g_last_error.clear();
-        return FDMC_OK;
-    } catch (const Fail& e) {
-        g_last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return FDMC_E_ALLOC;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return FDMC_E_INTERNAL;
-    }
-}
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 constexpr int D_MAX_SOBOL = 20000;  // models.hpp:9
 constexpr uint32_t TAG_QMC_FALLBACK = 0x51u, TAG_MLMC = 0x100u, TAG_CV_PILOT = 0x200u,
                    TAG_CV_MAIN = 0x201u;
-
-// Grow-only device buffer.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void* ensure(size_t n) {
-        if (n <= bytes) return p;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t want = std::max(n, (size_t)4096);
-        if (hipMalloc(&p, want) != hipSuccess) {
-            p = nullptr;
-            fail(FDMC_E_ALLOC, "hipMalloc of " + std::to_string(want) + " bytes failed");
-        }
-        bytes = want;
-        return p;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
 }  // namespace
-
-struct fdmc_context {
-    int device = 0;
-    int n_cu = 256;
-    hipStream_t stream = nullptr;
-    uint32_t* d_dirv = nullptr;   // [FDMC_JK_NDIM][32]
-    DevBuf partials, sums, salts, sdir, gen_sdir, pca_z, pca_dw, basket_L;
-    std::vector<uint32_t> sdir_key;  // salts + D of the cached sdir table
-    int sdir_D = -1;
-    double* h_sums = nullptr;     // pinned
-    size_t h_sums_cap = 0;
-    bool timing = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int64_t timed_launches = 0;
-    double timed_ms = 0.0;
-};
 
 struct fdmc_bb {
     fdmc_context* ctx;
@@ -181,6 +106,8 @@ void validate_payoff(const fdmc_payoff* p) {
     if (p->kind < 0 || p->kind > FDMC_PAYOFF_BASKET) fail(FDMC_E_INVALID, "unknown payoff kind");
 }
 
+}  // namespace
+
 // make_params (methods_cuda.cu:986-1051) for one leg.
 LegParams make_leg(const fdmc_model& m, const fdmc_payoff& p) {
     LegParams L{};
@@ -230,6 +157,8 @@ int model_kind_dev(const fdmc_model& m) {
     default: return MK_MULTI_GBM;
     }
 }
+
+namespace {
 // Single-asset models price a Basket payoff as a European on S_T, as the
 // reference's MC_TABLE does (methods_cuda.cu:345-347).
 int payoff_kind_dev(const fdmc_payoff& p) {

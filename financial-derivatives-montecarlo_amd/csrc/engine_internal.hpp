@@ -42,6 +42,16 @@ hipError_t launch_gemm_nt(const void* A, const void* Bt, float* C, int Mrows, in
 const void* basket_kernel(int mk, int src, int na);
 size_t basket_lds_bytes(int src, int D, int na);
 
+// k_book.hip
+size_t book_lds_bytes(int skg);
+const void* book_kernel(int mk, int skg, int gm);  // gm: 0 price, 1 adjoint Greeks, 2 tangent Greeks
+hipError_t launch_book_reset(const BookLaunch& a, hipStream_t s);
+hipError_t launch_book_plan(const BookLaunch& a, hipStream_t s);
+hipError_t launch_book_sweep(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s);
+hipError_t launch_book_pass(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s);
+hipError_t launch_book_status(const BookLaunch& a, hipStream_t s);
+hipError_t launch_book_final(const BookLaunch& a, hipStream_t s);
+
 // k_path_{raw,bb,prng}.hip: kernel for a (model, payoff, control) combination, or null.
 const void* path_kernel_raw(int mk, int pk, int cv, int cpk, int lb);
 const void* path_kernel_bb(int mk, int pk, int cv, int cpk, int lb);

@@ -135,4 +135,70 @@ inline size_t levels_lds_bytes(int nz, int max_D, int max_G) {
     return b + 4 * 4 * sizeof(double) + 16;
 }
 
+// ---- contract book (k_book.hip) ---------------------------------------------
+// Level l of a contract simulates 2^l fine steps; at most BOOK_NL levels.
+constexpr int BOOK_NL = 10;
+constexpr int BOOK_DMAX = 1 << (BOOK_NL - 1);     // 512 dates
+constexpr int BOOK_GMAX = BOOK_DMAX / 16;         // 16-step blocks of the finest level
+constexpr int BOOK_NACC = 4;                      // P_f - P_c, delta, vega, rho
+constexpr int BOOK_MAX_ITEMS = 128;               // work items per entry and pass
+constexpr int BOOK_MAX_T = 8;                     // distinct maturities (bridge tables)
+
+struct BookContract {
+    int kind;                 // PK_EUROPEAN / PK_ASIAN / PK_LOOKBACK / PK_BARRIER
+    int L;                    // finest level (n_steps = 2^L)
+    int t_idx;                // maturity slot of the bridge table
+    int entry0;               // entry id of (l, r) is entry0 + l R + r
+    float K, B, discount, Tf;
+    double T;
+    float bgk[BOOK_NL];       // BGK factor exp(-0.5826 sigma sqrt(T / 2^l)), 1 when unused
+    int slot[BOOK_NL];        // index among the contracts with L >= l (Sobol records of level l)
+};
+
+struct BookEntry {            // one (contract, level, replica) of a pass
+    int c, l, r, tpi;         // tpi: 256-point tiles per work item
+    int tile_begin, ntiles;   // first work item, work items
+    int64_t first, count;     // Sobol point range of this pass
+};
+
+struct BookState {            // per (contract, level), merged as run_mlqmc_cuda
+    double E[BOOK_NACC], S2[BOOK_NACC];
+    int64_t N, extra;
+};
+
+struct BookCtl {              // per contract
+    int iter, done, conv, pad;
+};
+
+struct BookOut {              // per contract, read back by the host
+    double v[8];              // price, se, delta, vega, rho, their se
+    int64_t n_samples, path_steps;
+    int iterations, flags;
+};
+
+struct BookLaunch {
+    LegParams leg;            // model (K, B, discount overwritten per contract)
+    int R, c_begin, n_contracts, n_entries;
+    int pilot_n, max_iter, greeks, items_target;
+    double eps;
+    const BookContract* contracts;
+    BookEntry* entries;       // this launch's entry array (batched: level-major; plain: one contract)
+    const int* entry_pos;     // entry id -> position in entries
+    BookState* state;         // [contracts][BOOK_NL]
+    BookCtl* ctl;             // [contracts]
+    const uint32_t* sdir[BOOK_NL];  // level l: [slots][R][2^l][SDIR_WORDS]
+    const float* bbtab;       // bridge sd per depth: [maturities][BOOK_NL levels][BOOK_NL depths]
+    unsigned int* queue;
+    int* total_items;
+    int* active;
+    double* partials;         // [items][BOOK_NACC]
+    double* entry_sums;       // [n_entries][BOOK_NACC]
+    float* scratch;           // per workgroup: checkpoints [2][BOOK_GMAX][256] (+ skeleton)
+    BookOut* out;
+};
+
+constexpr size_t book_scratch_floats(int skel_global) {
+    return (size_t)2 * BOOK_GMAX * 256 + (skel_global ? (size_t)(BOOK_GMAX + 1) * 256 : 0);
+}
+
 }  // namespace fdmc

@@ -12,6 +12,8 @@ import math
 import os
 from collections import namedtuple
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libfdmc.so")
 
@@ -76,9 +78,35 @@ class CCVPilot(C.Structure):
     _fields_ = [("beta", C.c_double), ("var_plain", C.c_double), ("var_cv", C.c_double)]
 
 
+class CBookContract(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n_steps", C.c_int32), ("K", C.c_double), ("B", C.c_double),
+                ("T", C.c_double), ("r", C.c_double), ("sigma_bgk", C.c_double)]
+
+
+class CBookConfig(C.Structure):
+    _fields_ = [("R", C.c_int32), ("pilot_n", C.c_int32), ("salt_base", C.c_uint32), ("greeks", C.c_int32),
+                ("max_iter", C.c_int32), ("iters_per_graph", C.c_int32)]
+
+
+class CBookResult(C.Structure):
+    _fields_ = [("price", C.c_double), ("std_error", C.c_double), ("delta", C.c_double), ("vega", C.c_double),
+                ("rho", C.c_double), ("delta_se", C.c_double), ("vega_se", C.c_double), ("rho_se", C.c_double),
+                ("n_samples", C.c_int64), ("path_steps", C.c_int64), ("levels", C.c_int32),
+                ("iterations", C.c_int32), ("flags", C.c_uint32), ("pad", C.c_int32)]
+
+
+class CBookStats(C.Structure):
+    _fields_ = [("time_s", C.c_double), ("tune_s", C.c_double), ("path_steps", C.c_int64), ("passes", C.c_int32),
+                ("graph_replays", C.c_int32), ("variant", C.c_int32), ("items_target", C.c_int32),
+                ("wg_per_cu", C.c_int32), ("pad", C.c_int32)]
+
+
 MCResult = namedtuple("MCResult", "price std_error n_samples time_s path_steps levels flags")
 CVPilot = namedtuple("CVPilot", "beta var_plain var_cv")
 Greeks = namedtuple("Greeks", "delta vega rho delta_se vega_se rho_se")
+BookResult = namedtuple("BookResult", "price std_error delta vega rho delta_se vega_se rho_se n_samples "
+                        "path_steps levels iterations flags")
+BookStats = namedtuple("BookStats", "time_s tune_s path_steps passes graph_replays variant items_target wg_per_cu")
 
 
 def _result(r):
@@ -291,6 +319,49 @@ class MLMCConfig:
         return CMLMCConfig(self.M, self.max_L, self.pilot_n, self.seed)
 
 
+# ---------------------------------------------------------------- contract book
+BOOK_LEVELS = 10          # levels 0..9: up to 512 dates
+BOOK_BATCHED, BOOK_PLAIN = 0, 1
+
+
+class BookContract:
+    """One contract of a book (payoff kind, dates, strike, barrier, maturity)."""
+
+    def __init__(self, kind, n_steps, K=100.0, B=0.0, T=1.0, r=0.05, sigma_bgk=0.2):
+        self.kind, self.n_steps, self.K, self.B, self.T, self.r, self.sigma_bgk = kind, n_steps, K, B, T, r, sigma_bgk
+
+    def to_c(self):
+        return CBookContract(self.kind, self.n_steps, self.K, self.B, self.T, self.r, self.sigma_bgk)
+
+
+class BookConfig:
+    def __init__(self, R=16, pilot_n=400, salt_base=42, greeks=True, max_iter=50, iters_per_graph=4):
+        self.R, self.pilot_n, self.salt_base, self.greeks = R, pilot_n, salt_base, greeks
+        self.max_iter, self.iters_per_graph = max_iter, iters_per_graph
+
+    def to_c(self):
+        # greeks: False/0 none, True/1 adjoint, 2 forward tangents
+        return CBookConfig(self.R, self.pilot_n, self.salt_base, int(self.greeks), self.max_iter,
+                           self.iters_per_graph)
+
+
+def local_vol_book(n=1024, seed=2026, r=0.05):
+    """Seeded book under the quadratic local vol sigma(S,t) = 0.2 + 0.1 e^{-t} ln(S/100)^2:
+    payoff kind uniform over European / Asian / lookback / barrier, K ~ U[70, 130],
+    T in {0.25, 0.5, 1, 2}, dates 2^k for k in 4..9 (16..512), barrier B = 1.3 K,
+    BGK vol 0.2 (the surface's level).  Returns (model, contracts)."""
+    rng = np.random.default_rng(seed)
+    kinds = [PAYOFF_EUROPEAN, PAYOFF_ASIAN, PAYOFF_LOOKBACK, PAYOFF_BARRIER]
+    kind = rng.integers(0, 4, n)
+    K = rng.uniform(70.0, 130.0, n)
+    T = rng.choice(np.array([0.25, 0.5, 1.0, 2.0]), n)
+    steps = 2 ** rng.integers(4, 10, n)
+    model = LocalVolQuadParams(S0=100.0, mu=r, a=0.2, b=0.1, c=1.0, s_ref=100.0, T=1.0)
+    book = [BookContract(kinds[int(kind[i])], int(steps[i]), float(K[i]), 1.3 * float(K[i]), float(T[i]), r, 0.2)
+            for i in range(n)]
+    return model, book
+
+
 # ---------------------------------------------------------------- library binding
 _P = C.c_void_p
 _SIGS = {
@@ -348,6 +419,11 @@ _SIGS = {
     "fdmc_sobol_scrambled_u32": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_sobol_normal": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_icdf_u32": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "fdmc_book_create": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CBookContract), C.c_int32,
+                                   C.POINTER(CBookConfig), C.POINTER(_P)]),
+    "fdmc_book_run": (C.c_int, [_P, C.c_double, C.c_int32, C.POINTER(CBookResult), C.POINTER(CBookStats)]),
+    "fdmc_book_level_sums": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
+    "fdmc_book_free": (C.c_int, [_P]),
 }
 
 _lib = None
@@ -594,6 +670,42 @@ class Engine:
 
     def icdf_u32(self, d_in, n, d_out):
         _check(self.lib, self.lib.fdmc_icdf_u32(self.ctx, d_in, n, d_out))
+
+    # -- contract book
+    def book_create(self, model, contracts, cfg=None):
+        return Book(self, model, contracts, cfg or BookConfig())
+
+
+class Book:
+    """A device-resident contract book (fdmc_book_*)."""
+
+    def __init__(self, eng, model, contracts, cfg):
+        self.eng, self.cfg, self.n = eng, cfg, len(contracts)
+        arr = (CBookContract * self.n)(*[k.to_c() for k in contracts])
+        m, c = model.to_c(), cfg.to_c()
+        h = _P()
+        _check(eng.lib, eng.lib.fdmc_book_create(eng.ctx, C.byref(m), arr, self.n, C.byref(c), C.byref(h)))
+        self.handle = h
+
+    def run(self, eps, mode=BOOK_BATCHED):
+        res = (CBookResult * self.n)()
+        st = CBookStats()
+        _check(self.eng.lib, self.eng.lib.fdmc_book_run(self.handle, eps, mode, res, C.byref(st)))
+        out = [BookResult(x.price, x.std_error, x.delta, x.vega, x.rho, x.delta_se, x.vega_se, x.rho_se,
+                          x.n_samples, x.path_steps, x.levels, x.iterations, x.flags) for x in res]
+        stats = BookStats(st.time_s, st.tune_s, st.path_steps, st.passes, st.graph_replays, st.variant,
+                          st.items_target, st.wg_per_cu)
+        return out, stats
+
+    def level_sums(self, c, l, offset, n):
+        s = (C.c_double * (4 * self.cfg.R))()
+        _check(self.eng.lib, self.eng.lib.fdmc_book_level_sums(self.handle, c, l, offset, n, s))
+        return [[s[r * 4 + k] for k in range(4)] for r in range(self.cfg.R)]
+
+    def free(self):
+        if self.handle:
+            self.eng.lib.fdmc_book_free(self.handle)
+            self.handle = None
 
 
 # ---------------------------------------------------------------- host numerics (utils.cpp)

@@ -135,9 +135,75 @@ typedef struct fdmc_greeks {
 
 typedef struct fdmc_cv_pilot_result { double beta, var_plain, var_cv; } fdmc_cv_pilot_result; /* CVPilot */
 
+/* ---- contract book (no counterpart in the reference, which prices one
+ * contract per run_mlqmc_cuda call, methods_cuda.cuh:90-96) ----
+ * Every contract of a book shares one one-factor model (GBM, Dupire or the
+ * quadratic local vol) and is priced by multilevel QMC with refinement M = 2 and
+ * a fixed finest level: level l simulates 2^l Euler steps over [0, T] (the
+ * coarse path of level l > 0 takes 2^{l-1} steps on summed fine increments),
+ * l = 0 .. log2(n_steps), driven by the Brownian bridge of Hong-Hickernell
+ * scrambled Sobol points with salt
+ *     salt_base + (c * FDMC_BOOK_LEVELS + l) * R + r
+ * (contract c, replica r; contract 0 has run_mlqmc_cuda's 42 + l R + r,
+ * methods_cuda.cu:1845).  Sample allocation, merges and the per-point variance
+ * S2_l follow run_mlqmc_cuda (methods_cuda.cu:1680-2019) level by level; a
+ * contract stops when no level needs more points (its bias is that of its own
+ * date grid, so there is no bias test). */
+#define FDMC_BOOK_LEVELS 10           /* levels 0..9: n_steps up to 512 */
+
+typedef struct fdmc_book_contract {
+    int32_t kind;                /* FDMC_PAYOFF_EUROPEAN, _ASIAN, _LOOKBACK or _BARRIER */
+    int32_t n_steps;             /* monitoring dates: a power of two, 1 .. 512 */
+    double K;                    /* strike (unused by the lookback) */
+    double B;                    /* up-and-out barrier (barrier only) */
+    double T;                    /* maturity; the book holds at most 8 distinct values */
+    double r;                    /* discount rate of European and barrier payoffs */
+    double sigma_bgk;            /* BGK continuity correction vol (lookback, barrier) */
+} fdmc_book_contract;
+
+typedef struct fdmc_book_config {
+    int32_t R;                   /* replicas, default 16 */
+    int32_t pilot_n;             /* points per replica and level in the first pass, default 400 */
+    uint32_t salt_base;          /* default 42 */
+    int32_t greeks;              /* delta, vega, rho of Lipschitz payoffs: 0 none, 1 pathwise adjoint
+                                    (default; reverse sweep recomputing states from the Sobol index),
+                                    2 forward tangents (same derivative, one pass) */
+    int32_t max_iter;            /* passes per contract (pilot included), default 50 */
+    int32_t iters_per_graph;     /* batched mode: passes captured per hipGraph, default 4 */
+} fdmc_book_config;
+
+typedef enum { FDMC_BOOK_BATCHED = 0,  /* all contracts per pass, device-side planning, hipGraph */
+               FDMC_BOOK_PLAIN = 1     /* one contract at a time, host-driven passes */
+} fdmc_book_mode;
+
+typedef struct fdmc_book_result {
+    double price, std_error;
+    double delta, vega, rho;     /* NaN for barrier contracts (not Lipschitz) or greeks = 0 */
+    double delta_se, vega_se, rho_se;
+    int64_t n_samples;           /* sum over levels of N_l R */
+    int64_t path_steps;          /* sum over levels of N_l R 2^l */
+    int32_t levels;              /* log2(n_steps) + 1 */
+    int32_t iterations;          /* passes that added points */
+    uint32_t flags;              /* FDMC_FLAG_NOT_CONVERGED */
+    int32_t pad;
+} fdmc_book_result;
+
+typedef struct fdmc_book_stats {
+    double time_s;               /* wall time of the run, device work included */
+    double tune_s;               /* autotuning time (first run only; not in time_s) */
+    int64_t path_steps;          /* all contracts */
+    int32_t passes;              /* device passes (batched) or host iterations (plain) */
+    int32_t graph_replays;       /* batched mode */
+    int32_t variant;             /* kernel variant chosen by the autotuner */
+    int32_t items_target;        /* noise dimensions per lane and work item chosen by the autotuner */
+    int32_t wg_per_cu;           /* persistent workgroups per CU chosen by the autotuner */
+    int32_t pad;
+} fdmc_book_stats;
+
 typedef struct fdmc_context fdmc_context;
 typedef struct fdmc_bb fdmc_bb;    /* DeviceBBData  (methods_cuda.cuh:52) */
 typedef struct fdmc_pca fdmc_pca;  /* DevicePCAData (methods_cuda.cuh:53) */
+typedef struct fdmc_book fdmc_book;
 
 /* ---- library / context ---- */
 int32_t fdmc_abi_version(void);
@@ -282,6 +348,27 @@ fdmc_status fdmc_sobol_normal(fdmc_context* ctx, uint32_t salt, int32_t D, int64
                               int64_t offset, float* d_out);
 /* Map n uint32 values through the device inverse CDF (test hook for the ICDF). */
 fdmc_status fdmc_icdf_u32(fdmc_context* ctx, const uint32_t* d_in, int64_t n, float* d_out);
+
+/* ---- contract book (replaces one run_mlqmc_cuda call per contract,
+ * methods_cuda.cuh:90-96, with one pass over every contract) ----
+ * fdmc_book_create copies the contracts, builds the scrambled direction records
+ * of every (contract, level, replica) and the bridge tables on the device.
+ * model: GBM, DUPIRE or LOCALVOL_QUAD (its T is ignored: each contract has its own). */
+fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model,
+                             const fdmc_book_contract* contracts, int32_t n_contracts,
+                             const fdmc_book_config* cfg, fdmc_book** out);
+/* Price (and Greeks) of every contract to standard error eps / sqrt(2) of the
+ * price.  mode: fdmc_book_mode; both modes run the same kernels and give
+ * bit-identical results.  results: n_contracts entries; stats may be NULL.
+ * The first call autotunes the kernel variant (reported in stats->tune_s). */
+fdmc_status fdmc_book_run(fdmc_book* book, double eps, int32_t mode, fdmc_book_result* results,
+                          fdmc_book_stats* stats);
+/* Per-replica sums of one pass of contract c, level l over points
+ * [offset, offset + n) of every replica: sums_out[r * 4 + k], k = price
+ * difference P_f - P_c, then its delta, vega and rho (test hook). */
+fdmc_status fdmc_book_level_sums(fdmc_book* book, int32_t c, int32_t l, int64_t offset, int64_t n,
+                                 double* sums_out);
+fdmc_status fdmc_book_free(fdmc_book* book);
 
 #ifdef __cplusplus
 }

@@ -163,6 +163,19 @@ int orc_run_mc_cv(const fdmc_model* main_model, const fdmc_model* ctrl_model,
                   double E_ctrl, double beta, double eps, int n_steps,
                   const fdmc_mc_config* cfg, fdmc_result* out);
 
+/* ---------------- contract book (fixed-level MLQMC with pathwise Greeks) ----------------
+ * out[0] = P_f - P_c of one coupled path of n_fine = 2^l scaled increments dW
+ * (coarse: n_fine/2 steps on summed pairs; level 0: no coarse leg), out[1..3] its
+ * derivatives wrt S0, the volatility parameter and mu = r (forward mode). */
+void orc_book_path(const fdmc_model* m, const fdmc_book_contract* k, const float* dW, int n_fine,
+                   float out[4]);
+/* Per-replica sums (R x 4) of level l of contract c over points [offset, offset+n). */
+int orc_book_level_sums(const fdmc_model* m, const fdmc_book_contract* k, int c, int l, int R,
+                        uint32_t salt_base, int64_t offset, int64_t n, double* sums);
+/* The whole allocation loop of one contract (fdmc_book_run's per-contract result). */
+int orc_book_contract_run(const fdmc_model* m, const fdmc_book_contract* k, int c, double eps,
+                          const fdmc_book_config* cfg, fdmc_book_result* out);
+
 const char* orc_last_error(void);
 int orc_num_threads(void);
 

@@ -623,3 +623,252 @@ int orc_run_mc_cv(const fdmc_model* main_model, const fdmc_model* ctrl_model,
     out->time_s = now_s() - t0;
     return 0;
 }
+
+/* ======================================================================
+ * Contract book: every contract priced by multilevel QMC with M = 2 and its
+ * finest level fixed at its own date grid (levels 0 .. log2(n_steps)), with
+ * pathwise Greeks of the level differences.  Allocation and merges are
+ * run_mlqmc_cuda's (methods_cuda.cu:1680-2019) applied level by level; the
+ * salt of (contract c, level l, replica r) is salt_base + (c * 10 + l) R + r,
+ * so contract 0 has the reference's 42 + l R + r (methods_cuda.cu:1845).
+ *
+ * The Greeks are computed here in FORWARD mode (tangents dS/dS0, dS/dvol,
+ * dS/dmu carried along the path), an independent restatement of the engine's
+ * reverse-mode adjoint: both are the pathwise derivative of the same FP32
+ * Euler recursion, so they agree to rounding.
+ * ====================================================================== */
+
+/* One Euler step of a one-factor model and its partial derivatives
+ * (methods_cuda.cu:271-291 for the step; the partials are its derivatives). */
+static float book_step(const orc_params* P, float S, float dw, float t, float h, float* fS,
+                       float* fvol, float* fmu, float* fS0) {
+    *fmu = S * h;
+    *fS0 = 0.0f;
+    switch (P->model_kind) {
+    case FDMC_MODEL_DUPIRE: {
+        float e = expf(-P->alpha * t);
+        float pw = powf(S / P->S0, P->beta_d - 1.0f);
+        float sl = P->sigma0 * e * pw;
+        *fS = 1.0f + P->mu * h + dw * sl * P->beta_d;
+        *fvol = e * pw * S * dw;
+        *fS0 = -sl * (P->beta_d - 1.0f) / P->S0 * S * dw;
+        return S + P->mu * S * h + sl * S * dw;
+    }
+    case FDMC_MODEL_LOCALVOL_QUAD: {
+        float l = logf(S / P->lv_sref);
+        float be = P->lv_b * expf(-P->lv_c * t);
+        float sl = P->lv_a + be * l * l;
+        *fS = 1.0f + P->mu * h + dw * (sl + 2.0f * be * l);
+        *fvol = S * dw;
+        return S + P->mu * S * h + sl * S * dw;
+    }
+    default:   /* GBM */
+        *fS = 1.0f + P->mu * h + P->sigma * dw;
+        *fvol = S * dw;
+        return S + P->mu * S * h + P->sigma * S * dw;
+    }
+}
+
+/* One leg (n steps of size h; pair = 1 sums consecutive increments for the
+ * coarse path) -> payoff y and its derivatives g[0..2] (S0, vol, mu = r). */
+static void book_leg(const orc_params* P, int kind, const float* dW, int n, int pair, float h,
+                     float bgk, float T, float* y, float g[3]) {
+    float S = P->S0, d[3] = {1.0f, 0.0f, 0.0f};
+    float sum = 0.0f, ds[3] = {0.0f, 0.0f, 0.0f};
+    float mn = P->S0, dmn[3] = {1.0f, 0.0f, 0.0f}, mx = 0.0f;
+    for (int j = 0; j < n; j++) {
+        float dw = pair ? dW[2 * j] + dW[2 * j + 1] : dW[j];
+        float fS, fv, fm, f0;
+        float Sn = book_step(P, S, dw, (float)j * h, h, &fS, &fv, &fm, &f0);
+        d[0] = fS * d[0] + f0;
+        d[1] = fS * d[1] + fv;
+        d[2] = fS * d[2] + fm;
+        S = Sn;
+        if (kind == FDMC_PAYOFF_ASIAN) {
+            sum += S;
+            for (int k = 0; k < 3; k++) ds[k] += d[k];
+        } else if (kind == FDMC_PAYOFF_LOOKBACK) {
+            if (S < mn) { mn = S; for (int k = 0; k < 3; k++) dmn[k] = d[k]; }
+        } else if (kind == FDMC_PAYOFF_BARRIER) {
+            mx = fmaxf(mx, S);
+        }
+    }
+    g[0] = g[1] = g[2] = 0.0f;
+    switch (kind) {
+    case FDMC_PAYOFF_EUROPEAN:
+        *y = fmaxf(S - P->K, 0.0f) * P->discount;
+        if (S > P->K) for (int k = 0; k < 3; k++) g[k] = P->discount * d[k];
+        g[2] -= T * *y;
+        break;
+    case FDMC_PAYOFF_ASIAN: {
+        float A = sum / (float)n;
+        *y = fmaxf(A - P->K, 0.0f);
+        if (A > P->K) for (int k = 0; k < 3; k++) g[k] = ds[k] / (float)n;
+        break;
+    }
+    case FDMC_PAYOFF_LOOKBACK:
+        *y = S - mn * bgk;
+        for (int k = 0; k < 3; k++) g[k] = d[k] - bgk * dmn[k];
+        break;
+    default:  /* barrier: not Lipschitz, no pathwise Greeks */
+        *y = (mx / bgk >= P->B) ? 0.0f : fmaxf(S - P->K, 0.0f) * P->discount;
+        break;
+    }
+}
+
+static float book_bgk(const fdmc_book_contract* k, float h) {
+    if (k->kind != FDMC_PAYOFF_LOOKBACK && k->kind != FDMC_PAYOFF_BARRIER) return 1.0f;
+    return expf(-0.5826f * (float)k->sigma_bgk * sqrtf(h));   /* payoffs.hpp:8 */
+}
+
+static int book_params(const fdmc_model* m, const fdmc_book_contract* k, orc_params* P) {
+    fdmc_payoff p;
+    memset(&p, 0, sizeof p);
+    p.kind = k->kind == FDMC_PAYOFF_LOOKBACK ? FDMC_PAYOFF_LOOKBACK : FDMC_PAYOFF_EUROPEAN;
+    p.K = k->K; p.r = k->r; p.T = k->T; p.sigma = k->sigma_bgk;
+    if (orc_make_params(m, &p, 1, P)) return -1;
+    P->K = (float)k->K;
+    P->B = (float)k->B;
+    P->discount = (k->kind == FDMC_PAYOFF_EUROPEAN || k->kind == FDMC_PAYOFF_BARRIER)
+                      ? (float)exp(-k->r * k->T) : 1.0f;
+    P->T = (float)k->T;
+    return 0;
+}
+
+void orc_book_path(const fdmc_model* m, const fdmc_book_contract* k, const float* dW, int nf,
+                   float out[4]) {
+    orc_params P;
+    book_params(m, k, &P);
+    float hf = (float)(k->T / nf), yf, gf[3], yc = 0.0f, gc[3] = {0.0f, 0.0f, 0.0f};
+    book_leg(&P, k->kind, dW, nf, 0, hf, book_bgk(k, hf), (float)k->T, &yf, gf);
+    if (nf >= 2) {
+        float hc = (float)(k->T / (nf / 2));
+        book_leg(&P, k->kind, dW, nf / 2, 1, hc, book_bgk(k, hc), (float)k->T, &yc, gc);
+    }
+    out[0] = yf - yc;
+    for (int i = 0; i < 3; i++) out[1 + i] = gf[i] - gc[i];
+}
+
+static int book_levels(const fdmc_book_contract* k) {
+    int L = 0;
+    while ((1 << L) < k->n_steps) L++;
+    return L;
+}
+
+int orc_book_level_sums(const fdmc_model* m, const fdmc_book_contract* k, int c, int l, int R,
+                        uint32_t salt_base, int64_t offset, int64_t n, double* sums) {
+    int nf = 1 << l;
+    if (l < 0 || l >= FDMC_BOOK_LEVELS || (1 << l) > k->n_steps) {
+        orc_set_error("book level out of range"); return -1;
+    }
+    orc_bb bb;
+    if (orc_bb_precompute(nf, k->T, &bb)) { orc_set_error("bb_precompute failed"); return -1; }
+    for (int r = 0; r < R; r++) {
+        uint32_t salt = salt_base + (uint32_t)((c * FDMC_BOOK_LEVELS + l) * R + r);
+        dir_table t;
+        dir_table_make(salt, nf, &t);
+        double tot[4] = {0, 0, 0, 0};
+#pragma omp parallel
+        {
+            double loc[4] = {0, 0, 0, 0};
+            float* z = (float*)malloc(sizeof(float) * (size_t)nf);
+            float* dW = (float*)malloc(sizeof(float) * (size_t)nf);
+#pragma omp for schedule(static)
+            for (int64_t p = 0; p < n; p++) {
+                float o[4];
+                sobol_normals_point(&t, (uint32_t)(offset + p), z);
+                orc_bb_apply_f32(&bb, z, dW);
+                orc_book_path(m, k, dW, nf, o);
+                for (int i = 0; i < 4; i++) loc[i] += (double)o[i];
+            }
+#pragma omp critical
+            for (int i = 0; i < 4; i++) tot[i] += loc[i];
+            free(z); free(dW);
+        }
+        memcpy(sums + 4 * r, tot, sizeof tot);
+        dir_table_free(&t);
+    }
+    orc_bb_release(&bb);
+    return 0;
+}
+
+int orc_book_contract_run(const fdmc_model* m, const fdmc_book_contract* k, int c, double eps,
+                          const fdmc_book_config* cfg, fdmc_book_result* out) {
+    const int R = cfg->R, L = book_levels(k);
+    double E[FDMC_BOOK_LEVELS][4], S2[FDMC_BOOK_LEVELS][4];
+    int64_t N[FDMC_BOOK_LEVELS] = {0}, extra[FDMC_BOOK_LEVELS];
+    double* s = (double*)malloc(sizeof(double) * 4 * (size_t)R);
+    double* rm = (double*)malloc(sizeof(double) * (size_t)R);
+    memset(out, 0, sizeof(*out));
+    memset(E, 0, sizeof E);
+    memset(S2, 0, sizeof S2);
+    int conv = 0, iter = 0;
+    while (iter < cfg->max_iter) {
+        int any = 0;
+        if (iter == 0) {
+            for (int l = 0; l <= L; l++) extra[l] = cfg->pilot_n;
+            any = 1;
+        } else {
+            /* stop once the price variance sum_l S2_l / (N_l R) is below eps^2/2
+             * (with every N_l at its Giles target it is, so this never waits
+             * longer than "no level needs more points") */
+            double var = 0.0, st = 0.0;
+            for (int l = 0; l <= L; l++) var += S2[l][0] / ((double)N[l] * R);
+            if (var <= 0.5 * eps * eps) { conv = 1; break; }
+            for (int l = 0; l <= L; l++) st += sqrt(S2[l][0] * (double)(1 << l));
+            for (int l = 0; l <= L; l++) {
+                int64_t Nopt = (int64_t)ceil(2.0 / ((double)R * eps * eps) *
+                                             sqrt(S2[l][0] / (double)(1 << l)) * st);
+                if (Nopt < 100) Nopt = 100;
+                extra[l] = Nopt > N[l] ? Nopt - N[l] : 0;
+                if (N[l] + extra[l] > 4294967296LL) extra[l] = 4294967296LL - N[l];
+                if (extra[l] > 0) any = 1;
+            }
+        }
+        if (!any) { conv = 1; break; }
+        for (int l = 0; l <= L; l++) {
+            if (extra[l] <= 0) continue;
+            if (orc_book_level_sums(m, k, c, l, R, cfg->salt_base, N[l], extra[l], s)) {
+                free(s); free(rm); return -1;
+            }
+            double wo = (double)N[l], wn = (double)extra[l];
+            for (int q = 0; q < 4; q++) {
+                double em, vv;
+                for (int r = 0; r < R; r++) rm[r] = s[4 * r + q] / wn;
+                mean_var(rm, R, &em, &vv);
+                if (N[l] == 0) {
+                    E[l][q] = em;
+                    S2[l][q] = vv * wn;
+                } else {
+                    E[l][q] = (E[l][q] * wo + em * wn) / (wo + wn);
+                    S2[l][q] = (S2[l][q] * wo + vv * wn * wn) / (wo + wn);
+                }
+            }
+            N[l] += extra[l];
+            out->path_steps += extra[l] * R * (int64_t)(1 << l);
+        }
+        iter++;
+    }
+    double v[4] = {0, 0, 0, 0}, val[4] = {0, 0, 0, 0};
+    for (int l = 0; l <= L; l++) {
+        for (int q = 0; q < 4; q++) {
+            val[q] += E[l][q];
+            v[q] += S2[l][q] / ((double)N[l] * R);
+        }
+        out->n_samples += N[l] * R;
+    }
+    out->price = val[0];
+    out->std_error = sqrt(v[0]);
+    int lip = k->kind != FDMC_PAYOFF_BARRIER && cfg->greeks;
+    out->delta = lip ? val[1] : NAN;
+    out->vega = lip ? val[2] : NAN;
+    out->rho = lip ? val[3] : NAN;
+    out->delta_se = lip ? sqrt(v[1]) : NAN;
+    out->vega_se = lip ? sqrt(v[2]) : NAN;
+    out->rho_se = lip ? sqrt(v[3]) : NAN;
+    out->levels = L + 1;
+    out->iterations = iter;
+    out->flags = conv ? 0u : FDMC_FLAG_NOT_CONVERGED;
+    free(s); free(rm);
+    return 0;
+}

@@ -101,6 +101,11 @@ _SIGS = {
     "orc_run_mc_cv": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff),
                                 C.POINTER(fdmc.CPayoff), C.c_double, C.c_double, C.c_double, C.c_int,
                                 C.POINTER(fdmc.CMCConfig), C.POINTER(fdmc.CResult)]),
+    "orc_book_path": (None, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CBookContract), _f32p, C.c_int, _f32p]),
+    "orc_book_level_sums": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CBookContract), C.c_int, C.c_int,
+                                      C.c_int, C.c_uint32, C.c_int64, C.c_int64, _f64p]),
+    "orc_book_contract_run": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CBookContract), C.c_int,
+                                        C.c_double, C.POINTER(fdmc.CBookConfig), C.POINTER(fdmc.CBookResult)]),
     "orc_last_error": (C.c_char_p, []),
     "orc_num_threads": (C.c_int, []),
 }
@@ -372,3 +377,27 @@ def run_mc_cv(main_model, ctrl_model, main_payoff, ctrl_payoff, E_ctrl, beta, ep
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# ---------------------------------------------------------------- contract book
+def book_path(model, contract, dW):
+    m, k = model.to_c(), contract.to_c()
+    dW = np.ascontiguousarray(dW, dtype=np.float32)
+    out = np.zeros(4, dtype=np.float32)
+    lib().orc_book_path(C.byref(m), C.byref(k), _ptr(dW, C.c_float), len(dW), _ptr(out, C.c_float))
+    return out
+
+
+def book_level_sums(model, contract, c, l, R, salt_base, offset, n):
+    m, k = model.to_c(), contract.to_c()
+    s = np.zeros(4 * R)
+    _chk(lib().orc_book_level_sums(C.byref(m), C.byref(k), c, l, R, salt_base, offset, n, _ptr(s, C.c_double)))
+    return s.reshape(R, 4)
+
+
+def book_contract_run(model, contract, c, eps, cfg):
+    m, k, g = model.to_c(), contract.to_c(), cfg.to_c()
+    x = fdmc.CBookResult()
+    _chk(lib().orc_book_contract_run(C.byref(m), C.byref(k), c, eps, C.byref(g), C.byref(x)))
+    return fdmc.BookResult(x.price, x.std_error, x.delta, x.vega, x.rho, x.delta_se, x.vega_se, x.rho_se,
+                           x.n_samples, x.path_steps, x.levels, x.iterations, x.flags)
