@@ -1,17 +1,27 @@
 """Benchmark of the path-simulation hot path on one MI355X.
 
-Workload (the second configuration of the baseline): scrambled-Sobol RQMC
-arithmetic-Asian call under GBM with a Brownian bridge and a geometric-Asian
-control variate: S0 = K = 100, r = mu = 0.05, sigma = 0.3, T = 1, D = 256 dates,
-R = 32 replicas (salts 42 + r).  One step = one fused pass over the largest
-doubling of that configuration, 2^20 points of every replica
-(32 x 2^20 x 256 = 8.6e9 path-steps), inputs (direction tables, bridge weights)
-resident in HBM.
+Workload (the configuration the headline metric is quoted on): the batched
+multilevel-QMC book of 1,024 contracts under the local vol
+sigma(S, t) = 0.2 + 0.1 exp(-t) ln(S/100)^2 (S0 = 100, r = 0.05): payoffs
+European / Asian / lookback / barrier (B = 1.3 K), K ~ U[70, 130],
+T in {0.25, 0.5, 1, 2}, 16..512 dates, drawn from seed 2026
+(fdmc.local_vol_book); R = 16 replicas, salt 42 + (10 c + l) 16 + r;
+eps = 1e-3 on every price; pathwise-adjoint delta, vega, rho.
+One step = one complete run of the book to eps (every pass of the device-side
+allocation loop, replayed as hipGraphs), inputs (direction records, bridge
+tables, contracts) resident in HBM.
 
-Printed line: path-steps/s of the timed steps (value), the dominant kernel's
-FP32-VALU roofline from HIP-event timings on the engine stream (31 algorithmic
-flops per path-step), the time to reach eps = 1e-3 with the full doubling loop,
-and the oracle's CPU throughput on a bounded sample of the same workload.
+Printed line: value = path-steps per run / wall time per run of the timed
+steps; time_to_eps = batched (all contracts per pass) vs plain (one contract at
+a time, a host round trip per pass) on the same kernels, plus the forward-
+tangent variant of the Greeks; roofline of the persistent book kernel from HIP
+events around every launch of one event-timed run (80 algorithmic FP32 flops
+per fine path-step); the CPU oracle on a bounded sample of the same book.
+The earlier configurations are reported as extra fields (c2_asian_rqmc,
+c3_pca_gemm, c4_heston_mlmc).
+
+--roofline-only runs just the event-timed book run with a fixed launch shape
+(--variant, --wpc), so that a profiler sees no other launch of that kernel.
 """
 import argparse
 import json
@@ -22,41 +32,95 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "financial-derivatives-montecarlo_amd"))
 
-FLOPS_PER_PATH_STEP = 31        # C2 forward: Euler 3 + BB 6 + Asian 1 + geo 1 + ICDF 20
+# Algorithmic FP32 flops per fine path-step (counts fixed before tuning, FMA = 2):
+# book, adjoint Greeks: forward 36 (local vol ln + square + FMA + Euler 9, ICDF 20,
+# bridge 6, payoff 1) + reverse 44 (forward recomputed 36 + block adjoint 8);
+# the coupled coarse leg (half a step more) is not counted.
+BOOK_FLOPS_PER_PATH_STEP = 80
+C2_FLOPS_PER_PATH_STEP = 31     # C2 forward: Euler 3 + BB 6 + Asian 1 + geo 1 + ICDF 20
 VALU_PEAK_TFLOPS = 157.3        # MI355X FP32 vector peak (packed)
 MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense FP16/BF16 matrix peak
 METRIC = "time-to-eps=1e-3 price+Greeks, batched vs plain (s); path-steps/s vs VALU peak"
+EPS = 1e-3
+BOOK_KERNEL = "k_book<LOCALVOL_QUAD, adjoint Greeks>"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(target_s=10.0):
-    """Oracle (FP32 paths, FP64 ICDF) on all host threads, same workload, fewer points."""
+def cpu_baseline(budget_s=15.0, eps=1e-2):
+    """Oracle (FP32 paths, FP64 inverse CDF and sums, forward-tangent Greeks) on
+    all host threads: the allocation loops of the book's contracts 0, 1, 2, ...
+    at eps, until budget_s has been spent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import fdmc
     import pyoracle as O
 
-    m = fdmc.GBMParams(100.0, 0.05, 0.3, 1.0)
-    a, g = fdmc.Asian(100.0), fdmc.GeomAsian(100.0)
-    E = fdmc.geom_asian_analytic(100, 100, 1, 0.05, 0.3, 256)
-    bb = O.OracleBB(256, 1.0)
-    R = 32
-    dts = []
-    for n in (256, 1024):   # two sizes: the per-call direction-table set-up cancels in the slope
-        t = time.perf_counter()
-        O.qmc_accumulate(m, a, 256, R, 42, 0, n, fdmc.NOISE_BB, bb=bb, ctrl_model=m, ctrl_payoff=g, E_ctrl=E,
-                         beta=1.0)
-        dts.append(time.perf_counter() - t)
-    per_point = max(1e-9, (dts[1] - dts[0]) / 768)
-    n2 = max(1024, int(target_s / per_point) // 64 * 64)
+    model, book = fdmc.local_vol_book(1024, 2026)
+    cfg = fdmc.BookConfig(R=16, greeks=True)
     t = time.perf_counter()
-    O.qmc_accumulate(m, a, 256, R, 42, 0, n2, fdmc.NOISE_BB, bb=bb, ctrl_model=m, ctrl_payoff=g, E_ctrl=E, beta=1.0)
+    ps, n = 0, 0
+    while n < len(book) and time.perf_counter() - t < budget_s:
+        ps += O.book_contract_run(model, book[n], n, eps, cfg).path_steps
+        n += 1
     dt = time.perf_counter() - t
-    ps = R * n2 * 256
     return {"value": ps / dt, "unit": "path-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": "R=32 x %d points x 256 steps (BB + CV), %.1f s" % (n2, dt)}
+            "sample": "contracts 0..%d of the book at eps=%g (price + Greeks), %.3g path-steps in %.1f s"
+                      % (n - 1, eps, ps, dt)}
+
+
+def c2_asian(eng, steps=5):
+    """Config 2: RQMC arithmetic Asian, GBM, D = 256 bridge, R = 32, geometric CV:
+    one fused pass over 2^20 points of every replica, and the time to eps = 1e-3."""
+    import fdmc
+
+    n_steps, R, N = 256, 32, 1 << 20
+    m = fdmc.GBMParams(100.0, 0.05, 0.3, 1.0)
+    asian, geo = fdmc.Asian(100.0), fdmc.GeomAsian(100.0)
+    E = fdmc.geom_asian_analytic(100, 100, 1, 0.05, 0.3, n_steps)
+    bb = eng.bb_upload(fdmc.bb_precompute(n_steps, 1.0))
+
+    def step():
+        return eng.qmc_accumulate(m, asian, n_steps, R, 42, 0, N, fdmc.NOISE_BB, dev_bb=bb,
+                                  ctrl_model=m, ctrl_payoff=geo, E_ctrl=E, beta=1.0)
+
+    step()
+    eng.synchronize()
+    eng.timing_enable(True)
+    eng.timing_read(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sums = step()
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    launches, kms = eng.timing_read(reset=True)
+    eng.timing_enable(False)
+    path_steps = R * N * n_steps
+    k_avg_ms = kms / max(1, launches)
+    achieved = path_steps * C2_FLOPS_PER_PATH_STEP / (k_avg_ms * 1e-3) / 1e12
+    cfg = fdmc.QMCConfig(R=32, N0=1 << 14, N_max=1 << 20)
+    tte = []
+    for _ in range(4):
+        r = eng.run_qmc_cv_cuda(m, m, asian, geo, E, 1.0, EPS, n_steps, cfg, fdmc.NOISE_BB, dev_bb=bb)
+        tte.append(r.time_s)
+    dE = fdmc.geom_asian_greeks(100, 100, 1, 0.05, 0.3, n_steps)
+    ttg = []
+    for _ in range(4):
+        rg, g = eng.run_qmc_greeks(m, asian, EPS, n_steps, cfg, fdmc.NOISE_BB, bb, ctrl_model=m, ctrl_payoff=geo,
+                                   E_ctrl=E, dE_ctrl=dE, beta=1.0)
+        ttg.append(rg.time_s)
+    bb.free()
+    return {"workload": "GBM S0=K=100 r=0.05 sigma=0.3 T=1, D=256 bridge, R=32 x 2^20 points, geometric CV",
+            "path_steps_per_s": path_steps * steps / wall, "ms_per_pass": 1e3 * wall / steps,
+            "price": sum(x[2] for x in sums) / (R * N),
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / VALU_PEAK_TFLOPS,
+                         "kernel": "k_path<GBM,Asian,CV_SHARED,GeomAsian,SOBOL_BB,LB=4>",
+                         "kernel_avg_ms": k_avg_ms, "flops_per_path_step": C2_FLOPS_PER_PATH_STEP},
+            "time_to_eps": {"eps": EPS, "price_s": sorted(tte[1:])[1], "price": r.price, "std_error": r.std_error,
+                            "price_and_greeks_s": sorted(ttg[1:])[1],
+                            "greeks": {"delta": g.delta, "vega": g.vega, "rho": g.rho}}}
 
 
 def c3_gemm(eng, reps=5):
@@ -120,13 +184,34 @@ def c4_heston_mlmc(eng):
             "levels": r.levels, "path_steps": r.path_steps, "path_steps_per_s": r.path_steps / sorted(ts)[1]}
 
 
+def book_roofline(eng, book):
+    """One event-timed run of the book: HIP events around every launch of the
+    persistent kernel on the engine stream."""
+    eng.timing_enable(True)
+    eng.timing_read(reset=True)
+    res, st = book.run(EPS)
+    launches, kms = eng.timing_read(reset=True)
+    eng.timing_enable(False)
+    k_avg_ms = kms / max(1, launches)
+    achieved = st.path_steps * BOOK_FLOPS_PER_PATH_STEP / (kms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / VALU_PEAK_TFLOPS, "traffic": None, "kernel": BOOK_KERNEL,
+            "kernel_avg_ms": k_avg_ms, "kernel_launches": launches, "kernel_total_ms": kms,
+            "path_steps": st.path_steps, "flops_per_path_step": BOOK_FLOPS_PER_PATH_STEP,
+            "variant": st.variant, "wg_per_cu": st.wg_per_cu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--points", type=int, default=1 << 20)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--contracts", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the c2/c3/c4 fields")
+    ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--wpc", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -134,79 +219,83 @@ def main():
 
     import fdmc
 
-    n_steps, R, N = 256, 32, args.points
-    m = fdmc.GBMParams(100.0, 0.05, 0.3, 1.0)
-    asian, geo = fdmc.Asian(100.0), fdmc.GeomAsian(100.0)
-    E = fdmc.geom_asian_analytic(100, 100, 1, 0.05, 0.3, n_steps)
+    model, contracts = fdmc.local_vol_book(args.contracts, 2026)
     eng = fdmc.Engine(0)
-    bb = eng.bb_upload(fdmc.bb_precompute(n_steps, 1.0))
+    if args.roofline_only:
+        book = eng.book_create(model, contracts, fdmc.BookConfig(R=16, greeks=1, variant=args.variant,
+                                                                 wg_per_cu=args.wpc))
+        rl = book_roofline(eng, book)
+        book.free()
+        eng.close()
+        print(json.dumps({"roofline": rl}))
+        return
 
-    def step():
-        return eng.qmc_accumulate(m, asian, n_steps, R, 42, 0, N, fdmc.NOISE_BB, dev_bb=bb,
-                                  ctrl_model=m, ctrl_payoff=geo, E_ctrl=E, beta=1.0)
-
+    book = eng.book_create(model, contracts, fdmc.BookConfig(R=16, greeks=1))
     log("warmup", args.warmup)
+    tune_s = 0.0
     for _ in range(args.warmup):
-        sums = step()
+        res, st = book.run(EPS)
+        tune_s = st.tune_s
     eng.synchronize()
-    eng.timing_enable(True)
-    eng.timing_read(reset=True)
     log("timed steps", args.steps)
+    walls = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sums = step()
+        t1 = time.perf_counter()
+        res, st = book.run(EPS)
+        walls.append(time.perf_counter() - t1)
     eng.synchronize()
     wall = time.perf_counter() - t0
-    launches, kms = eng.timing_read(reset=True)
-    eng.timing_enable(False)
-    path_steps = R * N * n_steps
+    path_steps = st.path_steps
     ms_per_step = 1e3 * wall / args.steps
     value = path_steps * args.steps / wall
-    k_avg_ms = kms / max(1, launches)
-    achieved = path_steps * FLOPS_PER_PATH_STEP / (k_avg_ms * 1e-3) / 1e12
-    price = sum(s[2] for s in sums) / (R * N)
-    log("kernel avg %.3f ms, wall/step %.3f ms, price %.6f" % (k_avg_ms, ms_per_step, price))
-
-    # time to eps = 1e-3 (price), full doubling loop from 2^14 points per replica
-    cfg = fdmc.QMCConfig(R=32, N0=1 << 14, N_max=1 << 20)
-    tte = []
-    for _ in range(4):
-        r = eng.run_qmc_cv_cuda(m, m, asian, geo, E, 1.0, 1e-3, n_steps, cfg, fdmc.NOISE_BB, dev_bb=bb)
-        tte.append(r.time_s)
-    tte = sorted(tte[1:])[1]
-    # same, with price + adjoint Greeks (delta, vega, rho) and the CV on the Greeks
-    dE = fdmc.geom_asian_greeks(100, 100, 1, 0.05, 0.3, n_steps)
-    ttg = []
-    for _ in range(4):
-        rg, g = eng.run_qmc_greeks(m, asian, 1e-3, n_steps, cfg, fdmc.NOISE_BB, bb, ctrl_model=m, ctrl_payoff=geo,
-                                   E_ctrl=E, dE_ctrl=dE, beta=1.0)
-        ttg.append(rg.time_s)
-    ttg = sorted(ttg[1:])[1]
-    bb.free()
-    log("c3 gemm")
-    c3 = c3_gemm(eng)
-    log("c4 heston mlmc")
-    c4 = c4_heston_mlmc(eng)
-    eng.close()
-
+    log("batched %.3f s per run, %.4g path-steps" % (ms_per_step / 1e3, path_steps))
+    log("roofline run")
+    rl = book_roofline(eng, book)
+    log("plain run")
+    t1 = time.perf_counter()
+    pres, pst = book.run(EPS, fdmc.BOOK_PLAIN)
+    plain_s = time.perf_counter() - t1
+    same = all(tuple(x) == tuple(y) or all(u == v or (u != u and v != v) for u, v in zip(x, y))
+               for x, y in zip(res, pres))
+    book.free()
+    log("tangent-mode book")
+    tb = eng.book_create(model, contracts, fdmc.BookConfig(R=16, greeks=2))
+    tb.run(EPS)
+    t1 = time.perf_counter()
+    tres, tst = tb.run(EPS)
+    tangent_s = time.perf_counter() - t1
+    tb.free()
+    nonconv = sum(1 for r in res if r.flags)
+    max_se_ratio = max(r.std_error for r in res) / (EPS / 2 ** 0.5)
+    lip = [r for r in res if r.delta == r.delta]
     out = {
         "metric": METRIC, "value": value, "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (scrambled Sobol points, seeded salts)",
-        "config": {"workload": "RQMC arithmetic-Asian call, GBM S0=K=100 r=0.05 sigma=0.3 T=1, D=256 Brownian "
-                               "bridge, R=32 replicas x %d points, geometric-Asian control variate" % N,
-                   "path_steps_per_step": path_steps, "replicas": R, "points_per_replica": N, "dates": n_steps},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / VALU_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_path<GBM,Asian,CV_SHARED,GeomAsian,SOBOL_BB,LB=4>",
-                     "kernel_avg_ms": k_avg_ms, "flops_per_path_step": FLOPS_PER_PATH_STEP},
-        "time_to_eps": {"eps": 1e-3, "price_s": tte, "price": r.price, "std_error": r.std_error,
-                        "n_samples": r.n_samples, "price_and_greeks_s": ttg,
-                        "greeks": {"delta": g.delta, "vega": g.vega, "rho": g.rho, "delta_se": g.delta_se,
-                                   "vega_se": g.vega_se, "rho_se": g.rho_se}},
-        "c3_pca_gemm": c3,
-        "c4_heston_mlmc": c4,
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded contract book, scrambled Sobol points)",
+        "config": {"workload": "batched MLQMC book: %d contracts (seed 2026) under sigma(S,t)=0.2+0.1e^-t ln(S/100)^2,"
+                               " European/Asian/lookback/barrier, K~U[70,130], T in {0.25,0.5,1,2}, 16-512 dates,"
+                               " R=16, eps=1e-3 per contract, pathwise-adjoint delta/vega/rho" % args.contracts,
+                   "contracts": args.contracts, "replicas": 16, "eps": EPS, "path_steps_per_step": path_steps,
+                   "passes_per_step": st.passes},
+        "roofline": rl,
+        "time_to_eps": {"eps": EPS, "batched_s": ms_per_step / 1e3, "plain_s": plain_s,
+                        "speedup_batched_vs_plain": plain_s / (ms_per_step / 1e3),
+                        "batched_run_s": sorted(walls), "plain_passes": pst.passes,
+                        "batched_graph_replays": st.graph_replays, "batched_equals_plain": same,
+                        "tangent_greeks_batched_s": tangent_s, "autotune_s": tune_s,
+                        "variant": st.variant, "wg_per_cu": st.wg_per_cu,
+                        "contracts_not_converged": nonconv, "max_se_over_target": max_se_ratio,
+                        "contracts_with_greeks": len(lip)},
     }
+    if not args.no_extras:
+        log("c2 asian")
+        out["c2_asian_rqmc"] = c2_asian(eng)
+        log("c3 gemm")
+        out["c3_pca_gemm"] = c3_gemm(eng)
+        log("c4 heston mlmc")
+        out["c4_heston_mlmc"] = c4_heston_mlmc(eng)
+    eng.close()
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -36,13 +36,13 @@ struct fdmc_book {
     int* h_active = nullptr;          // pinned
     // launch shape (autotuned on the first run)
     bool tuned = false;
-    int skg = 0, wg_per_cu = 2, items_target = 512, max_grid = 0;
+    int skg = 0, occ = 0, wg_per_cu = 2, items_target = 512, max_grid = 0;
     double tune_s = 0.0;
     // captured passes of the batched mode
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     double graph_eps = -1.0;
-    int graph_skg = -1, graph_grid = -1;
+    int graph_skg = -1, graph_occ = -1, graph_grid = -1;
 
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -141,9 +141,10 @@ void prepare_kernel(const void* fn, int skg) {
 
 // Launch-shape autotuner: times one sweep of every entry of the book at a fixed
 // size (512 points per (contract, level, replica)) for each candidate
-// (skeleton in LDS or in global scratch) x (persistent workgroups per CU) and
-// keeps the fastest.  Only the launch shape is tuned: the split of entries into
-// work items (items_target) is fixed, so results do not depend on the choice.
+// (skeleton in LDS or in global scratch) x (default or three-waves-per-SIMD
+// compilation) x (persistent workgroups per CU) and keeps the fastest.  Only
+// the launch shape is tuned: the split of entries into work items
+// (items_target) is fixed, so results do not depend on the choice.
 void autotune(fdmc_book* b) {
     const double t0 = now_s();
     fdmc_context* c = b->ctx;
@@ -157,29 +158,31 @@ void autotune(fdmc_book* b) {
     HIPC(hipEventCreate(&ev0));
     HIPC(hipEventCreate(&ev1));
     float best = 1e30f;
-    int best_skg = 0, best_wpc = 1;
+    int best_skg = 0, best_occ = 0, best_wpc = 1;
     try {
-        for (int skg = 0; skg < 2; skg++) {
-            const void* fn = book_kernel(b->mk, skg, b->greeks);
-            prepare_kernel(fn, skg);
-            const int occ = std::min(occupancy(fn, skg), 4);
-            for (int wpc = 1; wpc <= occ; wpc++) {
-                const int grid = std::min(c->n_cu * wpc, b->max_grid);
-                BookLaunch a = make_launch(b, false, 0, 1.0);
-                HIPC(launch_book_sweep(fn, skg, a, grid, c->stream));   // warm-up
-                HIPC(hipEventRecord(ev0, c->stream));
-                HIPC(launch_book_sweep(fn, skg, a, grid, c->stream));
-                HIPC(hipEventRecord(ev1, c->stream));
-                HIPC(hipEventSynchronize(ev1));
-                float ms = 0.0f;
-                HIPC(hipEventElapsedTime(&ms, ev0, ev1));
-                if (ms < best) {
-                    best = ms;
-                    best_skg = skg;
-                    best_wpc = wpc;
+        for (int occ = 0; occ < 2; occ++)
+            for (int skg = 0; skg < 2; skg++) {
+                const void* fn = book_kernel(b->mk, skg, b->greeks, occ);
+                prepare_kernel(fn, skg);
+                const int nb = std::min(occupancy(fn, skg), 4);
+                for (int wpc = 1; wpc <= nb; wpc++) {
+                    const int grid = std::min(c->n_cu * wpc, b->max_grid);
+                    BookLaunch a = make_launch(b, false, 0, 1.0);
+                    HIPC(launch_book_sweep(fn, skg, a, grid, c->stream));   // warm-up
+                    HIPC(hipEventRecord(ev0, c->stream));
+                    HIPC(launch_book_sweep(fn, skg, a, grid, c->stream));
+                    HIPC(hipEventRecord(ev1, c->stream));
+                    HIPC(hipEventSynchronize(ev1));
+                    float ms = 0.0f;
+                    HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+                    if (ms < best) {
+                        best = ms;
+                        best_skg = skg;
+                        best_occ = occ;
+                        best_wpc = wpc;
+                    }
                 }
             }
-        }
     } catch (...) {
         (void)hipEventDestroy(ev0);
         (void)hipEventDestroy(ev1);
@@ -188,6 +191,7 @@ void autotune(fdmc_book* b) {
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     b->skg = best_skg;
+    b->occ = best_occ;
     b->wg_per_cu = best_wpc;
     b->tuned = true;
     b->tune_s = now_s() - t0;
@@ -221,12 +225,42 @@ void read_results(fdmc_book* b, fdmc_book_result* res, int64_t* total_steps) {
     *total_steps = tot;
 }
 
-void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
+// With context timing on: the same passes launched one by one (no graph), the
+// persistent kernel of each pass bracketed by HIP events.
+void run_batched_timed(fdmc_book* b, double eps, fdmc_book_stats* st) {
     fdmc_context* c = b->ctx;
     const int grid = book_grid(b);
-    const void* fn = book_kernel(b->mk, b->skg, b->greeks);
+    const void* fn = book_kernel(b->mk, b->skg, b->greeks, b->occ);
     BookLaunch a = make_launch(b, false, 0, eps);
-    if (!b->graph_exec || b->graph_eps != eps || b->graph_skg != b->skg || b->graph_grid != grid) {
+    const double t0 = now_s();
+    HIPC(launch_book_reset(a, c->stream));
+    int passes = 0;
+    for (int it = 0; it <= b->max_iter; it++) {
+        HIPC(launch_book_pass(fn, b->skg, a, grid, c->stream, c->ev0, c->ev1));
+        HIPC(launch_book_status(a, c->stream));
+        HIPC(hipMemcpyAsync(b->h_active, a.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        float ms = 0.0f;
+        HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->timed_ms += ms;
+        c->timed_launches++;
+        passes++;
+        if (*b->h_active == 0) break;
+    }
+    HIPC(launch_book_final(a, c->stream));
+    st->time_s = now_s() - t0;
+    st->graph_replays = 0;
+    st->passes = passes;
+}
+
+void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
+    fdmc_context* c = b->ctx;
+    if (c->timing) return run_batched_timed(b, eps, st);
+    const int grid = book_grid(b);
+    const void* fn = book_kernel(b->mk, b->skg, b->greeks, b->occ);
+    BookLaunch a = make_launch(b, false, 0, eps);
+    if (!b->graph_exec || b->graph_eps != eps || b->graph_skg != b->skg || b->graph_occ != b->occ ||
+        b->graph_grid != grid) {
         b->drop_graph();
         HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
@@ -243,6 +277,7 @@ void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
         HIPC(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
         b->graph_eps = eps;
         b->graph_skg = b->skg;
+        b->graph_occ = b->occ;
         b->graph_grid = grid;
     }
     const double t0 = now_s();
@@ -250,17 +285,9 @@ void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
     int replays = 0;
     const int max_replays = (b->max_iter + b->ipg) / b->ipg + 1;
     for (;;) {
-        if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
         HIPC(hipGraphLaunch(b->graph_exec, c->stream));
-        if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
         HIPC(hipMemcpyAsync(b->h_active, a.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
-        if (c->timing) {
-            float ms = 0.0f;
-            HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            c->timed_ms += ms;
-            c->timed_launches++;
-        }
         replays++;
         if (*b->h_active == 0 || replays >= max_replays) break;
     }
@@ -273,7 +300,7 @@ void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
 void run_plain(fdmc_book* b, double eps, fdmc_book_stats* st) {
     fdmc_context* c = b->ctx;
     const int grid = book_grid(b);
-    const void* fn = book_kernel(b->mk, b->skg, b->greeks);
+    const void* fn = book_kernel(b->mk, b->skg, b->greeks, b->occ);
     const double t0 = now_s();
     BookLaunch all = make_launch(b, false, 0, eps);
     HIPC(launch_book_reset(all, c->stream));
@@ -281,10 +308,9 @@ void run_plain(fdmc_book* b, double eps, fdmc_book_stats* st) {
     for (int k = 0; k < b->n; k++) {
         BookLaunch a = make_launch(b, true, k, eps);
         for (int it = 0; it <= b->max_iter; it++) {
-            if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
-            HIPC(launch_book_pass(fn, b->skg, a, grid, c->stream));
+            HIPC(launch_book_pass(fn, b->skg, a, grid, c->stream, c->timing ? c->ev0 : nullptr,
+                                  c->timing ? c->ev1 : nullptr));
             HIPC(launch_book_status(a, c->stream));
-            if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
             HIPC(hipMemcpyAsync(b->h_active, a.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HIPC(hipStreamSynchronize(c->stream));
             if (c->timing) {
@@ -323,6 +349,9 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
         if (cfg->max_iter < 1) fail(FDMC_E_INVALID, "max_iter must be >= 1");
         if (cfg->iters_per_graph < 1 || cfg->iters_per_graph > 64)
             fail(FDMC_E_INVALID, "iters_per_graph must be in [1, 64]");
+        if (cfg->variant < -1 || cfg->variant > 3) fail(FDMC_E_INVALID, "variant must be -1 (autotune) or 0..3");
+        if (cfg->variant >= 0 && (cfg->wg_per_cu < 1 || cfg->wg_per_cu > 4))
+            fail(FDMC_E_INVALID, "wg_per_cu must be in [1, 4] with a fixed variant");
         std::vector<double> Ts;
         for (int i = 0; i < n; i++) {
             const fdmc_book_contract& k = contracts[i];
@@ -355,6 +384,12 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
             b->ipg = cfg->iters_per_graph;
             b->salt_base = cfg->salt_base;
             b->mk = model_kind_dev(*model);
+            if (cfg->variant >= 0) {
+                b->skg = cfg->variant & 1;
+                b->occ = cfg->variant >> 1;
+                b->wg_per_cu = cfg->wg_per_cu;
+                b->tuned = true;
+            }
             fdmc_payoff p{};
             p.kind = FDMC_PAYOFF_EUROPEAN;
             b->leg = make_leg(*model, p);
@@ -478,14 +513,14 @@ fdmc_status fdmc_book_run(fdmc_book* b, double eps, int32_t mode, fdmc_book_resu
         HIPC(hipSetDevice(b->ctx->device));
         fdmc_book_stats st{};
         if (!b->tuned) autotune(b);
-        for (int skg = 0; skg < 2; skg++) prepare_kernel(book_kernel(b->mk, skg, b->greeks), skg);
+        prepare_kernel(book_kernel(b->mk, b->skg, b->greeks, b->occ), b->skg);
         if (mode == FDMC_BOOK_BATCHED) run_batched(b, eps, &st);
         else run_plain(b, eps, &st);
         const double t0 = now_s();
         read_results(b, results, &st.path_steps);
         st.time_s += now_s() - t0;
         st.tune_s = b->tune_s;
-        st.variant = b->skg;
+        st.variant = b->skg + 2 * b->occ;
         st.items_target = b->items_target;
         st.wg_per_cu = b->wg_per_cu;
         if (stats) *stats = st;
@@ -504,7 +539,7 @@ fdmc_status fdmc_book_level_sums(fdmc_book* b, int32_t c, int32_t l, int64_t off
         HIPC(hipSetDevice(b->ctx->device));
         fdmc_context* ctx = b->ctx;
         if (!b->tuned) autotune(b);
-        prepare_kernel(book_kernel(b->mk, b->skg, b->greeks), b->skg);
+        prepare_kernel(book_kernel(b->mk, b->skg, b->greeks, b->occ), b->skg);
         const int ne = (C.L + 1) * b->R;
         std::vector<BookEntry> e(b->h_plain.begin() + C.entry0, b->h_plain.begin() + C.entry0 + ne);
         for (auto& x : e) {
@@ -513,7 +548,7 @@ fdmc_status fdmc_book_level_sums(fdmc_book* b, int32_t c, int32_t l, int64_t off
         }
         BookLaunch a = make_launch(b, true, c, 1.0);
         HIPC(hipMemcpyAsync(a.entries, e.data(), e.size() * sizeof(BookEntry), hipMemcpyHostToDevice, ctx->stream));
-        HIPC(launch_book_sweep(book_kernel(b->mk, b->skg, b->greeks), b->skg, a, book_grid(b), ctx->stream));
+        HIPC(launch_book_sweep(book_kernel(b->mk, b->skg, b->greeks, b->occ), b->skg, a, book_grid(b), ctx->stream));
         std::vector<double> s((size_t)ne * BOOK_NACC);
         HIPC(hipMemcpyAsync(s.data(), a.entry_sums, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPC(hipStreamSynchronize(ctx->stream));

@@ -29,386 +29,9 @@
 // path with the coarse path swept alongside; S_f and S_c are checkpointed at
 // block starts in a per-workgroup global scratch, increments are regenerated
 // from the Sobol index and the bridge skeleton.
-#include "k_greeks.inl"
+#include "k_book.inl"
 
 namespace fdmc {
-
-struct BookGeom {
-    int l, nf, nc, G;
-    float hf, hc, bgk_f, bgk_c, T;
-};
-
-// Sensitivities carried along a path: dS/dS0, dS/d(vol parameter), dS/dmu.
-struct Tan3 {
-    float s, v, m;
-};
-
-// One Euler step S -> S' and, with TAN, the tangent update
-// t' = fS t + (f0, fv, fm) (step_partials are the step's partial derivatives).
-template <int MK, bool TAN>
-__device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t, float dw, float tt, float h) {
-    if constexpr (TAN) {
-        float fS, fv, fm, f0;
-        step_partials<MK>(P, S, dw, tt, h, fS, fv, fm, f0);
-        t.s = fS * t.s + f0;
-        t.v = fS * t.v + fv;
-        t.m = fS * t.m + fm;
-    }
-    S = step_1f<MK>(P, S, dw, tt, h);
-}
-
-// Running payoff state of one leg (and, with TAN, the tangent sums the
-// payoff's derivative needs: sum over dates for the Asian, the tangent at the
-// running minimum for the lookback).
-template <int MK, int PK, bool TAN>
-struct BookLeg {
-    float S, run;
-    int kmin;            // index of the running minimum (lookback), 0 = S0
-    Tan3 t, acc;         // current tangent; Asian sum / lookback tangent at the minimum
-    __device__ __forceinline__ void init(const LegParams& P) {
-        S = P.S0;
-        run = run_init<PK>(P);
-        kmin = 0;
-        t = Tan3{1.0f, 0.0f, 0.0f};
-        acc = (PK == PK_LOOKBACK) ? Tan3{1.0f, 0.0f, 0.0f} : Tan3{0.0f, 0.0f, 0.0f};
-    }
-    __device__ __forceinline__ void step(const LegParams& P, float dw, float tt, float h, int idx) {
-        book_step<MK, TAN>(P, S, t, dw, tt, h);
-        if constexpr (PK == PK_LOOKBACK) {
-            if (S < run) {
-                run = S;
-                kmin = idx;
-                if constexpr (TAN) acc = t;
-            }
-        } else {
-            run_update<PK>(run, S);
-            if constexpr (TAN && PK == PK_ASIAN) {
-                acc.s += t.s;
-                acc.v += t.v;
-                acc.m += t.m;
-            }
-        }
-    }
-};
-
-// Direct sensitivities of a payoff: `each` at every S_k (k >= 1), `last` at
-// S_n, `cmin` at the running minimum.
-template <int PK>
-__device__ __forceinline__ void book_direct(const LegParams& P, float S_T, float run, int n, float bgk, float& each,
-                                            float& last, float& cmin) {
-    each = 0.0f;
-    last = 0.0f;
-    cmin = 0.0f;
-    if constexpr (PK == PK_ASIAN) each = (run / (float)n > P.K) ? 1.0f / (float)n : 0.0f;
-    if constexpr (PK == PK_EUROPEAN) last = (S_T > P.K) ? P.discount : 0.0f;
-    if constexpr (PK == PK_LOOKBACK) {
-        last = 1.0f;
-        cmin = -bgk;
-    }
-}
-
-// Adjoint state of one leg during the reverse sweep over blocks: adj = dY/dS
-// at the current block boundary through later dates only; v, m, s0 the
-// accumulated parameter adjoints.
-struct BookAdj {
-    float adj, v, m, s0;
-};
-
-// One block of the reverse sweep: from the checkpoint S, a forward sweep of the
-// block's n steps accumulates the block Jacobian dS_end/dS_start, the
-// in-block parameter tangents and the direct payoff terms weighted by
-// dS_k/dS_start; then
-//   adj_start = sum_k d_k dS_k/dS_start + adj_end dS_end/dS_start,
-//   theta_bar += sum_k d_k dS_k/dtheta + adj_end dS_end/dtheta.
-// For a one-dimensional state this is the exact vector-Jacobian product of the
-// block, with no per-step storage.
-// PAIR: the step increments are dw[2k] + dw[2k+1] (coarse leg).
-template <int MK, int N, bool PAIR>
-__device__ __forceinline__ void book_block_adj(const LegParams& P, float S, const float* dw, int nsteps, int k0,
-                                               float h, int n, float each, float last, float cmin, int kmin,
-                                               BookAdj& A) {
-    float jS = 1.0f, jv = 0.0f, jm = 0.0f, j0 = 0.0f;
-    float dS = 0.0f, dv = 0.0f, dm = 0.0f, d0 = 0.0f;
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-        if (k < nsteps) {
-            const int gi = k0 + k;
-            const float w = PAIR ? dw[2 * k] + dw[2 * k + 1] : dw[k];
-            float fS, fv, fm, f0;
-            step_partials<MK>(P, S, w, (float)gi * h, h, fS, fv, fm, f0);
-            S = step_1f<MK>(P, S, w, (float)gi * h, h);
-            jS = fS * jS;
-            jv = fS * jv + fv;
-            jm = fS * jm + fm;
-            j0 = fS * j0 + f0;
-            const int idx = gi + 1;
-            const float d = each + (idx == n ? last : 0.0f) + (idx == kmin ? cmin : 0.0f);
-            dS += d * jS;
-            dv += d * jv;
-            dm += d * jm;
-            d0 += d * j0;
-        }
-    }
-    A.v += dv + A.adj * jv;
-    A.m += dm + A.adj * jm;
-    A.s0 += d0 + A.adj * j0;
-    A.adj = dS + A.adj * jS;
-}
-
-// Brownian bridge on the uniform grid of 2^l dates (bb_precompute,
-// utils.cpp:54-101): every midpoint weight is 1/2 and the conditional sd
-// depends only on the depth, so the bridge is W_M = (W_L + W_R) / 2 + sd[k] z
-// with sdt[0] = sqrt(T) for the terminal value and sdt[k] for the midpoints
-// of schedule level k - 1 (FP32 values of bb_precompute's weights).
-//
-// GM: 0 price only, 1 adjoint Greeks (reverse sweep over 16-step blocks with
-// checkpoints, increments regenerated from the Sobol index), 2 forward
-// tangents (carried along the forward pass; no second pass).
-template <int MK, int PK, int GM>
-__device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                          const SobolLane& sl, float* sk, float* ckf, float* ckc, bool valid,
-                                          double (&acc)[BOOK_NACC]) {
-    constexpr bool LIP = PK != PK_BARRIER;
-    constexpr bool ADJ = LIP && GM == 1;
-    constexpr bool TAN = LIP && GM == 2;
-    const int G = g.G, nf = g.nf, nc = g.nc;
-    // bridge skeleton: W at block boundaries (every date when nf < 16)
-    sk[0] = 0.0f;
-    sk[G * 256] = sdt[0] * icdf_u32(sobol_coord(sl, 0));
-    for (int j = 1; j < G; j++) {
-        const int lev = 31 - __builtin_clz(j);
-        const int q = j - (1 << lev);
-        const int st = G >> lev;
-        const int L = q * st, R = L + st, M = L + (st >> 1);
-        sk[M * 256] = 0.5f * (sk[L * 256] + sk[R * 256]) + sdt[lev + 1] * icdf_u32(sobol_coord(sl, j));
-    }
-    const int lgG = 31 - __builtin_clz(G);
-    float sdb[4];
-#pragma unroll
-    for (int lv = 0; lv < 4; lv++) sdb[lv] = nf >= 16 ? sdt[lgG + lv + 1] : 0.0f;
-    auto gen_block = [&](int i, float (&dw)[16]) {
-        float W[17];
-        W[0] = sk[i * 256];
-        W[16] = sk[(i + 1) * 256];
-#pragma unroll
-        for (int lv = 0; lv < 4; lv++) {
-            const int j0 = (G << lv) + (i << lv);
-#pragma unroll
-            for (int q = 0; q < (1 << lv); q++) {
-                const int s = 16 >> lv;
-                const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                W[Mp] = 0.5f * (W[Lp] + W[Rp]) + sdb[lv] * icdf_u32(sobol_coord(sl, j0 + q));
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 16; k++) dw[k] = W[k + 1] - W[k];
-    };
-
-    // ---- forward
-    BookLeg<MK, PK, TAN> F, Cc;
-    F.init(P);
-    Cc.init(P);
-    if (nf >= 16) {
-        for (int i = 0; i < G; i++) {
-            float dw[16];
-            gen_block(i, dw);
-            if constexpr (ADJ) {
-                ckf[i * 256] = F.S;
-                ckc[i * 256] = Cc.S;
-            }
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const int gi = i * 16 + k;
-                F.step(P, dw[k], (float)gi * g.hf, g.hf, gi + 1);
-                if (k & 1) {
-                    const int gc = gi >> 1;
-                    Cc.step(P, dw[k - 1] + dw[k], (float)gc * g.hc, g.hc, gc + 1);
-                }
-            }
-        }
-    } else {
-        float prev = 0.0f;
-        for (int k = 0; k < nf; k++) {
-            const float dw = sk[(k + 1) * 256] - sk[k * 256];
-            F.step(P, dw, (float)k * g.hf, g.hf, k + 1);
-            if (k & 1) {
-                const int gc = k >> 1;
-                Cc.step(P, prev + dw, (float)gc * g.hc, g.hc, gc + 1);
-            }
-            prev = dw;
-        }
-    }
-    const float pf = payoff<PK>(P, F.S, F.run, nf, g.bgk_f);
-    const float pc = (nc > 0) ? payoff<PK>(P, Cc.S, Cc.run, nc, g.bgk_c) : 0.0f;
-
-    // ---- Greeks of P_f and P_c: delta, vega, rho
-    Tan3 gf{0.0f, 0.0f, 0.0f}, gc{0.0f, 0.0f, 0.0f};
-    if constexpr (TAN) {
-        auto grad = [&](const BookLeg<MK, PK, TAN>& Lg, int n, float bgk, float pay) {
-            Tan3 r{0.0f, 0.0f, 0.0f};
-            if constexpr (PK == PK_EUROPEAN) {
-                if (Lg.S > P.K) r = Tan3{P.discount * Lg.t.s, P.discount * Lg.t.v, P.discount * Lg.t.m};
-                r.m -= g.T * pay;    // d/dr of the discount e^{-rT}
-            } else if constexpr (PK == PK_ASIAN) {
-                if (Lg.run / (float)n > P.K) {
-                    const float w = 1.0f / (float)n;
-                    r = Tan3{Lg.acc.s * w, Lg.acc.v * w, Lg.acc.m * w};
-                }
-            } else if constexpr (PK == PK_LOOKBACK) {
-                r = Tan3{Lg.t.s - bgk * Lg.acc.s, Lg.t.v - bgk * Lg.acc.v, Lg.t.m - bgk * Lg.acc.m};
-            }
-            return r;
-        };
-        gf = grad(F, nf, g.bgk_f, pf);
-        if (nc > 0) gc = grad(Cc, nc, g.bgk_c, pc);
-    }
-    if constexpr (ADJ) {
-        float ef, lf, mf, ec, lc, mc;
-        book_direct<PK>(P, F.S, F.run, nf, g.bgk_f, ef, lf, mf);
-        book_direct<PK>(P, Cc.S, Cc.run, nc > 0 ? nc : 1, g.bgk_c, ec, lc, mc);
-        BookAdj Af{0.0f, 0.0f, 0.0f, 0.0f}, Ac{0.0f, 0.0f, 0.0f, 0.0f};
-        if (nf >= 16) {
-            for (int i = G - 1; i >= 0; i--) {
-                float dw[16];
-                gen_block(i, dw);
-                book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
-                book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, g.hf, nf, ef, lf, mf, F.kmin, Af);
-            }
-        } else {
-            float dw[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) dw[k] = (k < nf) ? sk[(k + 1) * 256] - sk[k * 256] : 0.0f;
-            book_block_adj<MK, 4, true>(P, P.S0, dw, nc, 0, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
-            book_block_adj<MK, 8, false>(P, P.S0, dw, nf, 0, g.hf, nf, ef, lf, mf, F.kmin, Af);
-        }
-        if (F.kmin == 0) Af.adj += mf;     // S0 itself is the running minimum
-        if (Cc.kmin == 0) Ac.adj += mc;
-        gf = Tan3{Af.adj + Af.s0, Af.v, Af.m};
-        gc = Tan3{Ac.adj + Ac.s0, Ac.v, Ac.m};
-        if constexpr (PK == PK_EUROPEAN) {
-            gf.m -= g.T * pf;       // d/dr of the discount e^{-rT}
-            gc.m -= g.T * pc;
-        }
-        if (nc == 0) gc = Tan3{0.0f, 0.0f, 0.0f};
-    }
-    if (valid) {
-        acc[0] += (double)(pf - pc);
-        acc[1] += (double)(gf.s - gc.s);
-        acc[2] += (double)(gf.v - gc.v);
-        acc[3] += (double)(gf.m - gc.m);
-    }
-}
-
-// SKG = 0: bridge skeleton in LDS; 1: in the workgroup's global scratch (less
-// LDS per workgroup, more workgroups per CU).
-template <int MK, int SKG, int GM>
-__global__ __launch_bounds__(256) void k_book(BookLaunch a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int tid = threadIdx.x, lane = tid & 63;
-    uint32_t* rec = (uint32_t*)smem;                                     // [BOOK_DMAX][20]
-    constexpr size_t REC_BYTES = (size_t)BOOK_DMAX * 20 * 4;
-    constexpr size_t SKEL_BYTES = SKG ? 0 : (size_t)(BOOK_GMAX + 1) * 256 * 4;
-    double* red = (double*)(smem + REC_BYTES + SKEL_BYTES);
-    int* s_item = (int*)(red + 4 * BOOK_NACC);
-    float* wscr = a.scratch + (size_t)blockIdx.x * book_scratch_floats(SKG);
-    float* ckf = wscr + tid;
-    float* ckc = ckf + BOOK_GMAX * 256;
-    float* sk;
-    if constexpr (SKG) sk = wscr + 2 * BOOK_GMAX * 256 + tid;
-    else sk = (float*)(smem + REC_BYTES) + tid;
-
-    SobolLane sl;
-    {
-        const uint32_t gl = (uint32_t)(lane ^ (lane >> 1));
-        sl.rec = rec;
-        sl.grec = nullptr;
-        sl.ga = gl & 7u;
-        sl.gb = gl >> 3;
-        sl.w = (uint32_t)(tid >> 6);
-        sl.hi_gray = 0;
-        sl.lds_mode = true;
-    }
-    const int total = *a.total_items;     // written by k_book_scan
-    int last_e = -1;
-    for (;;) {
-        __syncthreads();
-        if (tid == 0) *s_item = (int)atomicAdd(a.queue, 1u);
-        __syncthreads();
-        const int item = *s_item;
-        if (item >= total) break;         // every workgroup leaves once the queue is empty
-        int lo = 0, hi = a.n_entries - 1; // last entry with tile_begin <= item
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (a.entries[mid].tile_begin <= item) lo = mid;
-            else hi = mid - 1;
-        }
-        const BookEntry E = a.entries[lo];
-        const BookContract& C = a.contracts[E.c];
-        BookGeom g;
-        g.l = E.l;
-        g.nf = 1 << E.l;
-        g.nc = g.nf >> 1;
-        g.G = g.nf >= 16 ? g.nf >> 4 : g.nf;
-        g.hf = (float)(C.T / (double)g.nf);
-        g.hc = g.nc > 0 ? (float)(C.T / (double)g.nc) : 0.0f;
-        g.bgk_f = C.bgk[E.l];
-        g.bgk_c = E.l > 0 ? C.bgk[E.l - 1] : 1.0f;
-        g.T = C.Tf;
-        const int kind = C.kind;
-        LegParams P = a.leg;
-        P.K = C.K;
-        P.B = C.B;
-        P.discount = C.discount;
-        const int D = g.nf;
-        const uint32_t* grec = a.sdir[E.l] + ((size_t)C.slot[E.l] * a.R + E.r) * (size_t)D * SDIR_WORDS;
-        const float* sdt = a.bbtab + ((size_t)C.t_idx * BOOK_NL + E.l) * BOOK_NL;
-        const int64_t tfirst = E.first >> 8;
-        const int nt_all = (int)(((E.first + E.count - 1) >> 8) - tfirst + 1);
-        const int t0 = (item - E.tile_begin) * E.tpi;
-        const int t1 = min(t0 + E.tpi, nt_all);
-        double acc[BOOK_NACC];
-#pragma unroll
-        for (int i = 0; i < BOOK_NACC; i++) acc[i] = 0.0;
-        for (int t = t0; t < t1; t++) {
-            const int64_t base = (tfirst + t) << 8;
-            if (t > t0) __syncthreads();  // every lane is done with the previous tile's table
-            if (lo != last_e) {
-                for (int i = tid; i < D * 16; i += 256) {
-                    const int d = i >> 4, k = i & 15;
-                    rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
-                }
-                last_e = lo;
-            }
-            {
-                const uint32_t b32 = (uint32_t)base, gbase = b32 ^ (b32 >> 1);
-                for (int d = tid; d < D; d += 256) {
-                    const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-                    uint32_t s = 0;
-                    for (uint32_t x = gbase; x; x &= x - 1u) s ^= gr[32 + __builtin_ctz(x)];
-                    rec[d * 20 + 16] = s ^ gr[16];
-                    rec[d * 20 + 17] = s ^ gr[17];
-                    rec[d * 20 + 18] = s ^ gr[18];
-                    rec[d * 20 + 19] = s ^ gr[19];
-                }
-            }
-            __syncthreads();
-            const int64_t idx = base + tid;
-            const bool valid = idx >= E.first && idx < E.first + E.count;
-            switch (kind) {
-            case PK_EUROPEAN: book_path<MK, PK_EUROPEAN, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
-            case PK_ASIAN: book_path<MK, PK_ASIAN, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
-            case PK_LOOKBACK: book_path<MK, PK_LOOKBACK, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
-            default: book_path<MK, PK_BARRIER, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
-            }
-        }
-        block_sum<BOOK_NACC, 4>(acc, red);
-        if (tid == 0) {
-            double* out = a.partials + (int64_t)item * BOOK_NACC;
-#pragma unroll
-            for (int i = 0; i < BOOK_NACC; i++) out[i] = acc[i];
-        }
-    }
-}
 
 // ---- the allocation loop ---------------------------------------------------------
 // 0: the contract needs a pass (extra[] filled); 1: converged; 2: out of passes.
@@ -627,26 +250,13 @@ size_t book_lds_bytes(int skg) {
     return (size_t)BOOK_DMAX * 20 * 4 + (skg ? 0 : (size_t)(BOOK_GMAX + 1) * 256 * 4) + 4 * BOOK_NACC * 8 + 16;
 }
 
-template <int MK>
-const void* book_kernel_mk(int skg, int gm) {
-    switch (gm * 2 + skg) {
-    case 0: return (const void*)&k_book<MK, 0, 0>;
-    case 1: return (const void*)&k_book<MK, 1, 0>;
-    case 2: return (const void*)&k_book<MK, 0, 1>;
-    case 3: return (const void*)&k_book<MK, 1, 1>;
-    case 4: return (const void*)&k_book<MK, 0, 2>;
-    case 5: return (const void*)&k_book<MK, 1, 2>;
-    default: return nullptr;
-    }
-}
+template <int MK, int SKG, int GM>
+struct BookDefault {
+    static const void* fn() { return (const void*)&k_book<MK, SKG, GM>; }
+};
 
-const void* book_kernel(int mk, int skg, int gm) {
-    switch (mk) {
-    case MK_GBM: return book_kernel_mk<MK_GBM>(skg, gm);
-    case MK_DUPIRE: return book_kernel_mk<MK_DUPIRE>(skg, gm);
-    case MK_LVQ: return book_kernel_mk<MK_LVQ>(skg, gm);
-    default: return nullptr;
-    }
+const void* book_kernel(int mk, int skg, int gm, int occ) {
+    return occ ? book_kernel_o3(mk, skg, gm) : book_kernel_any<BookDefault>(mk, skg, gm);
 }
 
 hipError_t launch_book_reset(const BookLaunch& a, hipStream_t s) {
@@ -659,19 +269,25 @@ hipError_t launch_book_plan(const BookLaunch& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// scan -> persistent pass -> reduce (no planning or merging: level-sum hook and tuner)
-hipError_t launch_book_sweep(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s) {
+// scan -> persistent pass -> reduce (no planning or merging: level-sum hook and
+// tuner).  With events e0/e1 the persistent kernel alone is bracketed by them.
+hipError_t launch_book_sweep(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s, hipEvent_t e0,
+                             hipEvent_t e1) {
     hipLaunchKernelGGL(k_book_scan, dim3(1), dim3(1024), 0, s, a);
+    hipError_t e = hipSuccess;
+    if (e0 && (e = hipEventRecord(e0, s)) != hipSuccess) return e;
     void* args[] = {(void*)&a};
-    hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, book_lds_bytes(skg), s);
+    e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, book_lds_bytes(skg), s);
     if (e != hipSuccess) return e;
+    if (e1 && (e = hipEventRecord(e1, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_book_reduce, dim3((a.n_entries + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_book_pass(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s) {
+hipError_t launch_book_pass(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s, hipEvent_t e0,
+                            hipEvent_t e1) {
     hipLaunchKernelGGL(k_book_plan, dim3((a.n_contracts + 63) / 64), dim3(64), 0, s, a);
-    hipError_t e = launch_book_sweep(fn, skg, a, grid, s);
+    hipError_t e = launch_book_sweep(fn, skg, a, grid, s, e0, e1);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_book_merge, dim3((a.n_contracts * BOOK_NL + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();

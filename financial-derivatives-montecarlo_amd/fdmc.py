@@ -85,7 +85,8 @@ class CBookContract(C.Structure):
 
 class CBookConfig(C.Structure):
     _fields_ = [("R", C.c_int32), ("pilot_n", C.c_int32), ("salt_base", C.c_uint32), ("greeks", C.c_int32),
-                ("max_iter", C.c_int32), ("iters_per_graph", C.c_int32)]
+                ("max_iter", C.c_int32), ("iters_per_graph", C.c_int32), ("variant", C.c_int32),
+                ("wg_per_cu", C.c_int32)]
 
 
 class CBookResult(C.Structure):
@@ -335,14 +336,16 @@ class BookContract:
 
 
 class BookConfig:
-    def __init__(self, R=16, pilot_n=400, salt_base=42, greeks=True, max_iter=50, iters_per_graph=4):
+    def __init__(self, R=16, pilot_n=400, salt_base=42, greeks=True, max_iter=50, iters_per_graph=4, variant=-1,
+                 wg_per_cu=0):
         self.R, self.pilot_n, self.salt_base, self.greeks = R, pilot_n, salt_base, greeks
         self.max_iter, self.iters_per_graph = max_iter, iters_per_graph
+        self.variant, self.wg_per_cu = variant, wg_per_cu
 
     def to_c(self):
         # greeks: False/0 none, True/1 adjoint, 2 forward tangents
         return CBookConfig(self.R, self.pilot_n, self.salt_base, int(self.greeks), self.max_iter,
-                           self.iters_per_graph)
+                           self.iters_per_graph, self.variant, self.wg_per_cu)
 
 
 def local_vol_book(n=1024, seed=2026, r=0.05):

@@ -170,6 +170,9 @@ typedef struct fdmc_book_config {
                                     2 forward tangents (same derivative, one pass) */
     int32_t max_iter;            /* passes per contract (pilot included), default 50 */
     int32_t iters_per_graph;     /* batched mode: passes captured per hipGraph, default 4 */
+    int32_t variant;             /* launch shape: -1 autotune on the first run (default), else the
+                                    kernel variant of fdmc_book_stats.variant ... */
+    int32_t wg_per_cu;           /* ... and persistent workgroups per CU (variant >= 0 only) */
 } fdmc_book_config;
 
 typedef enum { FDMC_BOOK_BATCHED = 0,  /* all contracts per pass, device-side planning, hipGraph */
@@ -194,7 +197,8 @@ typedef struct fdmc_book_stats {
     int64_t path_steps;          /* all contracts */
     int32_t passes;              /* device passes (batched) or host iterations (plain) */
     int32_t graph_replays;       /* batched mode */
-    int32_t variant;             /* kernel variant chosen by the autotuner */
+    int32_t variant;             /* autotuned kernel: skeleton in LDS (0) or global (1), + 2 for the
+                                    three-waves-per-SIMD compilation */
     int32_t items_target;        /* noise dimensions per lane and work item chosen by the autotuner */
     int32_t wg_per_cu;           /* persistent workgroups per CU chosen by the autotuner */
     int32_t pad;
@@ -360,7 +364,10 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model,
 /* Price (and Greeks) of every contract to standard error eps / sqrt(2) of the
  * price.  mode: fdmc_book_mode; both modes run the same kernels and give
  * bit-identical results.  results: n_contracts entries; stats may be NULL.
- * The first call autotunes the kernel variant (reported in stats->tune_s). */
+ * The first call autotunes the kernel variant (reported in stats->tune_s)
+ * unless the config fixes it.  With fdmc_timing_enable on, the batched mode
+ * runs its passes without graph capture and brackets every pass kernel
+ * (k_book) with HIP events, for fdmc_timing_read; results are unchanged. */
 fdmc_status fdmc_book_run(fdmc_book* book, double eps, int32_t mode, fdmc_book_result* results,
                           fdmc_book_stats* stats);
 /* Per-replica sums of one pass of contract c, level l over points

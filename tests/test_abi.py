@@ -39,6 +39,11 @@ def test_ctypes_struct_layout_matches_header():
     assert C.sizeof(fdmc.CPayoff) == 8 + 5 * 8 + 8
     assert C.sizeof(fdmc.CResult) == 8 * 5 + 8
     assert C.sizeof(fdmc.CQMCConfig) == 32
+    # contract book: kind, n_steps + 5 doubles; 8 ints; 8 doubles + 2 int64 + 4 ints; 2 doubles + int64 + 6 ints
+    assert C.sizeof(fdmc.CBookContract) == 8 + 5 * 8
+    assert C.sizeof(fdmc.CBookConfig) == 8 * 4
+    assert C.sizeof(fdmc.CBookResult) == 8 * 8 + 2 * 8 + 4 * 4
+    assert C.sizeof(fdmc.CBookStats) == 3 * 8 + 6 * 4
 
 
 def test_null_context_is_rejected_on_host():
