@@ -257,3 +257,40 @@ MCResult run_qmc_greeks_cuda(const ModelVariant& model, const PayoffVariant& pay
     greeks = GreeksResult{g.delta, g.vega, g.rho, g.delta_se, g.vega_se, g.rho_se};
     return res(r);
 }
+
+std::vector<BookResult> run_book_cuda(const ModelVariant& model, const std::vector<BookContract>& contracts,
+                                      double eps, int R, BookMode mode, int greeks) {
+    CModel m = to_c(model);
+    std::vector<fdmc_book_contract> k;
+    k.reserve(contracts.size());
+    for (const BookContract& c : contracts) {
+        const CPayoff p = to_c(c.payoff);
+        fdmc_book_contract x{};
+        x.kind = p.p.kind;
+        x.n_steps = c.n_steps;
+        x.K = p.p.K;
+        x.B = p.p.B;
+        x.T = c.T > 0.0 ? c.T : p.p.T;
+        x.r = p.p.r;
+        x.sigma_bgk = p.p.sigma;
+        k.push_back(x);
+    }
+    fdmc_book_config cfg{R, 400, 42u, greeks, 50, 4, -1, 0};
+    fdmc_book* b = nullptr;
+    check(fdmc_book_create(ctx(), &m.m, k.data(), (int32_t)k.size(), &cfg, &b));
+    std::vector<fdmc_book_result> r(k.size());
+    fdmc_book_stats st{};
+    const fdmc_status s = fdmc_book_run(b, eps, mode == BookMode::Plain ? FDMC_BOOK_PLAIN : FDMC_BOOK_BATCHED,
+                                        r.data(), &st);
+    const std::string err = s == FDMC_OK ? std::string() : std::string(fdmc_last_error());
+    fdmc_book_free(b);
+    if (s != FDMC_OK) throw std::runtime_error(err);
+    std::vector<BookResult> out(k.size());
+    for (size_t i = 0; i < k.size(); i++) {
+        out[i].price = MCResult{r[i].price, r[i].std_error, (long long)r[i].n_samples, st.time_s};
+        out[i].greeks = GreeksResult{r[i].delta, r[i].vega, r[i].rho, r[i].delta_se, r[i].vega_se, r[i].rho_se};
+        out[i].iterations = r[i].iterations;
+        out[i].converged = (r[i].flags & FDMC_FLAG_NOT_CONVERGED) == 0;
+    }
+    return out;
+}

@@ -95,5 +95,26 @@ MCResult run_qmc_greeks_cuda(const ModelVariant& model, const PayoffVariant& pay
                              GreeksResult& greeks, const QMCConfig& cfg = {}, NoiseMode mode = NoiseMode::Raw,
                              DeviceBBData* dev_bb = nullptr);
 
+// Additive: a book of contracts under one one-factor model (GBM, Dupire or the
+// quadratic local vol), each priced by multilevel QMC with M = 2 and levels
+// 0..log2(n_steps) on the Brownian bridge (run_mlqmc_cuda's estimator per
+// contract, salt 42 + (10 c + l) R + r), with pathwise Greeks; every contract
+// runs in the same device passes.  No counterpart in the reference.
+struct BookContract {
+    PayoffVariant payoff;   // European, Asian, Lookback or Barrier
+    int n_steps;            // dates: a power of two, 1..512
+    double T;               // maturity (<= 0: European/Barrier use their own T)
+};
+struct BookResult {
+    MCResult price;
+    GreeksResult greeks;    // NaN for barriers and with greeks = 0
+    int iterations = 0;     // passes that added points
+    bool converged = false;
+};
+enum class BookMode { Batched, Plain };
+// greeks: 0 none, 1 pathwise adjoint (reverse sweep), 2 forward tangents.
+std::vector<BookResult> run_book_cuda(const ModelVariant& model, const std::vector<BookContract>& contracts,
+                                      double eps, int R = 16, BookMode mode = BookMode::Batched, int greeks = 1);
+
 // Select the HIP device used by this thread's engine context (before the first call).
 void set_device(int device);

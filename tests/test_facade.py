@@ -42,3 +42,12 @@ def test_facade_example_runs_reference_call_sequence():
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "MLQMC BB" in p.stdout and "facade rows outside tolerance: 0" in p.stdout
+
+
+@pytest.mark.gpu
+def test_facade_book_example_batched_equals_plain():
+    exe = os.path.join(PKG, "lib", "book_table")
+    p = subprocess.run([exe, "0.01", "48"], capture_output=True, text=True, timeout=300)
+    print(p.stdout)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "batched == plain: yes" in p.stdout
