@@ -55,7 +55,8 @@ int main(int argc, char** argv) {
         if (auto* b = std::get_if<Barrier>(&book[i].payoff)) K = b->K;
         std::printf("%4d %-9s %5d %8.3f %5.2f %12.6f %10.2e %9.5f %9.4f %9.4f %5d%s\n", i, names[kinds[i]],
                     book[i].n_steps, K, book[i].T, r.price.price, r.price.std_error, r.greeks.delta, r.greeks.vega,
-                    r.greeks.rho, r.iterations, same ? "" : "  (plain differs)");
+                    r.greeks.rho, r.iterations,
+                    !same ? "  (plain differs)" : !r.converged ? "  (not converged)" : "");
     }
     const double tb = std::chrono::duration<double>(t1 - t0).count();
     const double tp = std::chrono::duration<double>(t2 - t1).count();

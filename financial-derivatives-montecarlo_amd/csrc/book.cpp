@@ -30,8 +30,8 @@ struct fdmc_book {
     std::vector<BookContract> hc;
     std::vector<BookEntry> h_batched, h_plain;
     int n_entries = 0;
-    DevBuf contracts, entries_b, entries_p, pos_b, pos_p, state, ctl, bbtab, partials, entry_sums, counters, out,
-        scratch;
+    DevBuf contracts, entries_b, entries_p, pos_b, pos_p, state, ctl, bbtab, voltab, partials, entry_sums, counters,
+        out, scratch;
     DevBuf sdir[BOOK_NL];
     int* h_active = nullptr;          // pinned
     // launch shape (autotuned on the first run)
@@ -54,7 +54,7 @@ struct fdmc_book {
     ~fdmc_book() {
         drop_graph();
         if (h_active) (void)hipHostFree(h_active);
-        DevBuf* all[] = {&contracts, &entries_b, &entries_p, &pos_b, &pos_p, &state, &ctl, &bbtab,
+        DevBuf* all[] = {&contracts, &entries_b, &entries_p, &pos_b, &pos_p, &state, &ctl, &bbtab, &voltab,
                          &partials, &entry_sums, &counters, &out, &scratch};
         for (DevBuf* b : all) b->release();
         for (auto& s : sdir) s.release();
@@ -102,6 +102,7 @@ BookLaunch make_launch(fdmc_book* b, bool plain, int c, double eps) {
     a.ctl = (BookCtl*)b->ctl.p;
     for (int l = 0; l < BOOK_NL; l++) a.sdir[l] = (const uint32_t*)b->sdir[l].p;
     a.bbtab = (const float*)b->bbtab.p;
+    a.voltab = (const float*)b->voltab.p;
     unsigned int* cnt = (unsigned int*)b->counters.p;
     a.queue = cnt;
     a.total_items = (int*)(cnt + 1);
@@ -482,6 +483,22 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
             for (size_t ti = 0; ti < Ts.size(); ti++)
                 for (int l = 0; l < BOOK_NL; l++) bridge_sd(1 << l, Ts[ti], bb.data() + (ti * BOOK_NL + l) * BOOK_NL);
             upload(b->bbtab, bb.data(), bb.size() * sizeof(float));
+
+            // local-vol time factor of every fine step: b e^{-c t} (quadratic local vol),
+            // e^{-alpha t} (Dupire), t = k h in FP32 as the steppers form it
+            std::vector<float> vt((size_t)BOOK_MAX_T * BOOK_VT_STRIDE, 0.0f);
+            for (size_t ti = 0; ti < Ts.size(); ti++)
+                for (int l = 0; l < BOOK_NL; l++) {
+                    const int nf = 1 << l;
+                    const float h = (float)(Ts[ti] / (double)nf);
+                    float* row = vt.data() + ti * BOOK_VT_STRIDE + (size_t)(nf - 1);
+                    for (int k = 0; k < nf; k++) {
+                        const float t = (float)k * h;
+                        if (b->mk == MK_LVQ) row[k] = b->leg.lv_b * std::exp(-b->leg.lv_c * t);
+                        else if (b->mk == MK_DUPIRE) row[k] = std::exp(-b->leg.alpha * t);
+                    }
+                }
+            upload(b->voltab, vt.data(), vt.size() * sizeof(float));
 
             // state, partials, scratch
             b->state.ensure((size_t)n * BOOK_NL * sizeof(BookState));

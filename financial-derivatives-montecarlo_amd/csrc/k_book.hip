@@ -54,6 +54,10 @@ __device__ int book_decide(const BookLaunch& a, const BookContract& C, const Boo
         int64_t nopt = (int64_t)ceil(2.0 / ((double)a.R * a.eps * a.eps) * sqrt(s[l].S2[0] / (double)(1 << l)) * st);
         if (nopt < 100) nopt = 100;
         int64_t ex = nopt > s[l].N ? nopt - s[l].N : 0;
+        // a top-up adds at least a tenth of the level's points: with R replicas the
+        // variance estimate moves enough that exact Giles top-ups can crawl for
+        // dozens of passes just above the target
+        if (ex > 0) ex = max(ex, (s[l].N + 9) / 10);
         if (s[l].N + ex > 4294967296LL) ex = 4294967296LL - s[l].N;   // Sobol index stays in 32 bits
         extra[l] = ex;
         any = any || ex > 0;

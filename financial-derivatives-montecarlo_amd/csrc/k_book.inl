@@ -15,18 +15,59 @@ struct Tan3 {
     float s, v, m;
 };
 
+// The book's Euler step with the local vol's time factor read from a table
+// (uniform across the workgroup: a work item is one (contract, level), so the
+// step time is the same for every lane): v = b e^{-c t} for the quadratic local
+// vol, e^{-alpha t} for Dupire, unused for GBM.  Same step as step_1f.
+template <int MK>
+__device__ __forceinline__ float book_euler(const LegParams& P, float S, float dw, float v, float h) {
+    if constexpr (MK == MK_GBM) {
+        return S + P.mu * S * h + P.sigma * S * dw;
+    } else if constexpr (MK == MK_DUPIRE) {
+        const float sl = P.sigma0 * v * fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));
+        return S + P.mu * S * h + sl * S * dw;
+    } else {
+        const float l = fast_ln(S * P.inv_sref);
+        const float sl = fmaf(v, l * l, P.lv_a);
+        return S + P.mu * S * h + sl * S * dw;
+    }
+}
+
+// Partial derivatives of book_euler (as step_partials).
+template <int MK>
+__device__ __forceinline__ void book_partials(const LegParams& P, float S, float dw, float v, float h, float& fS,
+                                              float& fvol, float& fmu, float& fS0) {
+    fmu = S * h;
+    fS0 = 0.0f;
+    if constexpr (MK == MK_GBM) {
+        fS = 1.0f + P.mu * h + P.sigma * dw;
+        fvol = S * dw;
+    } else if constexpr (MK == MK_DUPIRE) {
+        const float pw = fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));
+        const float sl = P.sigma0 * v * pw;
+        fS = 1.0f + P.mu * h + dw * sl * (P.beta_m1 + 1.0f);
+        fvol = v * pw * S * dw;                        // d/d sigma0
+        fS0 = -sl * P.beta_m1 * P.inv_S0 * S * dw;     // local-vol normalisation by S0
+    } else {
+        const float l = fast_ln(S * P.inv_sref);
+        const float sl = fmaf(v, l * l, P.lv_a);
+        fS = 1.0f + P.mu * h + dw * (sl + 2.0f * v * l);
+        fvol = S * dw;                                 // d/d a
+    }
+}
+
 // One Euler step S -> S' and, with TAN, the tangent update
 // t' = fS t + (f0, fv, fm) (step_partials are the step's partial derivatives).
 template <int MK, bool TAN>
-__device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t, float dw, float tt, float h) {
+__device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t, float dw, float v, float h) {
     if constexpr (TAN) {
         float fS, fv, fm, f0;
-        step_partials<MK>(P, S, dw, tt, h, fS, fv, fm, f0);
+        book_partials<MK>(P, S, dw, v, h, fS, fv, fm, f0);
         t.s = fS * t.s + f0;
         t.v = fS * t.v + fv;
         t.m = fS * t.m + fm;
     }
-    S = step_1f<MK>(P, S, dw, tt, h);
+    S = book_euler<MK>(P, S, dw, v, h);
 }
 
 // Running payoff state of one leg (and, with TAN, the tangent sums the
@@ -44,8 +85,8 @@ struct BookLeg {
         t = Tan3{1.0f, 0.0f, 0.0f};
         acc = (PK == PK_LOOKBACK) ? Tan3{1.0f, 0.0f, 0.0f} : Tan3{0.0f, 0.0f, 0.0f};
     }
-    __device__ __forceinline__ void step(const LegParams& P, float dw, float tt, float h, int idx) {
-        book_step<MK, TAN>(P, S, t, dw, tt, h);
+    __device__ __forceinline__ void step(const LegParams& P, float dw, float v, float h, int idx) {
+        book_step<MK, TAN>(P, S, t, dw, v, h);
         if constexpr (PK == PK_LOOKBACK) {
             if (S < run) {
                 run = S;
@@ -94,11 +135,12 @@ struct BookAdj {
 //   theta_bar += sum_k d_k dS_k/dtheta + adj_end dS_end/dtheta.
 // For a one-dimensional state this is the exact vector-Jacobian product of the
 // block, with no per-step storage.
-// PAIR: the step increments are dw[2k] + dw[2k+1] (coarse leg).
+// PAIR: the step increments are dw[2k] + dw[2k+1] (coarse leg), whose step
+// times are the even fine times (vt holds the fine grid's time factors).
 template <int MK, int N, bool PAIR>
 __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, const float* dw, int nsteps, int k0,
-                                               float h, int n, float each, float last, float cmin, int kmin,
-                                               BookAdj& A) {
+                                               const float* __restrict__ vt, float h, int n, float each, float last,
+                                               float cmin, int kmin, BookAdj& A) {
     float jS = 1.0f, jv = 0.0f, jm = 0.0f, j0 = 0.0f;
     float dS = 0.0f, dv = 0.0f, dm = 0.0f, d0 = 0.0f;
 #pragma unroll
@@ -106,9 +148,10 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
         if (k < nsteps) {
             const int gi = k0 + k;
             const float w = PAIR ? dw[2 * k] + dw[2 * k + 1] : dw[k];
+            const float v = vt[PAIR ? 2 * gi : gi];
             float fS, fv, fm, f0;
-            step_partials<MK>(P, S, w, (float)gi * h, h, fS, fv, fm, f0);
-            S = step_1f<MK>(P, S, w, (float)gi * h, h);
+            book_partials<MK>(P, S, w, v, h, fS, fv, fm, f0);
+            S = book_euler<MK>(P, S, w, v, h);
             jS = fS * jS;
             jv = fS * jv + fv;
             jm = fS * jm + fm;
@@ -138,8 +181,8 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
 // tangents (carried along the forward pass; no second pass).
 template <int MK, int PK, int GM>
 __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                          const SobolLane& sl, float* sk, float* ckf, float* ckc, bool valid,
-                                          double (&acc)[BOOK_NACC]) {
+                                          const float* __restrict__ vt, const SobolLane& sl, float* sk, float* ckf,
+                                          float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
     constexpr bool LIP = PK != PK_BARRIER;
     constexpr bool ADJ = LIP && GM == 1;
     constexpr bool TAN = LIP && GM == 2;
@@ -191,10 +234,10 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
 #pragma unroll
             for (int k = 0; k < 16; k++) {
                 const int gi = i * 16 + k;
-                F.step(P, dw[k], (float)gi * g.hf, g.hf, gi + 1);
+                F.step(P, dw[k], vt[gi], g.hf, gi + 1);
                 if (k & 1) {
                     const int gc = gi >> 1;
-                    Cc.step(P, dw[k - 1] + dw[k], (float)gc * g.hc, g.hc, gc + 1);
+                    Cc.step(P, dw[k - 1] + dw[k], vt[2 * gc], g.hc, gc + 1);
                 }
             }
         }
@@ -202,10 +245,10 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
         float prev = 0.0f;
         for (int k = 0; k < nf; k++) {
             const float dw = sk[(k + 1) * 256] - sk[k * 256];
-            F.step(P, dw, (float)k * g.hf, g.hf, k + 1);
+            F.step(P, dw, vt[k], g.hf, k + 1);
             if (k & 1) {
                 const int gc = k >> 1;
-                Cc.step(P, prev + dw, (float)gc * g.hc, g.hc, gc + 1);
+                Cc.step(P, prev + dw, vt[2 * gc], g.hc, gc + 1);
             }
             prev = dw;
         }
@@ -243,15 +286,16 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
             for (int i = G - 1; i >= 0; i--) {
                 float dw[16];
                 gen_block(i, dw);
-                book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
-                book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, g.hf, nf, ef, lf, mf, F.kmin, Af);
+                book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vt, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
+                book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vt, g.hf, nf, ef, lf, mf, F.kmin,
+                                              Af);
             }
         } else {
             float dw[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) dw[k] = (k < nf) ? sk[(k + 1) * 256] - sk[k * 256] : 0.0f;
-            book_block_adj<MK, 4, true>(P, P.S0, dw, nc, 0, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
-            book_block_adj<MK, 8, false>(P, P.S0, dw, nf, 0, g.hf, nf, ef, lf, mf, F.kmin, Af);
+            book_block_adj<MK, 4, true>(P, P.S0, dw, nc, 0, vt, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
+            book_block_adj<MK, 8, false>(P, P.S0, dw, nf, 0, vt, g.hf, nf, ef, lf, mf, F.kmin, Af);
         }
         if (F.kmin == 0) Af.adj += mf;     // S0 itself is the running minimum
         if (Cc.kmin == 0) Ac.adj += mc;
@@ -334,6 +378,7 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         const int D = g.nf;
         const uint32_t* grec = a.sdir[E.l] + ((size_t)C.slot[E.l] * a.R + E.r) * (size_t)D * SDIR_WORDS;
         const float* sdt = a.bbtab + ((size_t)C.t_idx * BOOK_NL + E.l) * BOOK_NL;
+        const float* vt = a.voltab + (size_t)C.t_idx * BOOK_VT_STRIDE + (size_t)(D - 1);
         const int64_t tfirst = E.first >> 8;
         const int nt_all = (int)(((E.first + E.count - 1) >> 8) - tfirst + 1);
         const int t0 = (item - E.tile_begin) * E.tpi;
@@ -367,10 +412,10 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
             const int64_t idx = base + tid;
             const bool valid = idx >= E.first && idx < E.first + E.count;
             switch (kind) {
-            case PK_EUROPEAN: book_path<MK, PK_EUROPEAN, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
-            case PK_ASIAN: book_path<MK, PK_ASIAN, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
-            case PK_LOOKBACK: book_path<MK, PK_LOOKBACK, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
-            default: book_path<MK, PK_BARRIER, GM>(P, g, sdt, sl, sk, ckf, ckc, valid, acc); break;
+            case PK_EUROPEAN: book_path<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
+            case PK_ASIAN: book_path<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
+            case PK_LOOKBACK: book_path<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
+            default: book_path<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
             }
         }
         block_sum<BOOK_NACC, 4>(acc, red);

@@ -146,9 +146,10 @@ typedef struct fdmc_cv_pilot_result { double beta, var_plain, var_cv; } fdmc_cv_
  *     salt_base + (c * FDMC_BOOK_LEVELS + l) * R + r
  * (contract c, replica r; contract 0 has run_mlqmc_cuda's 42 + l R + r,
  * methods_cuda.cu:1845).  Sample allocation, merges and the per-point variance
- * S2_l follow run_mlqmc_cuda (methods_cuda.cu:1680-2019) level by level; a
- * contract stops when no level needs more points (its bias is that of its own
- * date grid, so there is no bias test). */
+ * S2_l follow run_mlqmc_cuda (methods_cuda.cu:1680-2019) level by level (a
+ * top-up adds at least a tenth of the level's points); a contract stops when
+ * sum_l S2_l / (N_l R) <= eps^2 / 2 (its bias is that of its own date grid, so
+ * there is no bias test). */
 #define FDMC_BOOK_LEVELS 10           /* levels 0..9: n_steps up to 512 */
 
 typedef struct fdmc_book_contract {

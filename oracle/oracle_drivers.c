@@ -821,6 +821,8 @@ int orc_book_contract_run(const fdmc_model* m, const fdmc_book_contract* k, int 
                                              sqrt(S2[l][0] / (double)(1 << l)) * st);
                 if (Nopt < 100) Nopt = 100;
                 extra[l] = Nopt > N[l] ? Nopt - N[l] : 0;
+                /* a top-up adds at least a tenth of the level's points (fdmc_book_run) */
+                if (extra[l] > 0 && extra[l] < (N[l] + 9) / 10) extra[l] = (N[l] + 9) / 10;
                 if (N[l] + extra[l] > 4294967296LL) extra[l] = 4294967296LL - N[l];
                 if (extra[l] > 0) any = 1;
             }
