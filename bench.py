@@ -43,6 +43,11 @@ MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense FP16/BF16 matrix peak
 METRIC = "time-to-eps=1e-3 price+Greeks, batched vs plain (s); path-steps/s vs VALU peak"
 EPS = 1e-3
 BOOK_KERNEL = "k_book<LOCALVOL_QUAD, adjoint Greeks>"
+# HBM bytes per launch of the book kernel on this workload: FETCH_SIZE and
+# WRITE_SIZE of two rocprofv3 --pmc runs of `bench.py --roofline-only`, scaled by
+# the same counters on a 512 MiB device-to-device copy (reads x2.0, writes x1.0);
+# a counter run cannot be part of the timed process, so the figure is recorded.
+BOOK_TRAFFIC_BYTES_PER_LAUNCH = 4.385e10
 
 
 def log(*a):
@@ -195,7 +200,7 @@ def book_roofline(eng, book):
     k_avg_ms = kms / max(1, launches)
     achieved = st.path_steps * BOOK_FLOPS_PER_PATH_STEP / (kms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / VALU_PEAK_TFLOPS, "traffic": None, "kernel": BOOK_KERNEL,
+            "frac": achieved / VALU_PEAK_TFLOPS, "traffic": BOOK_TRAFFIC_BYTES_PER_LAUNCH, "kernel": BOOK_KERNEL,
             "kernel_avg_ms": k_avg_ms, "kernel_launches": launches, "kernel_total_ms": kms,
             "path_steps": st.path_steps, "flops_per_path_step": BOOK_FLOPS_PER_PATH_STEP,
             "variant": st.variant, "wg_per_cu": st.wg_per_cu}
@@ -212,10 +217,16 @@ def main():
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=2)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="roofline-only: also copy this many MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    torch = None
+    if args.copy_mib:
+        import torch   # before the engine: both use the same HIP runtime library
 
     import fdmc
 
@@ -227,7 +238,17 @@ def main():
         rl = book_roofline(eng, book)
         book.free()
         eng.close()
-        print(json.dumps({"roofline": rl}))
+        out = {"roofline": rl}
+        if args.copy_mib:
+            # a plain copy of known size: reads and writes copy_mib MiB once each
+            n = args.copy_mib << 20
+            src = torch.ones(n, dtype=torch.uint8, device="cuda")
+            dst = torch.empty_like(src)
+            torch.cuda.synchronize()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            out["copy_calibration_bytes"] = n
+        print(json.dumps(out))
         return
 
     book = eng.book_create(model, contracts, fdmc.BookConfig(R=16, greeks=1))
