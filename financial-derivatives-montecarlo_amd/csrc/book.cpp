@@ -484,8 +484,9 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
                 for (int l = 0; l < BOOK_NL; l++) bridge_sd(1 << l, Ts[ti], bb.data() + (ti * BOOK_NL + l) * BOOK_NL);
             upload(b->bbtab, bb.data(), bb.size() * sizeof(float));
 
-            // local-vol time factor of every fine step: b e^{-c t} (quadratic local vol),
-            // e^{-alpha t} (Dupire), t = k h in FP32 as the steppers form it
+            // local-vol time factor of every fine step: b e^{-c t} ln(2)^2 (quadratic
+            // local vol, evaluated in log2), e^{-alpha t} (Dupire), t = k h in FP32 as
+            // the steppers form it
             std::vector<float> vt((size_t)BOOK_MAX_T * BOOK_VT_STRIDE, 0.0f);
             for (size_t ti = 0; ti < Ts.size(); ti++)
                 for (int l = 0; l < BOOK_NL; l++) {
@@ -494,7 +495,7 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
                     float* row = vt.data() + ti * BOOK_VT_STRIDE + (size_t)(nf - 1);
                     for (int k = 0; k < nf; k++) {
                         const float t = (float)k * h;
-                        if (b->mk == MK_LVQ) row[k] = b->leg.lv_b * std::exp(-b->leg.lv_c * t);
+                        if (b->mk == MK_LVQ) row[k] = b->leg.lv_b * std::exp(-b->leg.lv_c * t) * 0.480453014f;  // ln(2)^2
                         else if (b->mk == MK_DUPIRE) row[k] = std::exp(-b->leg.alpha * t);
                     }
                 }

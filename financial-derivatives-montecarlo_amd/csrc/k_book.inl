@@ -7,7 +7,7 @@ namespace fdmc {
 
 struct BookGeom {
     int l, nf, nc, G;
-    float hf, hc, bgk_f, bgk_c, T;
+    float hf, hc, muh_f, muh_c, bgk_f, bgk_c, T;
 };
 
 // Sensitivities carried along a path: dS/dS0, dS/d(vol parameter), dS/dmu.
@@ -15,43 +15,61 @@ struct Tan3 {
     float s, v, m;
 };
 
-// The book's Euler step with the local vol's time factor read from a table
-// (uniform across the workgroup: a work item is one (contract, level), so the
-// step time is the same for every lane): v = b e^{-c t} for the quadratic local
-// vol, e^{-alpha t} for Dupire, unused for GBM.  Same step as step_1f.
+// The book's Euler step S' = S + mu S h + sigma_loc S dw, evaluated as
+// S' = fma(S, fma(sigma_loc, dw, mu h), S) with mu h hoisted per work item (two
+// VALU operations; the kernel is VALU-issue bound).  The local vol's time
+// factor comes from a table (uniform across the workgroup: a work item is one
+// (contract, level), so the step time is the same for every lane):
+//   quadratic local vol: v = b e^{-c t} ln(2)^2 and
+//       sigma_loc = a + v log2(S/s_ref)^2  (= a + b e^{-c t} ln(S/s_ref)^2);
+//   Dupire: v = e^{-alpha t}, sigma_loc = sigma0 v (S/S0)^(beta-1);  GBM: unused.
 template <int MK>
-__device__ __forceinline__ float book_euler(const LegParams& P, float S, float dw, float v, float h) {
+__device__ __forceinline__ float book_sigma(const LegParams& P, float S, float v, float& aux) {
     if constexpr (MK == MK_GBM) {
-        return S + P.mu * S * h + P.sigma * S * dw;
+        aux = 0.0f;
+        return P.sigma;
     } else if constexpr (MK == MK_DUPIRE) {
-        const float sl = P.sigma0 * v * fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));
-        return S + P.mu * S * h + sl * S * dw;
+        aux = fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));   // (S/S0)^(beta-1)
+        return P.sigma0 * v * aux;
     } else {
-        const float l = fast_ln(S * P.inv_sref);
-        const float sl = fmaf(v, l * l, P.lv_a);
-        return S + P.mu * S * h + sl * S * dw;
+        aux = fast_log2(S * P.inv_sref);                          // log2(S/s_ref)
+        return fmaf(v, aux * aux, P.lv_a);
     }
 }
 
-// Partial derivatives of book_euler (as step_partials).
 template <int MK>
-__device__ __forceinline__ void book_partials(const LegParams& P, float S, float dw, float v, float h, float& fS,
-                                              float& fvol, float& fmu, float& fS0) {
+__device__ __forceinline__ float book_euler(const LegParams& P, float S, float dw, float v, float muh) {
+    if constexpr (MK == MK_GBM) {
+        // the reference's form: as one fma chain in S the unrolled GBM path sends the
+        // compiler into a very long optimisation, and GBM has no local-vol work to save
+        return S + muh * S + P.sigma * S * dw;
+    } else {
+        float aux;
+        const float sl = book_sigma<MK>(P, S, v, aux);
+        return fmaf(S, fmaf(sl, dw, muh), S);
+    }
+}
+
+// Partial derivatives of book_euler: fS = dS'/dS, fvol = dS'/d(sigma, sigma0
+// or a), fmu = dS'/dmu, fS0 = dS'/dS0 through the Dupire normalisation.
+template <int MK>
+__device__ __forceinline__ void book_partials(const LegParams& P, float S, float dw, float v, float h, float muh,
+                                              float& fS, float& fvol, float& fmu, float& fS0) {
+    constexpr float TWO_OVER_LN2 = 2.8853900817779268f;   // 2 / ln 2
+    float aux;
+    const float sl = book_sigma<MK>(P, S, v, aux);
     fmu = S * h;
     fS0 = 0.0f;
     if constexpr (MK == MK_GBM) {
-        fS = 1.0f + P.mu * h + P.sigma * dw;
+        fS = fmaf(sl, dw, 1.0f + muh);
         fvol = S * dw;
     } else if constexpr (MK == MK_DUPIRE) {
-        const float pw = fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));
-        const float sl = P.sigma0 * v * pw;
-        fS = 1.0f + P.mu * h + dw * sl * (P.beta_m1 + 1.0f);
-        fvol = v * pw * S * dw;                        // d/d sigma0
+        fS = fmaf(dw * sl, P.beta_m1 + 1.0f, 1.0f + muh);
+        fvol = v * aux * S * dw;                       // d/d sigma0
         fS0 = -sl * P.beta_m1 * P.inv_S0 * S * dw;     // local-vol normalisation by S0
     } else {
-        const float l = fast_ln(S * P.inv_sref);
-        const float sl = fmaf(v, l * l, P.lv_a);
-        fS = 1.0f + P.mu * h + dw * (sl + 2.0f * v * l);
+        // d sigma_loc/dS * S = 2 b e^{-c t} ln(S/s_ref) = (2 / ln 2) v log2(S/s_ref)
+        fS = fmaf(dw, fmaf(v * aux, TWO_OVER_LN2, sl), 1.0f + muh);
         fvol = S * dw;                                 // d/d a
     }
 }
@@ -59,15 +77,16 @@ __device__ __forceinline__ void book_partials(const LegParams& P, float S, float
 // One Euler step S -> S' and, with TAN, the tangent update
 // t' = fS t + (f0, fv, fm) (step_partials are the step's partial derivatives).
 template <int MK, bool TAN>
-__device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t, float dw, float v, float h) {
+__device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t, float dw, float v, float h,
+                                          float muh) {
     if constexpr (TAN) {
         float fS, fv, fm, f0;
-        book_partials<MK>(P, S, dw, v, h, fS, fv, fm, f0);
+        book_partials<MK>(P, S, dw, v, h, muh, fS, fv, fm, f0);
         t.s = fS * t.s + f0;
         t.v = fS * t.v + fv;
         t.m = fS * t.m + fm;
     }
-    S = book_euler<MK>(P, S, dw, v, h);
+    S = book_euler<MK>(P, S, dw, v, muh);
 }
 
 // Running payoff state of one leg (and, with TAN, the tangent sums the
@@ -85,8 +104,8 @@ struct BookLeg {
         t = Tan3{1.0f, 0.0f, 0.0f};
         acc = (PK == PK_LOOKBACK) ? Tan3{1.0f, 0.0f, 0.0f} : Tan3{0.0f, 0.0f, 0.0f};
     }
-    __device__ __forceinline__ void step(const LegParams& P, float dw, float v, float h, int idx) {
-        book_step<MK, TAN>(P, S, t, dw, v, h);
+    __device__ __forceinline__ void step(const LegParams& P, float dw, float v, float h, float muh, int idx) {
+        book_step<MK, TAN>(P, S, t, dw, v, h, muh);
         if constexpr (PK == PK_LOOKBACK) {
             if (S < run) {
                 run = S;
@@ -139,8 +158,8 @@ struct BookAdj {
 // times are the even fine times (vt holds the fine grid's time factors).
 template <int MK, int N, bool PAIR>
 __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, const float* dw, int nsteps, int k0,
-                                               const float* __restrict__ vt, float h, int n, float each, float last,
-                                               float cmin, int kmin, BookAdj& A) {
+                                               const float* __restrict__ vt, float h, float muh, int n, float each,
+                                               float last, float cmin, int kmin, BookAdj& A) {
     float jS = 1.0f, jv = 0.0f, jm = 0.0f, j0 = 0.0f;
     float dS = 0.0f, dv = 0.0f, dm = 0.0f, d0 = 0.0f;
 #pragma unroll
@@ -150,8 +169,8 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
             const float w = PAIR ? dw[2 * k] + dw[2 * k + 1] : dw[k];
             const float v = vt[PAIR ? 2 * gi : gi];
             float fS, fv, fm, f0;
-            book_partials<MK>(P, S, w, v, h, fS, fv, fm, f0);
-            S = book_euler<MK>(P, S, w, v, h);
+            book_partials<MK>(P, S, w, v, h, muh, fS, fv, fm, f0);
+            S = book_euler<MK>(P, S, w, v, muh);
             jS = fS * jS;
             jv = fS * jv + fv;
             jm = fS * jm + fm;
@@ -234,10 +253,10 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
 #pragma unroll
             for (int k = 0; k < 16; k++) {
                 const int gi = i * 16 + k;
-                F.step(P, dw[k], vt[gi], g.hf, gi + 1);
+                F.step(P, dw[k], vt[gi], g.hf, g.muh_f, gi + 1);
                 if (k & 1) {
                     const int gc = gi >> 1;
-                    Cc.step(P, dw[k - 1] + dw[k], vt[2 * gc], g.hc, gc + 1);
+                    Cc.step(P, dw[k - 1] + dw[k], vt[2 * gc], g.hc, g.muh_c, gc + 1);
                 }
             }
         }
@@ -245,10 +264,10 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
         float prev = 0.0f;
         for (int k = 0; k < nf; k++) {
             const float dw = sk[(k + 1) * 256] - sk[k * 256];
-            F.step(P, dw, vt[k], g.hf, k + 1);
+            F.step(P, dw, vt[k], g.hf, g.muh_f, k + 1);
             if (k & 1) {
                 const int gc = k >> 1;
-                Cc.step(P, prev + dw, vt[2 * gc], g.hc, gc + 1);
+                Cc.step(P, prev + dw, vt[2 * gc], g.hc, g.muh_c, gc + 1);
             }
             prev = dw;
         }
@@ -286,16 +305,17 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
             for (int i = G - 1; i >= 0; i--) {
                 float dw[16];
                 gen_block(i, dw);
-                book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vt, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
-                book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vt, g.hf, nf, ef, lf, mf, F.kmin,
-                                              Af);
+                book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vt, g.hc, g.muh_c, nc, ec, lc, mc,
+                                            Cc.kmin, Ac);
+                book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vt, g.hf, g.muh_f, nf, ef, lf, mf,
+                                              F.kmin, Af);
             }
         } else {
             float dw[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) dw[k] = (k < nf) ? sk[(k + 1) * 256] - sk[k * 256] : 0.0f;
-            book_block_adj<MK, 4, true>(P, P.S0, dw, nc, 0, vt, g.hc, nc, ec, lc, mc, Cc.kmin, Ac);
-            book_block_adj<MK, 8, false>(P, P.S0, dw, nf, 0, vt, g.hf, nf, ef, lf, mf, F.kmin, Af);
+            book_block_adj<MK, 4, true>(P, P.S0, dw, nc, 0, vt, g.hc, g.muh_c, nc, ec, lc, mc, Cc.kmin, Ac);
+            book_block_adj<MK, 8, false>(P, P.S0, dw, nf, 0, vt, g.hf, g.muh_f, nf, ef, lf, mf, F.kmin, Af);
         }
         if (F.kmin == 0) Af.adj += mf;     // S0 itself is the running minimum
         if (Cc.kmin == 0) Ac.adj += mc;
@@ -367,6 +387,8 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         g.G = g.nf >= 16 ? g.nf >> 4 : g.nf;
         g.hf = (float)(C.T / (double)g.nf);
         g.hc = g.nc > 0 ? (float)(C.T / (double)g.nc) : 0.0f;
+        g.muh_f = a.leg.mu * g.hf;
+        g.muh_c = a.leg.mu * g.hc;
         g.bgk_f = C.bgk[E.l];
         g.bgk_c = E.l > 0 ? C.bgk[E.l - 1] : 1.0f;
         g.T = C.Tf;
