@@ -165,10 +165,12 @@ def c3_gemm(eng, reps=5):
         wall = (time.perf_counter() - t0) / reps
         n, ms = eng.timing_read(reset=True)
         eng.timing_enable(False)
-        flops = 2.0 * 16 * (1 << 18) * 1024 * 1024
-        tf = flops / (ms / n * 1e-3) / 1e12
-        out["bf16" if bf16 else "fp16"] = {"gemm_ms": ms / n, "tflops": tf, "frac_mfma_peak": tf / MFMA_PEAK_TFLOPS,
-                                           "pass_ms": 1e3 * wall,
+        # a pass covers 16 x 2^18 rows in one or more GEMM launches (replica groups
+        # of at most 8 GiB of dW): flops of all timed passes over all GEMM time
+        flops = 2.0 * 16 * (1 << 18) * 1024 * 1024 * reps
+        tf = flops / (ms * 1e-3) / 1e12
+        out["bf16" if bf16 else "fp16"] = {"gemm_ms": ms / n, "gemm_launches_per_pass": n // reps, "tflops": tf,
+                                           "frac_mfma_peak": tf / MFMA_PEAK_TFLOPS, "pass_ms": 1e3 * wall,
                                            "price": sum(r[0] for r in sums) / (16 << 18)}
     pca.free()
     return out

@@ -12,6 +12,8 @@
 //               128 x 128 workgroup tiles, 4 waves of 64 x 64, BK = 32, LDS
 //               double buffering; tiles are assigned so the 8 column tiles of
 //               one row block run on the same XCD (its L2 keeps the A rows).
+//   k_gemm_nt256: the same with 256 x 256 tiles (4 waves of 128 x 128), used
+//               whenever the shape allows it.
 #include "k_path.inl"
 
 namespace fdmc {
@@ -125,23 +127,25 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const T* __restrict__ A, const 
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
 
-    // global -> LDS: each operand tile is 128 rows x 32 k = 512 chunks of 8
-    uint4 ra[2], rb[2];
+    // global -> LDS: each operand tile is 128 rows x 32 k = 512 chunks of 8;
+    // thread tid moves chunks tid and tid + 256 (rows tid / 4 and tid / 4 + 64),
+    // staged in named registers (an array here ends up in scratch memory)
+    const int grow = tid >> 2, gq = (tid & 3) * 8;
+    const T* ga = A + (int64_t)(m0 + grow) * K + gq;
+    const T* gb = Bt + (int64_t)(n0 + grow) * K + gq;
+    const int64_t gs = (int64_t)64 * K;
+    uint4 ra0, ra1, rb0, rb1;
     auto gload = [&](int k0) {
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const int c = tid + 256 * s, row = c >> 2, q = (c & 3) * 8;
-            ra[s] = *(const uint4*)(A + (int64_t)(m0 + row) * K + k0 + q);
-            rb[s] = *(const uint4*)(Bt + (int64_t)(n0 + row) * K + k0 + q);
-        }
+        ra0 = *(const uint4*)(ga + k0);
+        ra1 = *(const uint4*)(ga + gs + k0);
+        rb0 = *(const uint4*)(gb + k0);
+        rb1 = *(const uint4*)(gb + gs + k0);
     };
     auto lstore = [&](int buf) {
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const int c = tid + 256 * s, row = c >> 2, q = (c & 3) * 8;
-            *(uint4*)(&As[buf][row * G_LD + q]) = ra[s];
-            *(uint4*)(&Bs[buf][row * G_LD + q]) = rb[s];
-        }
+        *(uint4*)(&As[buf][grow * G_LD + gq]) = ra0;
+        *(uint4*)(&As[buf][(grow + 64) * G_LD + gq]) = ra1;
+        *(uint4*)(&Bs[buf][grow * G_LD + gq]) = rb0;
+        *(uint4*)(&Bs[buf][(grow + 64) * G_LD + gq]) = rb1;
     };
     gload(0);
     lstore(0);
@@ -181,6 +185,119 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const T* __restrict__ A, const 
             }
 }
 
+// ---- NT GEMM, 256 x 256 workgroup tiles -------------------------------------------
+// Each of the 4 waves owns a 128 x 128 block (4 x 4 MFMA 32x32 tiles, 256
+// accumulators per lane): per k16 step a wave reads 8 fragments (8 KB of LDS)
+// for 16 MFMAs, half the LDS bytes per flop of the 128 x 128 kernel, whose
+// fragment reads pace its MFMAs.  One wave per SIMD; LDS double buffered
+// (2 x 40 KB, rows padded to 40 elements).  Requires Mrows % 256 == 0,
+// N % 256 == 0, K % 32 == 0 (checked on the host).
+constexpr int G2_M = 256, G2_N = 256, G2_K = 32, G2_LD = G2_K + 8;
+constexpr size_t G2_LDS_BYTES = (size_t)2 * (G2_M + G2_N) * G2_LD * 2;
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_gemm_nt256(const T* __restrict__ A, const T* __restrict__ Bt,
+                                                    float* __restrict__ C, int Mrows, int N, int K) {
+    extern __shared__ __align__(16) unsigned char gsm[];
+    T* As = (T*)gsm;                          // [2][G2_M * G2_LD]
+    T* Bs = As + 2 * G2_M * G2_LD;            // [2][G2_N * G2_LD]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // XCD-aware order: each XCD takes a contiguous range of tiles, column tiles
+    // fastest, so the column tiles of one row block share that XCD's L2.
+    const int ntn = N / G2_N, ntm = Mrows / G2_M, ntiles = ntn * ntm;
+    const int id = blockIdx.x;
+    const int per = (ntiles + 7) / 8;
+    const int t = (id % 8) * per + id / 8;
+    if (t >= ntiles) return;
+    const int m0 = (t / ntn) * G2_M, n0 = (t % ntn) * G2_N;
+    const int wm = (w >> 1) * 128, wn = (w & 1) * 128;
+
+    f32x16 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+
+    // global -> LDS: each operand tile is 256 rows x 32 k = 1024 chunks of 8;
+    // thread tid moves chunks tid + 256 s (rows tid / 4 + 64 s), staged in eight
+    // named registers (an array here ends up in scratch memory)
+    const int grow = tid >> 2, gq = (tid & 3) * 8;
+    const T* ga = A + (int64_t)(m0 + grow) * K + gq;
+    const T* gb = Bt + (int64_t)(n0 + grow) * K + gq;
+    const int64_t gs = (int64_t)64 * K;                 // 64 rows further
+    const int lo = grow * G2_LD + gq, ls = 64 * G2_LD;
+    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+    auto gload = [&](int k0) {
+        ra0 = *(const uint4*)(ga + k0);
+        ra1 = *(const uint4*)(ga + gs + k0);
+        ra2 = *(const uint4*)(ga + 2 * gs + k0);
+        ra3 = *(const uint4*)(ga + 3 * gs + k0);
+        rb0 = *(const uint4*)(gb + k0);
+        rb1 = *(const uint4*)(gb + gs + k0);
+        rb2 = *(const uint4*)(gb + 2 * gs + k0);
+        rb3 = *(const uint4*)(gb + 3 * gs + k0);
+    };
+    auto lstore = [&](int buf) {
+        T* a = As + buf * G2_M * G2_LD + lo;
+        T* b = Bs + buf * G2_N * G2_LD + lo;
+        *(uint4*)(a) = ra0;
+        *(uint4*)(a + ls) = ra1;
+        *(uint4*)(a + 2 * ls) = ra2;
+        *(uint4*)(a + 3 * ls) = ra3;
+        *(uint4*)(b) = rb0;
+        *(uint4*)(b + ls) = rb1;
+        *(uint4*)(b + 2 * ls) = rb2;
+        *(uint4*)(b + 3 * ls) = rb3;
+    };
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    auto compute = [&](int buf) {
+        const T* Ab = As + buf * G2_M * G2_LD;
+        const T* Bb = Bs + buf * G2_N * G2_LD;
+#pragma unroll
+        for (int kk = 0; kk < G2_K; kk += 16) {
+            const T* fa[4];
+            const T* fb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                fa[i] = Ab + (wm + 32 * i + lr) * G2_LD + kk + lk;
+                fb[i] = Bb + (wn + 32 * i + lr) * G2_LD + kk + lk;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = mfma32<T>(fa[i], fb[j], acc[i][j]);
+        }
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    const int nk = K / G2_K;
+    // the last k tile is peeled so that every staged load in the loop is
+    // unconditional (the staging registers then stay registers)
+    for (int kt = 0; kt + 1 < nk; kt++) {
+        const int buf = kt & 1;
+        gload((kt + 1) * G2_K);
+        compute(buf);
+        lstore(buf ^ 1);
+        __syncthreads();
+    }
+    compute((nk - 1) & 1);
+    // D layout of 32x32 MFMA: lane l holds column l % 32, element e at row
+    // (e / 4) * 8 + (l / 32) * 4 + e % 4.
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                const int col = n0 + wn + j * 32 + lr;
+                C[(int64_t)row * N + col] = acc[i][j][e];
+            }
+}
+
 // ---- host launchers ------------------------------------------------------------
 size_t gen_z_lds_bytes(int D) { return (size_t)(D <= SOBOL_LDS_MAX_D ? D * 20 * 4 : 0) + 256 * 72 * 2; }
 
@@ -206,6 +323,21 @@ hipError_t launch_gen_z(const uint32_t* sdir, int D, int ld, int R, int64_t firs
 
 hipError_t launch_gemm_nt(const void* A, const void* Bt, float* C, int Mrows, int N, int K, bool bf16,
                           hipStream_t st) {
+    if (Mrows % G2_M == 0 && N % G2_N == 0 && K % G2_K == 0) {
+        const int64_t tiles = (int64_t)(Mrows / G2_M) * (N / G2_N);
+        const int64_t grid = (tiles + 7) / 8 * 8;
+        if (grid > 0x7FFFFFFF) return hipErrorInvalidValue;
+        const void* fn = bf16 ? (const void*)&k_gemm_nt256<__bf16> : (const void*)&k_gemm_nt256<_Float16>;
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G2_LDS_BYTES);
+        if (e != hipSuccess) return e;
+        if (bf16)
+            hipLaunchKernelGGL(k_gemm_nt256<__bf16>, dim3((unsigned)grid), dim3(256), G2_LDS_BYTES, st,
+                               (const __bf16*)A, (const __bf16*)Bt, C, Mrows, N, K);
+        else
+            hipLaunchKernelGGL(k_gemm_nt256<_Float16>, dim3((unsigned)grid), dim3(256), G2_LDS_BYTES, st,
+                               (const _Float16*)A, (const _Float16*)Bt, C, Mrows, N, K);
+        return hipGetLastError();
+    }
     if (Mrows % GB_M || N % GB_N || K % GB_K) return hipErrorInvalidValue;
     const int64_t tiles = (int64_t)(Mrows / GB_M) * (N / GB_N);
     const int64_t grid = (tiles + 7) / 8 * 8;

@@ -33,10 +33,12 @@ def c3(engine):
 
 
 @pytest.mark.parametrize("bf16", [False, True], ids=["fp16", "bf16"])
-def test_gemm_matches_fp64_gemm(engine, torch_cuda, c3, bf16):
+@pytest.mark.parametrize("n", [4096, 4224], ids=["tiles256", "tiles128"])
+def test_gemm_matches_fp64_gemm(engine, torch_cuda, c3, bf16, n):
+    # 4096 rows take the 256 x 256-tile kernel, 4224 (not a multiple of 256) the 128 x 128 one
     torch = torch_cuda
     sig, C, M, pca = c3
-    n, salt, off = 4096, 42, 512
+    salt, off = 42, 512
     out = torch.empty((n, 1024), dtype=torch.float32, device="cuda")
     engine.pca_transform(pca, salt, off, n, bf16, out.data_ptr())
     got = out.cpu().numpy().astype(np.float64)
