@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=2)
+    ap.add_argument("--only", choices=["c2", "c3", "c4"], default=None,
+                    help="run just one of the extra configurations and print its fields")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="roofline-only: also copy this many MiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -234,6 +236,12 @@ def main():
 
     model, contracts = fdmc.local_vol_book(args.contracts, 2026)
     eng = fdmc.Engine(0)
+    if args.only:
+        fn = {"c2": c2_asian, "c3": c3_gemm, "c4": c4_heston_mlmc}[args.only]
+        out = {args.only: fn(eng)}
+        eng.close()
+        print(json.dumps(out))
+        return
     if args.roofline_only:
         book = eng.book_create(model, contracts, fdmc.BookConfig(R=16, greeks=1, variant=args.variant,
                                                                  wg_per_cu=args.wpc))

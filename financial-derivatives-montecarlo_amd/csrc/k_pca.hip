@@ -189,10 +189,12 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const T* __restrict__ A, const 
 // Each of the 4 waves owns a 128 x 128 block (4 x 4 MFMA 32x32 tiles, 256
 // accumulators per lane): per k16 step a wave reads 8 fragments (8 KB of LDS)
 // for 16 MFMAs, half the LDS bytes per flop of the 128 x 128 kernel, whose
-// fragment reads pace its MFMAs.  One wave per SIMD; LDS double buffered
-// (2 x 40 KB, rows padded to 40 elements).  Requires Mrows % 256 == 0,
-// N % 256 == 0, K % 32 == 0 (checked on the host).
-constexpr int G2_M = 256, G2_N = 256, G2_K = 32, G2_LD = G2_K + 8;
+// fragment reads pace its MFMAs.  One wave per SIMD, so a k tile is 64 deep
+// (2048 MFMA cycles per wave between barriers, enough to cover the next tile's
+// HBM loads); LDS double buffered (2 x 72 KB, rows padded to 72 elements).
+// Loads run two k tiles ahead.  Requires Mrows % 256 == 0, N % 256 == 0,
+// K % 128 == 0 (checked on the host).
+constexpr int G2_M = 256, G2_N = 256, G2_K = 64, G2_LD = G2_K + 8;
 constexpr size_t G2_LDS_BYTES = (size_t)2 * (G2_M + G2_N) * G2_LD * 2;
 
 template <typename T>
@@ -220,37 +222,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
 
-    // global -> LDS: each operand tile is 256 rows x 32 k = 1024 chunks of 8;
-    // thread tid moves chunks tid + 256 s (rows tid / 4 + 64 s), staged in eight
-    // named registers (an array here ends up in scratch memory)
-    const int grow = tid >> 2, gq = (tid & 3) * 8;
+    // global -> LDS: each operand tile is 256 rows x 64 k = 2048 chunks of 8;
+    // thread tid moves chunks tid + 256 s (rows tid / 8 + 32 s, chunk tid % 8).
+    // Two named register sets (arrays here end up in scratch memory) hold the
+    // next two k tiles: tile kt + 2 is requested while tile kt is multiplied,
+    // and tile kt + 1, requested an iteration earlier, is then stored to LDS.
+    const int grow = tid >> 3, gq = (tid & 7) * 8;
     const T* ga = A + (int64_t)(m0 + grow) * K + gq;
     const T* gb = Bt + (int64_t)(n0 + grow) * K + gq;
-    const int64_t gs = (int64_t)64 * K;                 // 64 rows further
-    const int lo = grow * G2_LD + gq, ls = 64 * G2_LD;
-    uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-    auto gload = [&](int k0) {
-        ra0 = *(const uint4*)(ga + k0);
-        ra1 = *(const uint4*)(ga + gs + k0);
-        ra2 = *(const uint4*)(ga + 2 * gs + k0);
-        ra3 = *(const uint4*)(ga + 3 * gs + k0);
-        rb0 = *(const uint4*)(gb + k0);
-        rb1 = *(const uint4*)(gb + gs + k0);
-        rb2 = *(const uint4*)(gb + 2 * gs + k0);
-        rb3 = *(const uint4*)(gb + 3 * gs + k0);
-    };
-    auto lstore = [&](int buf) {
-        T* a = As + buf * G2_M * G2_LD + lo;
-        T* b = Bs + buf * G2_N * G2_LD + lo;
-        *(uint4*)(a) = ra0;
-        *(uint4*)(a + ls) = ra1;
-        *(uint4*)(a + 2 * ls) = ra2;
-        *(uint4*)(a + 3 * ls) = ra3;
-        *(uint4*)(b) = rb0;
-        *(uint4*)(b + ls) = rb1;
-        *(uint4*)(b + 2 * ls) = rb2;
-        *(uint4*)(b + 3 * ls) = rb3;
-    };
+    const int64_t gs = (int64_t)32 * K;                 // 32 rows further
+    const int lo = grow * G2_LD + gq, ls = 32 * G2_LD;
+#define FDMC_G2_SET(p) uint4 p##a0, p##a1, p##a2, p##a3, p##a4, p##a5, p##a6, p##a7, \
+                             p##b0, p##b1, p##b2, p##b3, p##b4, p##b5, p##b6, p##b7
+#define FDMC_G2_LOAD(p, k0)                                     \
+    do {                                                        \
+        const int k_ = (k0);                                    \
+        p##a0 = *(const uint4*)(ga + k_);                       \
+        p##a1 = *(const uint4*)(ga + gs + k_);                  \
+        p##a2 = *(const uint4*)(ga + 2 * gs + k_);              \
+        p##a3 = *(const uint4*)(ga + 3 * gs + k_);              \
+        p##a4 = *(const uint4*)(ga + 4 * gs + k_);              \
+        p##a5 = *(const uint4*)(ga + 5 * gs + k_);              \
+        p##a6 = *(const uint4*)(ga + 6 * gs + k_);              \
+        p##a7 = *(const uint4*)(ga + 7 * gs + k_);              \
+        p##b0 = *(const uint4*)(gb + k_);                       \
+        p##b1 = *(const uint4*)(gb + gs + k_);                  \
+        p##b2 = *(const uint4*)(gb + 2 * gs + k_);              \
+        p##b3 = *(const uint4*)(gb + 3 * gs + k_);              \
+        p##b4 = *(const uint4*)(gb + 4 * gs + k_);              \
+        p##b5 = *(const uint4*)(gb + 5 * gs + k_);              \
+        p##b6 = *(const uint4*)(gb + 6 * gs + k_);              \
+        p##b7 = *(const uint4*)(gb + 7 * gs + k_);              \
+    } while (0)
+#define FDMC_G2_STORE(p, buf)                                   \
+    do {                                                        \
+        T* a_ = As + (buf) * G2_M * G2_LD + lo;                 \
+        T* b_ = Bs + (buf) * G2_N * G2_LD + lo;                 \
+        *(uint4*)(a_) = p##a0;                                  \
+        *(uint4*)(a_ + ls) = p##a1;                             \
+        *(uint4*)(a_ + 2 * ls) = p##a2;                         \
+        *(uint4*)(a_ + 3 * ls) = p##a3;                         \
+        *(uint4*)(a_ + 4 * ls) = p##a4;                         \
+        *(uint4*)(a_ + 5 * ls) = p##a5;                         \
+        *(uint4*)(a_ + 6 * ls) = p##a6;                         \
+        *(uint4*)(a_ + 7 * ls) = p##a7;                         \
+        *(uint4*)(b_) = p##b0;                                  \
+        *(uint4*)(b_ + ls) = p##b1;                             \
+        *(uint4*)(b_ + 2 * ls) = p##b2;                         \
+        *(uint4*)(b_ + 3 * ls) = p##b3;                         \
+        *(uint4*)(b_ + 4 * ls) = p##b4;                         \
+        *(uint4*)(b_ + 5 * ls) = p##b5;                         \
+        *(uint4*)(b_ + 6 * ls) = p##b6;                         \
+        *(uint4*)(b_ + 7 * ls) = p##b7;                         \
+    } while (0)
+    FDMC_G2_SET(x);
+    FDMC_G2_SET(y);
     const int lr = lane & 31, lk = (lane >> 5) * 8;
     auto compute = [&](int buf) {
         const T* Ab = As + buf * G2_M * G2_LD;
@@ -270,20 +296,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 for (int j = 0; j < 4; j++) acc[i][j] = mfma32<T>(fa[i], fb[j], acc[i][j]);
         }
     };
-    gload(0);
-    lstore(0);
+    const int nk = K / G2_K;                 // even (checked on the host)
+    const int klast = (nk - 1) * G2_K;
+    FDMC_G2_LOAD(x, 0);
+    FDMC_G2_STORE(x, 0);
+    FDMC_G2_LOAD(x, G2_K);                   // tile 1 in flight
     __syncthreads();
-    const int nk = K / G2_K;
-    // the last k tile is peeled so that every staged load in the loop is
-    // unconditional (the staging registers then stay registers)
-    for (int kt = 0; kt + 1 < nk; kt++) {
-        const int buf = kt & 1;
-        gload((kt + 1) * G2_K);
-        compute(buf);
-        lstore(buf ^ 1);
+    // two k tiles per trip so that the register sets alternate; loads past the
+    // last tile are clamped to it (their LDS copy is never read)
+    for (int kt = 0; kt < nk; kt += 2) {
+        FDMC_G2_LOAD(y, min((kt + 2) * G2_K, klast));
+        compute(0);
+        FDMC_G2_STORE(x, 1);                 // tile kt + 1
+        __syncthreads();
+        FDMC_G2_LOAD(x, min((kt + 3) * G2_K, klast));
+        compute(1);
+        FDMC_G2_STORE(y, 0);                 // tile kt + 2
         __syncthreads();
     }
-    compute((nk - 1) & 1);
+#undef FDMC_G2_SET
+#undef FDMC_G2_LOAD
+#undef FDMC_G2_STORE
     // D layout of 32x32 MFMA: lane l holds column l % 32, element e at row
     // (e / 4) * 8 + (l / 32) * 4 + e % 4.
 #pragma unroll
@@ -294,7 +327,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             for (int e = 0; e < 16; e++) {
                 const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
                 const int col = n0 + wn + j * 32 + lr;
-                C[(int64_t)row * N + col] = acc[i][j][e];
+                __builtin_nontemporal_store(acc[i][j][e], C + (int64_t)row * N + col);   // streamed once
             }
 }
 
@@ -323,7 +356,7 @@ hipError_t launch_gen_z(const uint32_t* sdir, int D, int ld, int R, int64_t firs
 
 hipError_t launch_gemm_nt(const void* A, const void* Bt, float* C, int Mrows, int N, int K, bool bf16,
                           hipStream_t st) {
-    if (Mrows % G2_M == 0 && N % G2_N == 0 && K % G2_K == 0) {
+    if (Mrows % G2_M == 0 && N % G2_N == 0 && K % (2 * G2_K) == 0) {
         const int64_t tiles = (int64_t)(Mrows / G2_M) * (N / G2_N);
         const int64_t grid = (tiles + 7) / 8 * 8;
         if (grid > 0x7FFFFFFF) return hipErrorInvalidValue;
