@@ -37,16 +37,17 @@ __device__ __forceinline__ float icdf_u32(uint32_t s) {
     const uint32_t f = upper ? ~s : s;
     const float q = fmaf((float)f, 0x1p-32f, 0x1p-33f);
     const float t = fmaf(-q, q, q);                       // q (1 - q)
-    const float w = -(fast_log2(t) + 2.0f) * 0.69314718055994531f;
+    const float u = fast_log2(t);                         // w = -ln2 (u + 2)
     float p;
-    if (w < FDMC_ICDF_W_SPLIT) {
-        constexpr float c[] = FDMC_ICDF_CENTRAL_COEFFS;
-        const float x = w - FDMC_ICDF_C_CENTRAL;
+    if (u > FDMC_ICDF_U_SPLIT) {                          // w < FDMC_ICDF_W_SPLIT
+        constexpr float c[] = FDMC_ICDF_CENTRAL_COEFFS_U; // same polynomial, in u - U0
+        const float x = u - FDMC_ICDF_U0;
         p = c[0];
 #pragma unroll
         for (int i = 1; i <= FDMC_ICDF_DEG_CENTRAL; i++) p = fmaf(p, x, c[i]);
     } else {
         constexpr float c[] = FDMC_ICDF_TAIL_COEFFS;
+        const float w = -(u + 2.0f) * 0.69314718055994531f;
         const float x = fast_sqrt(w) - FDMC_ICDF_C_TAIL;
         p = c[0];
 #pragma unroll
