@@ -128,17 +128,12 @@ BookLaunch make_launch(fdmc_book* b, bool plain, int c, double eps) {
     return a;
 }
 
-int occupancy(const void* fn, int skg) {
+int occupancy(const void* fn) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, book_lds_bytes(skg)) != hipSuccess) nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 256, 0) != hipSuccess) nb = 1;   // static LDS
     return std::max(nb, 1);
 }
 
-void prepare_kernel(const void* fn, int skg) {
-    const size_t lds = book_lds_bytes(skg);
-    if (lds > 160 * 1024) fail(FDMC_E_UNSUPPORTED, "book LDS footprint too large");
-    if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-}
 
 // Launch-shape autotuner: times one sweep of every entry of the book at a fixed
 // size (512 points per (contract, level, replica)) for each candidate
@@ -164,8 +159,7 @@ void autotune(fdmc_book* b) {
         for (int occ = 0; occ < 2; occ++)
             for (int skg = 0; skg < 2; skg++) {
                 const void* fn = book_kernel(b->mk, skg, b->greeks, occ);
-                prepare_kernel(fn, skg);
-                const int nb = std::min(occupancy(fn, skg), 4);
+                const int nb = std::min(occupancy(fn), 4);
                 for (int wpc = 1; wpc <= nb; wpc++) {
                     const int grid = std::min(c->n_cu * wpc, b->max_grid);
                     BookLaunch a = make_launch(b, false, 0, 1.0);
@@ -531,7 +525,6 @@ fdmc_status fdmc_book_run(fdmc_book* b, double eps, int32_t mode, fdmc_book_resu
         HIPC(hipSetDevice(b->ctx->device));
         fdmc_book_stats st{};
         if (!b->tuned) autotune(b);
-        prepare_kernel(book_kernel(b->mk, b->skg, b->greeks, b->occ), b->skg);
         if (mode == FDMC_BOOK_BATCHED) run_batched(b, eps, &st);
         else run_plain(b, eps, &st);
         const double t0 = now_s();
@@ -557,7 +550,6 @@ fdmc_status fdmc_book_level_sums(fdmc_book* b, int32_t c, int32_t l, int64_t off
         HIPC(hipSetDevice(b->ctx->device));
         fdmc_context* ctx = b->ctx;
         if (!b->tuned) autotune(b);
-        prepare_kernel(book_kernel(b->mk, b->skg, b->greeks, b->occ), b->skg);
         const int ne = (C.L + 1) * b->R;
         std::vector<BookEntry> e(b->h_plain.begin() + C.entry0, b->h_plain.begin() + C.entry0 + ne);
         for (auto& x : e) {

@@ -57,6 +57,33 @@ __device__ __forceinline__ float icdf_u32(uint32_t s) {
     return upper ? y : -y;
 }
 
+// The same f(w) as a table of cubics, 16 segments per binade of t = q (1-q):
+// the row is bits 19..27 of t's FP32 encoding (low 5 exponent bits, top 4
+// mantissa bits; t in [2^-33, 1/4] spans 32 binades), the variable the
+// remaining 19 mantissa bits as an integer.  No logarithm and no branch: a
+// 16-byte LDS read and three FMAs.  Kernels copy the table into LDS with
+// icdf_tab_load before use.
+static __constant__ __align__(16) float g_icdf_tab[FDMC_ICDF_TAB_ROWS * 4] = FDMC_ICDF_TAB;
+
+__device__ __forceinline__ void icdf_tab_load(float4* lds, int tid, int nthreads) {
+    for (int i = tid; i < FDMC_ICDF_TAB_ROWS; i += nthreads) lds[i] = ((const float4*)g_icdf_tab)[i];
+}
+
+// -Phi^-1((s + 1/2) 2^-32) from the LDS table: the result takes its sign bit
+// from s with one OR, and callers fold the negation into the scale they apply.
+__device__ __forceinline__ float icdf_u32_tab_neg(uint32_t s, const float4* tab) {
+    const uint32_t f = s ^ (uint32_t)((int32_t)s >> 31);   // folded to the lower half
+    const float q = fmaf((float)f, 0x1p-32f, 0x1p-33f);
+    const uint32_t b = __float_as_uint(fmaf(-q, q, q));    // t = q (1 - q)
+    const float4 c = *(const float4*)((const char*)tab + ((b >> 15) & 0x1FF0u));
+    const float x = (float)(b & 0x7FFFFu);
+    const float p = fmaf(fmaf(fmaf(c.w, x, c.z), x, c.y), x, c.x);
+    const float y = fmaf(-2.0f, q, 1.0f) * p;               // (1 - 2q) f >= 0
+    return __uint_as_float(__float_as_uint(y) | (s & 0x80000000u));
+}
+
+__device__ __forceinline__ float icdf_u32_tab(uint32_t s, const float4* tab) { return -icdf_u32_tab_neg(s, tab); }
+
 // ---- Philox4x32-10 -------------------------------------------------------
 struct u32x4 { uint32_t x, y, z, w; };
 

@@ -43,7 +43,6 @@ const void* basket_kernel(int mk, int src, int na);
 size_t basket_lds_bytes(int src, int D, int na);
 
 // k_book.hip
-size_t book_lds_bytes(int skg);
 // gm: 0 price, 1 adjoint Greeks, 2 tangent Greeks; occ 1: the three-waves-per-SIMD compilation
 const void* book_kernel(int mk, int skg, int gm, int occ);
 const void* book_kernel_o3(int mk, int skg, int gm);   // k_book_o3.hip

@@ -250,9 +250,6 @@ __global__ void k_book_final(BookLaunch a) {
 }
 
 // ---- host launchers ---------------------------------------------------------------
-size_t book_lds_bytes(int skg) {
-    return (size_t)BOOK_DMAX * 20 * 4 + (skg ? 0 : (size_t)(BOOK_GMAX + 1) * 256 * 4) + 4 * BOOK_NACC * 8 + 16;
-}
 
 template <int MK, int SKG, int GM>
 struct BookDefault {
@@ -281,7 +278,7 @@ hipError_t launch_book_sweep(const void* fn, int skg, const BookLaunch& a, int g
     hipError_t e = hipSuccess;
     if (e0 && (e = hipEventRecord(e0, s)) != hipSuccess) return e;
     void* args[] = {(void*)&a};
-    e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, book_lds_bytes(skg), s);
+    e = hipLaunchKernel(fn, dim3(grid), dim3(256), args, 0, s);   // LDS is static
     if (e != hipSuccess) return e;
     if (e1 && (e = hipEventRecord(e1, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_book_reduce, dim3((a.n_entries + 3) / 4), dim3(256), 0, s, a);

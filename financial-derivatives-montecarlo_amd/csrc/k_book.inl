@@ -200,21 +200,22 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
 // tangents (carried along the forward pass; no second pass).
 template <int MK, int PK, int GM>
 __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                          const float* __restrict__ vt, const SobolLane& sl, float* sk, float* ckf,
-                                          float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
+                                          const float* __restrict__ vt, const SobolLane& sl, const float4* itab,
+                                          float* sk, float* ckf, float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
     constexpr bool LIP = PK != PK_BARRIER;
     constexpr bool ADJ = LIP && GM == 1;
     constexpr bool TAN = LIP && GM == 2;
     const int G = g.G, nf = g.nf, nc = g.nc;
     // bridge skeleton: W at block boundaries (every date when nf < 16)
     sk[0] = 0.0f;
-    sk[G * 256] = sdt[0] * icdf_u32(sobol_coord(sl, 0));
+    // icdf_u32_tab_neg gives -z: the bridge subtracts sd * (-z)
+    sk[G * 256] = -sdt[0] * icdf_u32_tab_neg(sobol_coord(sl, 0), itab);
     for (int j = 1; j < G; j++) {
         const int lev = 31 - __builtin_clz(j);
         const int q = j - (1 << lev);
         const int st = G >> lev;
         const int L = q * st, R = L + st, M = L + (st >> 1);
-        sk[M * 256] = 0.5f * (sk[L * 256] + sk[R * 256]) + sdt[lev + 1] * icdf_u32(sobol_coord(sl, j));
+        sk[M * 256] = 0.5f * (sk[L * 256] + sk[R * 256]) - sdt[lev + 1] * icdf_u32_tab_neg(sobol_coord(sl, j), itab);
     }
     const int lgG = 31 - __builtin_clz(G);
     float sdb[4];
@@ -231,7 +232,7 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
             for (int q = 0; q < (1 << lv); q++) {
                 const int s = 16 >> lv;
                 const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                W[Mp] = 0.5f * (W[Lp] + W[Rp]) + sdb[lv] * icdf_u32(sobol_coord(sl, j0 + q));
+                W[Mp] = 0.5f * (W[Lp] + W[Rp]) - sdb[lv] * icdf_u32_tab_neg(sobol_coord(sl, j0 + q), itab);
             }
         }
 #pragma unroll
@@ -339,12 +340,15 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
 // LDS per workgroup, more workgroups per CU).
 template <int MK, int SKG, int GM>
 __device__ __forceinline__ void book_body(const BookLaunch& a) {
-    extern __shared__ __align__(16) unsigned char smem[];
+    // static LDS (its size depends only on SKG): addresses fold into the
+    // instructions' offset fields instead of adding a dynamic base to each
+    __shared__ __align__(16) unsigned char smem[book_lds_static(SKG)];
     const int tid = threadIdx.x, lane = tid & 63;
     uint32_t* rec = (uint32_t*)smem;                                     // [BOOK_DMAX][20]
     constexpr size_t REC_BYTES = (size_t)BOOK_DMAX * 20 * 4;
     constexpr size_t SKEL_BYTES = SKG ? 0 : (size_t)(BOOK_GMAX + 1) * 256 * 4;
-    double* red = (double*)(smem + REC_BYTES + SKEL_BYTES);
+    float4* itab = (float4*)(smem + REC_BYTES + SKEL_BYTES);             // inverse-CDF table
+    double* red = (double*)(smem + REC_BYTES + SKEL_BYTES + (size_t)FDMC_ICDF_TAB_ROWS * 16);
     int* s_item = (int*)(red + 4 * BOOK_NACC);
     float* wscr = a.scratch + (size_t)blockIdx.x * book_scratch_floats(SKG);
     float* ckf = wscr + tid;
@@ -364,6 +368,7 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         sl.hi_gray = 0;
         sl.lds_mode = true;
     }
+    icdf_tab_load(itab, tid, 256);        // visible after the loop's first barrier
     const int total = *a.total_items;     // written by k_book_scan
     int last_e = -1;
     for (;;) {
@@ -434,10 +439,10 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
             const int64_t idx = base + tid;
             const bool valid = idx >= E.first && idx < E.first + E.count;
             switch (kind) {
-            case PK_EUROPEAN: book_path<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
-            case PK_ASIAN: book_path<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
-            case PK_LOOKBACK: book_path<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
-            default: book_path<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, sk, ckf, ckc, valid, acc); break;
+            case PK_EUROPEAN: book_path<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+            case PK_ASIAN: book_path<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+            case PK_LOOKBACK: book_path<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+            default: book_path<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
             }
         }
         block_sum<BOOK_NACC, 4>(acc, red);

@@ -6,6 +6,8 @@
 // launches on one stream never race on shared constant memory.
 #pragma once
 
+#include "icdf_coeffs.h"
+
 #include <stddef.h>
 #include <stdint.h>
 
@@ -198,6 +200,15 @@ struct BookLaunch {
     float* scratch;           // per workgroup: checkpoints [2][BOOK_GMAX][256] (+ skeleton)
     BookOut* out;
 };
+
+// LDS of one book workgroup: direction records [BOOK_DMAX][20] words, the
+// bridge skeleton when it is kept in LDS, the inverse-CDF table, the block
+// reduction and the work-item slot.
+constexpr size_t book_lds_static(int skel_global) {
+    return (size_t)BOOK_DMAX * 20 * 4 + (skel_global ? 0 : (size_t)(BOOK_GMAX + 1) * 256 * 4) +
+           (size_t)FDMC_ICDF_TAB_ROWS * 16 + 4 * BOOK_NACC * 8 + 16;
+}
+static_assert(book_lds_static(0) <= 160 * 1024, "book LDS footprint exceeds a CU's LDS");
 
 constexpr size_t book_scratch_floats(int skel_global) {
     return (size_t)2 * BOOK_GMAX * 256 + (skel_global ? (size_t)(BOOK_GMAX + 1) * 256 : 0);

@@ -67,9 +67,50 @@ def u_form(p1):
     return u0, u_split, d
 
 
+TAB_SUB = 4          # 2^4 segments per binade of t
+TAB_DEG = 3
+TAB_XBITS = 23 - TAB_SUB
+
+
+def tab_exponent(row):
+    """Biased FP32 exponent of t for a table row: the row index is bits 19..27 of
+    t's FP32 encoding (the low 5 exponent bits and the top 4 mantissa bits), and t
+    lies in [2^-33, 2^-2], i.e. biased exponents 94..125."""
+    e5 = row >> TAB_SUB
+    return e5 + 96 if e5 <= 29 else e5 + 64
+
+
+def fit_table():
+    """Piecewise cubic f(t) on 1/16 of a binade of t = q(1-q) each, in the
+    segment's integer mantissa x = bits(t) & (2^19 - 1): the device reads the row
+    from the bits of t, with no logarithm and no branch.  Row (E, j) covers
+    t = 2^(E-127) (1 + (j + x 2^-19) / 16); the point t = 1/4 (q = 1/2 to FP32
+    rounding, biased exponent 125) is the constant f(0) = sqrt(pi/2), and rows no
+    t in [2^-33, 1/4] reaches repeat it."""
+    rows = 1 << (5 + TAB_SUB)
+    tab = np.zeros((rows, TAB_DEG + 1))
+    f0 = np.sqrt(np.pi / 2.0)
+    worst = 0.0
+    X = 0.5 - 0.5 * np.cos(np.linspace(0.0, np.pi, 513))     # Chebyshev-spaced, both ends
+    X = np.minimum(X, 1.0 - 1e-9)
+    for r in range(rows):
+        E = tab_exponent(r)
+        j = r & ((1 << TAB_SUB) - 1)
+        if E >= 125:
+            tab[r, 0] = f0
+            continue
+        t = 2.0 ** (E - 127) * (1.0 + (j + X) / (1 << TAB_SUB))
+        f = f_of_w(-np.log(4.0 * t))
+        a = np.polyfit(X, f, TAB_DEG, w=1.0 / f)[::-1]          # a_0 .. a_3 in X in [0, 1)
+        worst = max(worst, np.max(np.abs(np.polyval(a[::-1], X) / f - 1.0)))
+        tab[r] = a * 2.0 ** (-TAB_XBITS * np.arange(TAB_DEG + 1))   # in x = X 2^19
+    return tab, worst
+
+
 def main():
     p1, e1, p2, e2 = fit()
     u0, u_split, d = u_form(p1)
+    tab, e3 = fit_table()
     out = os.path.join(os.path.dirname(__file__), "..", "csrc", "icdf_coeffs.h")
     with open(out, "w") as f:
         f.write("/* Generated by gen/fit_icdf.py -- do not edit.\n")
@@ -89,7 +130,15 @@ def main():
         f.write("#define FDMC_ICDF_U0 %.17gf\n" % u0)
         f.write("#define FDMC_ICDF_U_SPLIT %.17gf\n" % u_split)
         f.write("#define FDMC_ICDF_CENTRAL_COEFFS_U {" + ", ".join("%.9ef" % c for c in d) + "}\n")
-    print("central err %.3e tail err %.3e -> %s" % (e1, e2, out))
+        f.write("/* Table form (piecewise cubic in the bits of t = q(1-q)); row r = bits(t) >> 19\n")
+        f.write(" * & 511 holds {c0, c1, c2, c3} of f = c0 + x (c1 + x (c2 + x c3)), x = bits(t) & 0x7FFFF.\n")
+        f.write(" * max relative fit error %.3e (FP64 evaluation). */\n" % e3)
+        f.write("#define FDMC_ICDF_TAB_ROWS %d\n" % len(tab))
+        f.write("#define FDMC_ICDF_TAB { \\\n")
+        for r in range(len(tab)):
+            f.write("    " + ", ".join("%.9ef" % c for c in tab[r]) + (", \\\n" if r + 1 < len(tab) else " \\\n"))
+        f.write("}\n")
+    print("central err %.3e tail err %.3e table err %.3e -> %s" % (e1, e2, e3, out))
 
 
 if __name__ == "__main__":

@@ -1,8 +1,10 @@
 """The FP32 inverse normal CDF's generated constants (csrc/icdf_coeffs.h), evaluated
-on the CPU the way icdf_u32 (csrc/device_math.hpp) evaluates them: FP32 Horner
-steps, q = (f + 1/2) 2^-32 folded to the lower half, u = log2(q (1 - q)).  The
-device's log2 is the hardware's (1 ulp); here it is the correctly rounded one, so
-the bound checked here is a little tighter than the GPU test's 4e-7."""
+on the CPU the way csrc/device_math.hpp evaluates them: FP32 Horner steps,
+q = (f + 1/2) 2^-32 folded to the lower half, t = q (1 - q), and either
+u = log2(t) with the two-branch polynomial (icdf_u32) or the piecewise cubic read
+from the bits of t (icdf_u32_tab).  The device's log2 is the hardware's (1 ulp);
+here it is the correctly rounded one, so the bound checked here is a little
+tighter than the GPU test's 4e-7."""
 import os
 import re
 
@@ -24,7 +26,10 @@ def _consts():
     def val(name):
         return F32(float(re.search(r"#define %s ([^\s]+?)f?\n" % name, h).group(1)))
 
-    return dict(cw=arr("FDMC_ICDF_CENTRAL_COEFFS"), cu=arr("FDMC_ICDF_CENTRAL_COEFFS_U"),
+    tab = np.array([float(x.strip().rstrip("f")) for x in
+                    re.search(r"#define FDMC_ICDF_TAB \{(.*?)\}", h, re.S).group(1).replace("\\", "").split(",")],
+                   np.float32).reshape(-1, 4)
+    return dict(tab=tab, cw=arr("FDMC_ICDF_CENTRAL_COEFFS"), cu=arr("FDMC_ICDF_CENTRAL_COEFFS_U"),
                 ct=arr("FDMC_ICDF_TAIL_COEFFS"), u0=val("FDMC_ICDF_U0"), us=val("FDMC_ICDF_U_SPLIT"),
                 cc=val("FDMC_ICDF_C_CENTRAL"), ctl=val("FDMC_ICDF_C_TAIL"), ws=val("FDMC_ICDF_W_SPLIT"))
 
@@ -47,6 +52,38 @@ def _icdf(s, k):
     pt = _horner(k["ct"], (np.sqrt(w) - k["ctl"]).astype(np.float32))
     y = ((F32(1.0) - F32(2.0) * q) * np.where(u > k["us"], pc, pt)).astype(np.float32)
     return np.where(upper, y, -y).astype(np.float64), q.astype(np.float64), upper
+
+
+def _icdf_tab(s, k):
+    upper = (s & 0x80000000) != 0
+    f = np.where(upper, ~s, s).astype(np.uint32)
+    q = (f.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32)
+    t = (-q * q + q).astype(np.float32)
+    b = t.view(np.uint32)
+    c = k["tab"][(b >> 19) & 0x1FF]
+    x = (b & 0x7FFFF).astype(np.float32)
+    p = _horner(c[:, ::-1].T, x)
+    y = ((F32(1.0) - F32(2.0) * q) * p).astype(np.float32)
+    return np.where(upper, y, -y).astype(np.float64), q.astype(np.float64), upper
+
+
+def _samples():
+    return np.concatenate([np.random.default_rng(7).integers(0, 2 ** 32, 1_000_000, dtype=np.uint64),
+                           np.arange(0, 4096, dtype=np.uint64), 2 ** 32 - 1 - np.arange(0, 4096, dtype=np.uint64),
+                           (2 ** 31 + np.arange(-4096, 4096)).astype(np.uint64),
+                           np.logspace(0, 31.9, 20000, base=2.0).astype(np.uint64)]).astype(np.uint32)
+
+
+def test_fp32_evaluation_error_table_form():
+    k = _consts()
+    s = _samples()
+    y, q, upper = _icdf_tab(s, k)
+    ref = np.where(upper, -ndtri(q), ndtri(q))
+    err = np.abs(y - ref) / np.maximum(1.0, np.abs(ref))
+    assert err.max() < 2.5e-7, (err.max(), s[np.argmax(err)])
+    # every row the kernels can read is fitted (no default row is reached but t = 1/4)
+    b = (-(q * q) + q).astype(np.float32).view(np.uint32)
+    assert set(np.unique((b >> 23) & 0xFF)) <= set(range(94, 126))
 
 
 def test_fp32_evaluation_error():
