@@ -69,20 +69,26 @@ __device__ __forceinline__ void icdf_tab_load(float4* lds, int tid, int nthreads
     for (int i = tid; i < FDMC_ICDF_TAB_ROWS; i += nthreads) lds[i] = ((const float4*)g_icdf_tab)[i];
 }
 
-// -Phi^-1((s + 1/2) 2^-32) from the LDS table: the result takes its sign bit
-// from s with one OR, and callers fold the negation into the scale they apply.
-__device__ __forceinline__ float icdf_u32_tab_neg(uint32_t s, const float4* tab) {
-    const uint32_t f = s ^ (uint32_t)((int32_t)s >> 31);   // folded to the lower half
-    const float q = fmaf((float)f, 0x1p-32f, 0x1p-33f);
+// -sd Phi^-1((s + 1/2) 2^-32) from the LDS table, for a scale sd > 0 and
+// m2sd = -2 sd (kept by the caller).  The fold to the lower half comes from
+// reading s as a signed integer i: f + 1/2 = |i + 1/2| for f = s (i >= 0) and
+// f = ~s (i < 0), and the absolute value is a free source modifier.  (Near
+// u = 1/2, i = 2^31 - 1 rounds to 2^31 in FP32 and the result's sign can flip
+// on a value of size 2^-32 sd.)  The result takes its sign bit from s with one
+// OR, and callers fold the negation into the sum they add it to.
+__device__ __forceinline__ float icdf_u32_tab_scaled_neg(uint32_t s, const float4* tab, float sd, float m2sd) {
+    const float q = fabsf(fmaf((float)(int32_t)s, 0x1p-32f, 0x1p-33f));
     const uint32_t b = __float_as_uint(fmaf(-q, q, q));    // t = q (1 - q)
     const float4 c = *(const float4*)((const char*)tab + ((b >> 15) & 0x1FF0u));
     const float x = (float)(b & 0x7FFFFu);
     const float p = fmaf(fmaf(fmaf(c.w, x, c.z), x, c.y), x, c.x);
-    const float y = fmaf(-2.0f, q, 1.0f) * p;               // (1 - 2q) f >= 0
+    const float y = fmaf(q, m2sd, sd) * p;                 // (1 - 2q) sd f >= 0
     return __uint_as_float(__float_as_uint(y) | (s & 0x80000000u));
 }
 
-__device__ __forceinline__ float icdf_u32_tab(uint32_t s, const float4* tab) { return -icdf_u32_tab_neg(s, tab); }
+__device__ __forceinline__ float icdf_u32_tab(uint32_t s, const float4* tab) {
+    return -icdf_u32_tab_scaled_neg(s, tab, 1.0f, -2.0f);
+}
 
 // ---- Philox4x32-10 -------------------------------------------------------
 struct u32x4 { uint32_t x, y, z, w; };

@@ -89,6 +89,10 @@ __device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t,
     S = book_euler<MK>(P, S, dw, v, muh);
 }
 
+// Greeks carried as forward tangents along the path (GM = 2, payoffs with a
+// pathwise derivative).
+#define LIP_TAN(PK, GM) ((PK) != PK_BARRIER && (GM) == 2)
+
 // Running payoff state of one leg (and, with TAN, the tangent sums the
 // payoff's derivative needs: sum over dates for the Asian, the tangent at the
 // running minimum for the lookback).
@@ -155,11 +159,12 @@ struct BookAdj {
 // For a one-dimensional state this is the exact vector-Jacobian product of the
 // block, with no per-step storage.
 // PAIR: the step increments are dw[2k] + dw[2k+1] (coarse leg), whose step
-// times are the even fine times (vt holds the fine grid's time factors).
+// times are the even fine times (vb holds the block's fine-grid time factors,
+// in registers).
 template <int MK, int N, bool PAIR>
 __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, const float* dw, int nsteps, int k0,
-                                               const float* __restrict__ vt, float h, float muh, int n, float each,
-                                               float last, float cmin, int kmin, BookAdj& A) {
+                                               const float* vb, float h, float muh, int n, float each, float last,
+                                               float cmin, int kmin, BookAdj& A) {
     float jS = 1.0f, jv = 0.0f, jm = 0.0f, j0 = 0.0f;
     float dS = 0.0f, dv = 0.0f, dm = 0.0f, d0 = 0.0f;
 #pragma unroll
@@ -167,7 +172,7 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
         if (k < nsteps) {
             const int gi = k0 + k;
             const float w = PAIR ? dw[2 * k] + dw[2 * k + 1] : dw[k];
-            const float v = vt[PAIR ? 2 * gi : gi];
+            const float v = vb[PAIR ? 2 * k : k];
             float fS, fv, fm, f0;
             book_partials<MK>(P, S, w, v, h, muh, fS, fv, fm, f0);
             S = book_euler<MK>(P, S, w, v, muh);
@@ -189,90 +194,17 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
     A.adj = dS + A.adj * jS;
 }
 
-// Brownian bridge on the uniform grid of 2^l dates (bb_precompute,
-// utils.cpp:54-101): every midpoint weight is 1/2 and the conditional sd
-// depends only on the depth, so the bridge is W_M = (W_L + W_R) / 2 + sd[k] z
-// with sdt[0] = sqrt(T) for the terminal value and sdt[k] for the midpoints
-// of schedule level k - 1 (FP32 values of bb_precompute's weights).
-//
-// GM: 0 price only, 1 adjoint Greeks (reverse sweep over 16-step blocks with
-// checkpoints, increments regenerated from the Sobol index), 2 forward
-// tangents (carried along the forward pass; no second pass).
-template <int MK, int PK, int GM>
-__device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                          const float* __restrict__ vt, const SobolLane& sl, const float4* itab,
-                                          float* sk, float* ckf, float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
-    constexpr bool LIP = PK != PK_BARRIER;
-    constexpr bool ADJ = LIP && GM == 1;
-    constexpr bool TAN = LIP && GM == 2;
-    const int G = g.G, nf = g.nf, nc = g.nc;
-    // bridge skeleton: W at block boundaries (every date when nf < 16)
-    sk[0] = 0.0f;
-    // icdf_u32_tab_neg gives -z: the bridge subtracts sd * (-z)
-    sk[G * 256] = -sdt[0] * icdf_u32_tab_neg(sobol_coord(sl, 0), itab);
-    for (int j = 1; j < G; j++) {
-        const int lev = 31 - __builtin_clz(j);
-        const int q = j - (1 << lev);
-        const int st = G >> lev;
-        const int L = q * st, R = L + st, M = L + (st >> 1);
-        sk[M * 256] = 0.5f * (sk[L * 256] + sk[R * 256]) - sdt[lev + 1] * icdf_u32_tab_neg(sobol_coord(sl, j), itab);
-    }
-    const int lgG = 31 - __builtin_clz(G);
-    float sdb[4];
-#pragma unroll
-    for (int lv = 0; lv < 4; lv++) sdb[lv] = nf >= 16 ? sdt[lgG + lv + 1] : 0.0f;
-    auto gen_block = [&](int i, float (&dw)[16]) {
-        float W[17];
-        W[0] = sk[i * 256];
-        W[16] = sk[(i + 1) * 256];
-#pragma unroll
-        for (int lv = 0; lv < 4; lv++) {
-            const int j0 = (G << lv) + (i << lv);
-#pragma unroll
-            for (int q = 0; q < (1 << lv); q++) {
-                const int s = 16 >> lv;
-                const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                W[Mp] = 0.5f * (W[Lp] + W[Rp]) - sdb[lv] * icdf_u32_tab_neg(sobol_coord(sl, j0 + q), itab);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 16; k++) dw[k] = W[k + 1] - W[k];
-    };
-
-    // ---- forward
-    BookLeg<MK, PK, TAN> F, Cc;
-    F.init(P);
-    Cc.init(P);
-    if (nf >= 16) {
-        for (int i = 0; i < G; i++) {
-            float dw[16];
-            gen_block(i, dw);
-            if constexpr (ADJ) {
-                ckf[i * 256] = F.S;
-                ckc[i * 256] = Cc.S;
-            }
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const int gi = i * 16 + k;
-                F.step(P, dw[k], vt[gi], g.hf, g.muh_f, gi + 1);
-                if (k & 1) {
-                    const int gc = gi >> 1;
-                    Cc.step(P, dw[k - 1] + dw[k], vt[2 * gc], g.hc, g.muh_c, gc + 1);
-                }
-            }
-        }
-    } else {
-        float prev = 0.0f;
-        for (int k = 0; k < nf; k++) {
-            const float dw = sk[(k + 1) * 256] - sk[k * 256];
-            F.step(P, dw, vt[k], g.hf, g.muh_f, k + 1);
-            if (k & 1) {
-                const int gc = k >> 1;
-                Cc.step(P, prev + dw, vt[2 * gc], g.hc, g.muh_c, gc + 1);
-            }
-            prev = dw;
-        }
-    }
+// Payoffs of both legs, their Greeks and the accumulation of the level
+// difference.  `sweep(ef, lf, mf, ec, lc, mc, Af, Ac)` runs the adjoint reverse
+// sweep of the fine and coarse legs given the payoffs' direct sensitivities.
+template <int MK, int PK, int GM, class Sweep>
+__device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
+                                          const BookLeg<MK, PK, LIP_TAN(PK, GM)>& F,
+                                          const BookLeg<MK, PK, LIP_TAN(PK, GM)>& Cc, bool valid,
+                                          double (&acc)[BOOK_NACC], Sweep&& sweep) {
+    constexpr bool ADJ = PK != PK_BARRIER && GM == 1;
+    constexpr bool TAN = LIP_TAN(PK, GM);
+    const int nf = g.nf, nc = g.nc;
     const float pf = payoff<PK>(P, F.S, F.run, nf, g.bgk_f);
     const float pc = (nc > 0) ? payoff<PK>(P, Cc.S, Cc.run, nc, g.bgk_c) : 0.0f;
 
@@ -302,22 +234,7 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
         book_direct<PK>(P, F.S, F.run, nf, g.bgk_f, ef, lf, mf);
         book_direct<PK>(P, Cc.S, Cc.run, nc > 0 ? nc : 1, g.bgk_c, ec, lc, mc);
         BookAdj Af{0.0f, 0.0f, 0.0f, 0.0f}, Ac{0.0f, 0.0f, 0.0f, 0.0f};
-        if (nf >= 16) {
-            for (int i = G - 1; i >= 0; i--) {
-                float dw[16];
-                gen_block(i, dw);
-                book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vt, g.hc, g.muh_c, nc, ec, lc, mc,
-                                            Cc.kmin, Ac);
-                book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vt, g.hf, g.muh_f, nf, ef, lf, mf,
-                                              F.kmin, Af);
-            }
-        } else {
-            float dw[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) dw[k] = (k < nf) ? sk[(k + 1) * 256] - sk[k * 256] : 0.0f;
-            book_block_adj<MK, 4, true>(P, P.S0, dw, nc, 0, vt, g.hc, g.muh_c, nc, ec, lc, mc, Cc.kmin, Ac);
-            book_block_adj<MK, 8, false>(P, P.S0, dw, nf, 0, vt, g.hf, g.muh_f, nf, ef, lf, mf, F.kmin, Af);
-        }
+        sweep(ef, lf, mf, ec, lc, mc, Af, Ac);
         if (F.kmin == 0) Af.adj += mf;     // S0 itself is the running minimum
         if (Cc.kmin == 0) Ac.adj += mc;
         gf = Tan3{Af.adj + Af.s0, Af.v, Af.m};
@@ -333,6 +250,160 @@ __device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g,
         acc[1] += (double)(gf.s - gc.s);
         acc[2] += (double)(gf.v - gc.v);
         acc[3] += (double)(gf.m - gc.m);
+    }
+}
+
+// Levels with NF < 16 dates (NF = 1, 2, 4, 8): the whole bridge in registers,
+// unrolled, with no skeleton in memory.
+template <int MK, int PK, int GM, int NF>
+__device__ __forceinline__ void book_path_short(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
+                                                const float* __restrict__ vt, const SobolLane& sl,
+                                                const float4* itab, bool valid, double (&acc)[BOOK_NACC]) {
+    constexpr bool TAN = LIP_TAN(PK, GM);
+    constexpr int NC = NF / 2;
+    float W[NF + 1];
+    W[0] = 0.0f;
+    // icdf_u32_tab_scaled_neg gives -sd z: the bridge subtracts it
+    W[NF] = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0), itab, sdt[0], -2.0f * sdt[0]);
+#pragma unroll
+    for (int j = 1; j < NF; j++) {
+        const int lev = 31 - __builtin_clz(j);
+        const int q = j - (1 << lev);
+        const int st = NF >> lev;
+        const int L = q * st, R = L + st, M = L + (st >> 1);
+        const float sd = sdt[lev + 1];
+        W[M] = 0.5f * (W[L] + W[R]) - icdf_u32_tab_scaled_neg(sobol_coord(sl, j), itab, sd, -2.0f * sd);
+    }
+    float dw[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) dw[k] = W[k + 1] - W[k];
+    BookLeg<MK, PK, TAN> F, Cc;
+    F.init(P);
+    Cc.init(P);
+    float vb[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) vb[k] = vt[k];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+        F.step(P, dw[k], vb[k], g.hf, g.muh_f, k + 1);
+        if (k & 1) Cc.step(P, dw[k - 1] + dw[k], vb[k - 1], g.hc, g.muh_c, (k >> 1) + 1);
+    }
+    book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,
+                          [&](float ef, float lf, float mf, float ec, float lc, float mc, BookAdj& Af, BookAdj& Ac) {
+                              if constexpr (NC > 0)
+                                  book_block_adj<MK, NC, true>(P, P.S0, dw, NC, 0, vb, g.hc, g.muh_c, NC, ec, lc, mc,
+                                                               Cc.kmin, Ac);
+                              book_block_adj<MK, NF, false>(P, P.S0, dw, NF, 0, vb, g.hf, g.muh_f, NF, ef, lf, mf,
+                                                            F.kmin, Af);
+                          });
+}
+
+// Levels with 16 or more dates: the bridge skeleton at the 16-date block
+// boundaries lives in sk (LDS or the workgroup's scratch), each block's 15
+// inner points are generated in registers; the adjoint regenerates them block
+// by block from the Sobol index in the reverse sweep, from checkpoints of S.
+template <int MK, int PK, int GM>
+__device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
+                                               const float* __restrict__ vt, const SobolLane& sl, const float4* itab,
+                                               float* sk, float* ckf, float* ckc, bool valid,
+                                               double (&acc)[BOOK_NACC]) {
+    constexpr bool ADJ = PK != PK_BARRIER && GM == 1;
+    constexpr bool TAN = LIP_TAN(PK, GM);
+    const int G = g.G;
+    sk[0] = 0.0f;
+    sk[G * 256] = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0), itab, sdt[0], -2.0f * sdt[0]);
+    for (int j = 1; j < G; j++) {
+        const int lev = 31 - __builtin_clz(j);
+        const int q = j - (1 << lev);
+        const int st = G >> lev;
+        const int L = q * st, R = L + st, M = L + (st >> 1);
+        const float sd = sdt[lev + 1];
+        sk[M * 256] = 0.5f * (sk[L * 256] + sk[R * 256]) -
+                      icdf_u32_tab_scaled_neg(sobol_coord(sl, j), itab, sd, -2.0f * sd);
+    }
+    const int lgG = 31 - __builtin_clz(G);
+    float sdb[4], m2sdb[4];
+#pragma unroll
+    for (int lv = 0; lv < 4; lv++) {
+        sdb[lv] = sdt[lgG + lv + 1];
+        m2sdb[lv] = -2.0f * sdb[lv];
+    }
+    auto gen_block = [&](int i, float (&dw)[16]) {
+        float W[17];
+        W[0] = sk[i * 256];
+        W[16] = sk[(i + 1) * 256];
+#pragma unroll
+        for (int lv = 0; lv < 4; lv++) {
+            const int j0 = (G << lv) + (i << lv);
+#pragma unroll
+            for (int q = 0; q < (1 << lv); q++) {
+                const int s = 16 >> lv;
+                const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
+                W[Mp] = 0.5f * (W[Lp] + W[Rp]) -
+                        icdf_u32_tab_scaled_neg(sobol_coord(sl, j0 + q), itab, sdb[lv], m2sdb[lv]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) dw[k] = W[k + 1] - W[k];
+    };
+
+    BookLeg<MK, PK, TAN> F, Cc;
+    F.init(P);
+    Cc.init(P);
+    // the block's 16 local-vol time factors, loaded once into registers (the
+    // coarse step at fine index 2 gc uses the even ones)
+    auto load_vb = [&](int i, float (&vb)[16]) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) vb[k] = vt[i * 16 + k];
+    };
+    for (int i = 0; i < G; i++) {
+        float dw[16], vb[16];
+        load_vb(i, vb);
+        gen_block(i, dw);
+        if constexpr (ADJ) {
+            ckf[i * 256] = F.S;
+            ckc[i * 256] = Cc.S;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int gi = i * 16 + k;
+            F.step(P, dw[k], vb[k], g.hf, g.muh_f, gi + 1);
+            if (k & 1) Cc.step(P, dw[k - 1] + dw[k], vb[k - 1], g.hc, g.muh_c, (gi >> 1) + 1);
+        }
+    }
+    book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,
+                          [&](float ef, float lf, float mf, float ec, float lc, float mc, BookAdj& Af, BookAdj& Ac) {
+                              for (int i = G - 1; i >= 0; i--) {
+                                  float dw[16], vb[16];
+                                  load_vb(i, vb);
+                                  gen_block(i, dw);
+                                  book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vb, g.hc, g.muh_c,
+                                                              g.nc, ec, lc, mc, Cc.kmin, Ac);
+                                  book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vb, g.hf, g.muh_f,
+                                                                g.nf, ef, lf, mf, F.kmin, Af);
+                              }
+                          });
+}
+
+// Brownian bridge on the uniform grid of 2^l dates (bb_precompute,
+// utils.cpp:54-101): every midpoint weight is 1/2 and the conditional sd
+// depends only on the depth, so the bridge is W_M = (W_L + W_R) / 2 + sd[k] z
+// with sdt[0] = sqrt(T) for the terminal value and sdt[k] for the midpoints
+// of schedule level k - 1 (FP32 values of bb_precompute's weights).
+//
+// GM: 0 price only, 1 adjoint Greeks (reverse sweep over 16-step blocks with
+// checkpoints, increments regenerated from the Sobol index), 2 forward
+// tangents (carried along the forward pass; no second pass).
+template <int MK, int PK, int GM>
+__device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
+                                          const float* __restrict__ vt, const SobolLane& sl, const float4* itab,
+                                          float* sk, float* ckf, float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
+    switch (g.l) {
+    case 0: book_path_short<MK, PK, GM, 1>(P, g, sdt, vt, sl, itab, valid, acc); break;
+    case 1: book_path_short<MK, PK, GM, 2>(P, g, sdt, vt, sl, itab, valid, acc); break;
+    case 2: book_path_short<MK, PK, GM, 4>(P, g, sdt, vt, sl, itab, valid, acc); break;
+    case 3: book_path_short<MK, PK, GM, 8>(P, g, sdt, vt, sl, itab, valid, acc); break;
+    default: book_path_long<MK, PK, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
     }
 }
 
@@ -375,7 +446,9 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         __syncthreads();
         if (tid == 0) *s_item = (int)atomicAdd(a.queue, 1u);
         __syncthreads();
-        const int item = *s_item;
+        // uniform over the workgroup: the entry, contract, tables and loop bounds
+        // derived from it live in scalar registers and load through scalar loads
+        const int item = __builtin_amdgcn_readfirstlane(*s_item);
         if (item >= total) break;         // every workgroup leaves once the queue is empty
         int lo = 0, hi = a.n_entries - 1; // last entry with tile_begin <= item
         while (lo < hi) {
@@ -423,16 +496,25 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
                 }
                 last_e = lo;
             }
-            {
-                const uint32_t b32 = (uint32_t)base, gbase = b32 ^ (b32 >> 1);
-                for (int d = tid; d < D; d += 256) {
-                    const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+            // words 16..19 of a record: the wave offsets' direction sums XOR the
+            // sum over the set bits of gray(base) (bits 7..31; base is a multiple
+            // of 256 below 2^32).  The first tile of a work item sums them (loads
+            // in parallel); each further tile n changes gray(256 n) in bits 7 and
+            // 8 + ctz(n) only, so it XORs two direction numbers into the words.
+            const uint32_t n = (uint32_t)(tfirst + t);
+            for (int d = tid; d < D; d += 256) {
+                const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+                uint4* r = (uint4*)(rec + d * 20 + 16);
+                if (t == t0) {
+                    const uint32_t b32 = n << 8, gbase = b32 ^ (b32 >> 1);
                     uint32_t s = 0;
-                    for (uint32_t x = gbase; x; x &= x - 1u) s ^= gr[32 + __builtin_ctz(x)];
-                    rec[d * 20 + 16] = s ^ gr[16];
-                    rec[d * 20 + 17] = s ^ gr[17];
-                    rec[d * 20 + 18] = s ^ gr[18];
-                    rec[d * 20 + 19] = s ^ gr[19];
+#pragma unroll
+                    for (int j = 7; j < 32; j++) s ^= ((gbase >> j) & 1u) ? gr[32 + j] : 0u;
+                    *r = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
+                } else {
+                    const uint32_t dl = gr[32 + 7] ^ gr[32 + 8 + __builtin_ctz(n)];
+                    uint4 v = *r;
+                    *r = make_uint4(v.x ^ dl, v.y ^ dl, v.z ^ dl, v.w ^ dl);
                 }
             }
             __syncthreads();
