@@ -255,16 +255,19 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
 
 // Levels with NF < 16 dates (NF = 1, 2, 4, 8): the whole bridge in registers,
 // unrolled, with no skeleton in memory.
+// dm[d]: XOR of the current tile's high Sobol words relative to the words in
+// the LDS records (the work item's first tile), per dimension.
 template <int MK, int PK, int GM, int NF>
 __device__ __forceinline__ void book_path_short(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
                                                 const float* __restrict__ vt, const SobolLane& sl,
-                                                const float4* itab, bool valid, double (&acc)[BOOK_NACC]) {
+                                                const float4* itab, const uint32_t (&dm)[8], bool valid,
+                                                double (&acc)[BOOK_NACC]) {
     constexpr bool TAN = LIP_TAN(PK, GM);
     constexpr int NC = NF / 2;
     float W[NF + 1];
     W[0] = 0.0f;
     // icdf_u32_tab_scaled_neg gives -sd z: the bridge subtracts it
-    W[NF] = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0), itab, sdt[0], -2.0f * sdt[0]);
+    W[NF] = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0) ^ dm[0], itab, sdt[0], -2.0f * sdt[0]);
 #pragma unroll
     for (int j = 1; j < NF; j++) {
         const int lev = 31 - __builtin_clz(j);
@@ -272,7 +275,7 @@ __device__ __forceinline__ void book_path_short(const LegParams& P, const BookGe
         const int st = NF >> lev;
         const int L = q * st, R = L + st, M = L + (st >> 1);
         const float sd = sdt[lev + 1];
-        W[M] = 0.5f * (W[L] + W[R]) - icdf_u32_tab_scaled_neg(sobol_coord(sl, j), itab, sd, -2.0f * sd);
+        W[M] = 0.5f * (W[L] + W[R]) - icdf_u32_tab_scaled_neg(sobol_coord(sl, j) ^ dm[j], itab, sd, -2.0f * sd);
     }
     float dw[NF];
 #pragma unroll
@@ -395,15 +398,15 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
 // checkpoints, increments regenerated from the Sobol index), 2 forward
 // tangents (carried along the forward pass; no second pass).
 template <int MK, int PK, int GM>
-__device__ __forceinline__ void book_path(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                          const float* __restrict__ vt, const SobolLane& sl, const float4* itab,
-                                          float* sk, float* ckf, float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
+__device__ __forceinline__ void book_path_short_any(const LegParams& P, const BookGeom& g,
+                                                    const float* __restrict__ sdt, const float* __restrict__ vt,
+                                                    const SobolLane& sl, const float4* itab, const uint32_t (&dm)[8],
+                                                    bool valid, double (&acc)[BOOK_NACC]) {
     switch (g.l) {
-    case 0: book_path_short<MK, PK, GM, 1>(P, g, sdt, vt, sl, itab, valid, acc); break;
-    case 1: book_path_short<MK, PK, GM, 2>(P, g, sdt, vt, sl, itab, valid, acc); break;
-    case 2: book_path_short<MK, PK, GM, 4>(P, g, sdt, vt, sl, itab, valid, acc); break;
-    case 3: book_path_short<MK, PK, GM, 8>(P, g, sdt, vt, sl, itab, valid, acc); break;
-    default: book_path_long<MK, PK, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+    case 0: book_path_short<MK, PK, GM, 1>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+    case 1: book_path_short<MK, PK, GM, 2>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+    case 2: book_path_short<MK, PK, GM, 4>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+    default: book_path_short<MK, PK, GM, 8>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
     }
 }
 
@@ -486,45 +489,79 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         double acc[BOOK_NACC];
 #pragma unroll
         for (int i = 0; i < BOOK_NACC; i++) acc[i] = 0.0;
-        for (int t = t0; t < t1; t++) {
-            const int64_t base = (tfirst + t) << 8;
-            if (t > t0) __syncthreads();  // every lane is done with the previous tile's table
-            if (lo != last_e) {
-                for (int i = tid; i < D * 16; i += 256) {
-                    const int d = i >> 4, k = i & 15;
-                    rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
-                }
-                last_e = lo;
+        // LDS records of the entry (direction words of the lane and wave bits),
+        // then words 16..19: the wave offsets' direction sums XOR the sum over
+        // the set bits of gray(base) (bits 7..31; base is a multiple of 256
+        // below 2^32) for the item's first tile, loads in parallel.  The
+        // previous item ended with barriers, so the records are free.
+        if (lo != last_e) {
+            for (int i = tid; i < D * 16; i += 256) {
+                const int d = i >> 4, k = i & 15;
+                rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
             }
-            // words 16..19 of a record: the wave offsets' direction sums XOR the
-            // sum over the set bits of gray(base) (bits 7..31; base is a multiple
-            // of 256 below 2^32).  The first tile of a work item sums them (loads
-            // in parallel); each further tile n changes gray(256 n) in bits 7 and
-            // 8 + ctz(n) only, so it XORs two direction numbers into the words.
-            const uint32_t n = (uint32_t)(tfirst + t);
+            last_e = lo;
+        }
+        {
+            const uint32_t b32 = (uint32_t)(tfirst + t0) << 8, gbase = b32 ^ (b32 >> 1);
             for (int d = tid; d < D; d += 256) {
                 const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-                uint4* r = (uint4*)(rec + d * 20 + 16);
-                if (t == t0) {
-                    const uint32_t b32 = n << 8, gbase = b32 ^ (b32 >> 1);
-                    uint32_t s = 0;
+                uint32_t s = 0;
 #pragma unroll
-                    for (int j = 7; j < 32; j++) s ^= ((gbase >> j) & 1u) ? gr[32 + j] : 0u;
-                    *r = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
-                } else {
-                    const uint32_t dl = gr[32 + 7] ^ gr[32 + 8 + __builtin_ctz(n)];
-                    uint4 v = *r;
-                    *r = make_uint4(v.x ^ dl, v.y ^ dl, v.z ^ dl, v.w ^ dl);
+                for (int j = 7; j < 32; j++) s ^= ((gbase >> j) & 1u) ? gr[32 + j] : 0u;
+                *(uint4*)(rec + d * 20 + 16) = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
+            }
+        }
+        __syncthreads();
+        // Each further tile n changes gray(256 n) in bits 7 and 8 + ctz(n) only:
+        // two direction numbers per dimension.
+        if (E.l < 4) {
+            // at most 8 dimensions: each lane keeps the change since the first
+            // tile in registers, so the tiles need no barrier
+            uint32_t dm[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            for (int t = t0; t < t1; t++) {
+                const uint32_t n = (uint32_t)(tfirst + t);
+                if (t > t0) {
+                    const int c = __builtin_ctz(n);
+#pragma unroll
+                    for (int d = 0; d < 8; d++) {
+                        if (d < D) {
+                            const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+                            dm[d] ^= gr[32 + 7] ^ gr[32 + 8 + c];
+                        }
+                    }
+                }
+                const int64_t idx = ((int64_t)n << 8) + tid;
+                const bool valid = idx >= E.first && idx < E.first + E.count;
+                switch (kind) {
+                case PK_EUROPEAN: book_path_short_any<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+                case PK_ASIAN: book_path_short_any<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+                case PK_LOOKBACK: book_path_short_any<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+                default: book_path_short_any<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
                 }
             }
-            __syncthreads();
-            const int64_t idx = base + tid;
-            const bool valid = idx >= E.first && idx < E.first + E.count;
-            switch (kind) {
-            case PK_EUROPEAN: book_path<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
-            case PK_ASIAN: book_path<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
-            case PK_LOOKBACK: book_path<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
-            default: book_path<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+        } else {
+            for (int t = t0; t < t1; t++) {
+                const uint32_t n = (uint32_t)(tfirst + t);
+                if (t > t0) {
+                    __syncthreads();      // every lane is done with the previous tile's words
+                    const int c = __builtin_ctz(n);
+                    for (int d = tid; d < D; d += 256) {
+                        const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+                        const uint32_t dl = gr[32 + 7] ^ gr[32 + 8 + c];
+                        uint4* r = (uint4*)(rec + d * 20 + 16);
+                        const uint4 v = *r;
+                        *r = make_uint4(v.x ^ dl, v.y ^ dl, v.z ^ dl, v.w ^ dl);
+                    }
+                    __syncthreads();
+                }
+                const int64_t idx = ((int64_t)n << 8) + tid;
+                const bool valid = idx >= E.first && idx < E.first + E.count;
+                switch (kind) {
+                case PK_EUROPEAN: book_path_long<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+                case PK_ASIAN: book_path_long<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+                case PK_LOOKBACK: book_path_long<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+                default: book_path_long<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+                }
             }
         }
         block_sum<BOOK_NACC, 4>(acc, red);
