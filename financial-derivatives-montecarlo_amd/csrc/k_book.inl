@@ -89,6 +89,11 @@ __device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t,
     S = book_euler<MK>(P, S, dw, v, muh);
 }
 
+// Levels 0..BOOK_SHORT_L (at most BOOK_SHORT_D dates) take the register-only
+// path.
+constexpr int BOOK_SHORT_L = 4;
+constexpr int BOOK_SHORT_D = 1 << BOOK_SHORT_L;
+
 // Greeks carried as forward tangents along the path (GM = 2, payoffs with a
 // pathwise derivative).
 #define LIP_TAN(PK, GM) ((PK) != PK_BARRIER && (GM) == 2)
@@ -247,20 +252,23 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
     }
     if (valid) {
         acc[0] += (double)(pf - pc);
-        acc[1] += (double)(gf.s - gc.s);
-        acc[2] += (double)(gf.v - gc.v);
-        acc[3] += (double)(gf.m - gc.m);
+        if constexpr (ADJ || TAN) {     // no pathwise Greeks for the barrier: the sums stay 0
+            acc[1] += (double)(gf.s - gc.s);
+            acc[2] += (double)(gf.v - gc.v);
+            acc[3] += (double)(gf.m - gc.m);
+        }
     }
 }
 
-// Levels with NF < 16 dates (NF = 1, 2, 4, 8): the whole bridge in registers,
-// unrolled, with no skeleton in memory.
+// Levels of NF <= 16 dates (NF = 1, 2, 4, 8, 16): the whole bridge in
+// registers, unrolled, with no skeleton in memory; the same arithmetic, in the
+// same order, as one block of the long path.
 // dm[d]: XOR of the current tile's high Sobol words relative to the words in
 // the LDS records (the work item's first tile), per dimension.
 template <int MK, int PK, int GM, int NF>
 __device__ __forceinline__ void book_path_short(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
                                                 const float* __restrict__ vt, const SobolLane& sl,
-                                                const float4* itab, const uint32_t (&dm)[8], bool valid,
+                                                const float4* itab, const uint32_t (&dm)[NF], bool valid,
                                                 double (&acc)[BOOK_NACC]) {
     constexpr bool TAN = LIP_TAN(PK, GM);
     constexpr int NC = NF / 2;
@@ -397,16 +405,35 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
 // GM: 0 price only, 1 adjoint Greeks (reverse sweep over 16-step blocks with
 // checkpoints, increments regenerated from the Sobol index), 2 forward
 // tangents (carried along the forward pass; no second pass).
-template <int MK, int PK, int GM>
-__device__ __forceinline__ void book_path_short_any(const LegParams& P, const BookGeom& g,
-                                                    const float* __restrict__ sdt, const float* __restrict__ vt,
-                                                    const SobolLane& sl, const float4* itab, const uint32_t (&dm)[8],
-                                                    bool valid, double (&acc)[BOOK_NACC]) {
-    switch (g.l) {
-    case 0: book_path_short<MK, PK, GM, 1>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
-    case 1: book_path_short<MK, PK, GM, 2>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
-    case 2: book_path_short<MK, PK, GM, 4>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
-    default: book_path_short<MK, PK, GM, 8>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+// The tiles [t0, t1) of a work item at a level of NF <= 16 dates.  The LDS
+// records hold the high Sobol words of the item's first tile; each lane keeps
+// their change since then per dimension in registers (dm), from the
+// direction numbers of bits 7..31 in LDS (hl[d][25]): tile n changes
+// gray(256 n) in bits 7 and 8 + ctz(n) only.  No barrier between tiles.
+template <int MK, int GM, int NF>
+__device__ __forceinline__ void book_tiles_short(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
+                                                 const float* __restrict__ vt, const SobolLane& sl,
+                                                 const float4* itab, const uint32_t* hl, int kind, int64_t tfirst,
+                                                 int t0, int t1, int64_t first, int64_t count, int tid,
+                                                 double (&acc)[BOOK_NACC]) {
+    uint32_t dm[NF];
+#pragma unroll
+    for (int d = 0; d < NF; d++) dm[d] = 0u;
+    for (int t = t0; t < t1; t++) {
+        const uint32_t n = (uint32_t)(tfirst + t);
+        if (t > t0) {
+            const int c = __builtin_ctz(n);
+#pragma unroll
+            for (int d = 0; d < NF; d++) dm[d] ^= hl[d * 25] ^ hl[d * 25 + 1 + c];
+        }
+        const int64_t idx = ((int64_t)n << 8) + tid;
+        const bool valid = idx >= first && idx < first + count;
+        switch (kind) {
+        case PK_EUROPEAN: book_path_short<MK, PK_EUROPEAN, GM, NF>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+        case PK_ASIAN: book_path_short<MK, PK_ASIAN, GM, NF>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+        case PK_LOOKBACK: book_path_short<MK, PK_LOOKBACK, GM, NF>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+        default: book_path_short<MK, PK_BARRIER, GM, NF>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
+        }
     }
 }
 
@@ -494,11 +521,16 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         // the set bits of gray(base) (bits 7..31; base is a multiple of 256
         // below 2^32) for the item's first tile, loads in parallel.  The
         // previous item ended with barriers, so the records are free.
+        // Levels of at most 16 dates (dimensions) also keep the direction
+        // numbers of bits 7..31 in LDS after the 16 records (hl[d][25]).
+        uint32_t* hl = rec + BOOK_SHORT_D * 20;
         if (lo != last_e) {
             for (int i = tid; i < D * 16; i += 256) {
                 const int d = i >> 4, k = i & 15;
                 rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
             }
+            if (E.l <= BOOK_SHORT_L)
+                for (int i = tid; i < D * 25; i += 256) hl[i] = grec[(int64_t)(i / 25) * SDIR_WORDS + 39 + i % 25];
             last_e = lo;
         }
         {
@@ -514,30 +546,13 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         __syncthreads();
         // Each further tile n changes gray(256 n) in bits 7 and 8 + ctz(n) only:
         // two direction numbers per dimension.
-        if (E.l < 4) {
-            // at most 8 dimensions: each lane keeps the change since the first
-            // tile in registers, so the tiles need no barrier
-            uint32_t dm[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-            for (int t = t0; t < t1; t++) {
-                const uint32_t n = (uint32_t)(tfirst + t);
-                if (t > t0) {
-                    const int c = __builtin_ctz(n);
-#pragma unroll
-                    for (int d = 0; d < 8; d++) {
-                        if (d < D) {
-                            const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-                            dm[d] ^= gr[32 + 7] ^ gr[32 + 8 + c];
-                        }
-                    }
-                }
-                const int64_t idx = ((int64_t)n << 8) + tid;
-                const bool valid = idx >= E.first && idx < E.first + E.count;
-                switch (kind) {
-                case PK_EUROPEAN: book_path_short_any<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
-                case PK_ASIAN: book_path_short_any<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
-                case PK_LOOKBACK: book_path_short_any<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
-                default: book_path_short_any<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, itab, dm, valid, acc); break;
-                }
+        if (E.l <= BOOK_SHORT_L) {
+            switch (E.l) {
+            case 0: book_tiles_short<MK, GM, 1>(P, g, sdt, vt, sl, itab, hl, kind, tfirst, t0, t1, E.first, E.count, tid, acc); break;
+            case 1: book_tiles_short<MK, GM, 2>(P, g, sdt, vt, sl, itab, hl, kind, tfirst, t0, t1, E.first, E.count, tid, acc); break;
+            case 2: book_tiles_short<MK, GM, 4>(P, g, sdt, vt, sl, itab, hl, kind, tfirst, t0, t1, E.first, E.count, tid, acc); break;
+            case 3: book_tiles_short<MK, GM, 8>(P, g, sdt, vt, sl, itab, hl, kind, tfirst, t0, t1, E.first, E.count, tid, acc); break;
+            default: book_tiles_short<MK, GM, 16>(P, g, sdt, vt, sl, itab, hl, kind, tfirst, t0, t1, E.first, E.count, tid, acc); break;
             }
         } else {
             for (int t = t0; t < t1; t++) {
