@@ -33,10 +33,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "financial-derivatives-montecarlo_amd"))
 
 # Algorithmic FP32 flops per fine path-step (counts fixed before tuning, FMA = 2):
-# book, adjoint Greeks: forward 36 (local vol ln + square + FMA + Euler 9, ICDF 20,
-# bridge 6, payoff 1) + reverse 44 (forward recomputed 36 + block adjoint 8);
-# the coupled coarse leg (half a step more) is not counted.
-BOOK_FLOPS_PER_PATH_STEP = 80
+# book forward 36 (local vol ln + square + FMA + Euler 9, ICDF 20, bridge 6,
+# payoff 1) on every path; reverse 44 (forward recomputed 36 + block adjoint 8)
+# only on the paths of contracts with pathwise Greeks (the barrier's are NaN and
+# it runs no reverse pass); the coupled coarse leg (half a step more) is not counted.
+BOOK_FWD_FLOPS = 36
+BOOK_REV_FLOPS = 44
 C2_FLOPS_PER_PATH_STEP = 31     # C2 forward: Euler 3 + BB 6 + Asian 1 + geo 1 + ICDF 20
 VALU_PEAK_TFLOPS = 157.3        # MI355X FP32 vector peak (packed)
 MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense FP16/BF16 matrix peak
@@ -191,7 +193,16 @@ def c4_heston_mlmc(eng):
             "levels": r.levels, "path_steps": r.path_steps, "path_steps_per_s": r.path_steps / sorted(ts)[1]}
 
 
-def book_roofline(eng, book):
+def book_flops(contracts, res):
+    """Algorithmic flops of one run: forward flops on every fine path-step,
+    reverse flops on those of contracts with pathwise Greeks."""
+    import fdmc
+
+    return sum(r.path_steps * (BOOK_FWD_FLOPS + (BOOK_REV_FLOPS if k.kind != fdmc.PAYOFF_BARRIER else 0))
+               for k, r in zip(contracts, res))
+
+
+def book_roofline(eng, book, contracts):
     """One event-timed run of the book: HIP events around every launch of the
     persistent kernel on the engine stream."""
     eng.timing_enable(True)
@@ -200,11 +211,13 @@ def book_roofline(eng, book):
     launches, kms = eng.timing_read(reset=True)
     eng.timing_enable(False)
     k_avg_ms = kms / max(1, launches)
-    achieved = st.path_steps * BOOK_FLOPS_PER_PATH_STEP / (kms * 1e-3) / 1e12
+    flops = book_flops(contracts, res)
+    achieved = flops / (kms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / VALU_PEAK_TFLOPS, "traffic": BOOK_TRAFFIC_BYTES_PER_LAUNCH, "kernel": BOOK_KERNEL,
             "kernel_avg_ms": k_avg_ms, "kernel_launches": launches, "kernel_total_ms": kms,
-            "path_steps": st.path_steps, "flops_per_path_step": BOOK_FLOPS_PER_PATH_STEP,
+            "path_steps": st.path_steps, "flops": flops, "flops_per_path_step": flops / st.path_steps,
+            "flops_forward_reverse": [BOOK_FWD_FLOPS, BOOK_REV_FLOPS],
             "variant": st.variant, "wg_per_cu": st.wg_per_cu}
 
 
@@ -245,7 +258,7 @@ def main():
     if args.roofline_only:
         book = eng.book_create(model, contracts, fdmc.BookConfig(R=16, greeks=1, variant=args.variant,
                                                                  wg_per_cu=args.wpc))
-        rl = book_roofline(eng, book)
+        rl = book_roofline(eng, book, contracts)
         book.free()
         eng.close()
         out = {"roofline": rl}
@@ -282,7 +295,7 @@ def main():
     value = path_steps * args.steps / wall
     log("batched %.3f s per run, %.4g path-steps" % (ms_per_step / 1e3, path_steps))
     log("roofline run")
-    rl = book_roofline(eng, book)
+    rl = book_roofline(eng, book, contracts)
     log("plain run")
     t1 = time.perf_counter()
     pres, pst = book.run(EPS, fdmc.BOOK_PLAIN)

@@ -2,7 +2,7 @@
 //
 //  * splitmix32 / Hong-Hickernell row products (methods_cuda.cu:753-795)
 //  * inverse normal CDF: own least-squares fit (gen/fit_icdf.py) of the
-//    Giles-form ratio, evaluated with the hardware log2 / sqrt
+//    Giles-form ratio as a table of piecewise cubics read from LDS
 //  * Philox4x32-10 (Salmon et al., SC'11) + Box-Muller for pseudo-random paths
 //  * Euler steppers and payoff accumulators (methods_cuda.cu:271-352)
 //  * wave64 FP64 reductions
@@ -29,35 +29,12 @@ __device__ __forceinline__ float fast_ln(float x) { return __builtin_amdgcn_logf
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Inverse standard normal CDF of u = (s + 1/2) 2^-32, never 0 or 1.  The value
-// is folded to the lower half (q = min(u, 1-u)) from the integer so that both
-// tails keep full relative precision; w = -ln(4 q (1-q)).
-__device__ __forceinline__ float icdf_u32(uint32_t s) {
-    const bool upper = (s & 0x80000000u) != 0u;
-    const uint32_t f = upper ? ~s : s;
-    const float q = fmaf((float)f, 0x1p-32f, 0x1p-33f);
-    const float t = fmaf(-q, q, q);                       // q (1 - q)
-    const float u = fast_log2(t);                         // w = -ln2 (u + 2)
-    float p;
-    if (u > FDMC_ICDF_U_SPLIT) {                          // w < FDMC_ICDF_W_SPLIT
-        constexpr float c[] = FDMC_ICDF_CENTRAL_COEFFS_U; // same polynomial, in u - U0
-        const float x = u - FDMC_ICDF_U0;
-        p = c[0];
-#pragma unroll
-        for (int i = 1; i <= FDMC_ICDF_DEG_CENTRAL; i++) p = fmaf(p, x, c[i]);
-    } else {
-        constexpr float c[] = FDMC_ICDF_TAIL_COEFFS;
-        const float w = -(u + 2.0f) * 0.69314718055994531f;
-        const float x = fast_sqrt(w) - FDMC_ICDF_C_TAIL;
-        p = c[0];
-#pragma unroll
-        for (int i = 1; i <= FDMC_ICDF_DEG_TAIL; i++) p = fmaf(p, x, c[i]);
-    }
-    const float y = fmaf(-2.0f, q, 1.0f) * p;               // (1 - 2q) f(w)
-    return upper ? y : -y;
-}
-
-// The same f(w) as a table of cubics, 16 segments per binade of t = q (1-q):
+// Inverse standard normal CDF of u = (s + 1/2) 2^-32, never 0 or 1, in the
+// form of M. Giles ("Approximating the erfinv function", GPU Computing Gems
+// Jade, 2011): with q = min(u, 1-u) folded to the lower half from the integer
+// (both tails keep full relative precision) and w = -ln(4 q (1-q)),
+// Phi^-1(u) = +-(1 - 2q) f(w) with f smooth; the coefficients are fitted in
+// gen/fit_icdf.py.  f is a table of cubics, 16 segments per binade of t = q (1-q):
 // the row is bits 19..27 of t's FP32 encoding (low 5 exponent bits, top 4
 // mantissa bits; t in [2^-33, 1/4] spans 32 binades), the variable the
 // remaining 19 mantissa bits as an integer.  No logarithm and no branch: a

@@ -328,7 +328,7 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     double* d_sums = (double*)c->sums.ensure((size_t)s.R * nacc * sizeof(double));
 
     const size_t lds = s.greeks ? greeks_lds_bytes(nz, D, a.G, s.n_steps) : path_lds_bytes(nz, D, a.G);
-    if (lds > 160 * 1024) fail(FDMC_E_UNSUPPORTED, "LDS footprint of this pass exceeds 160 KiB");
+    if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "LDS footprint of this pass exceeds 160 KiB");
     if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     void* args[] = {(void*)&a};
     const bool time_path = c->timing && nz != NZ_ROWS;   // PCA passes time their GEMM
@@ -446,6 +446,7 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
     double* part = (double*)c->partials.ensure((size_t)s.R * nwg * nacc * sizeof(double));
     double* d_sums = (double*)c->sums.ensure((size_t)s.R * nacc * sizeof(double));
     const size_t lds = basket_lds_bytes(src, D, na);
+    if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "basket LDS footprint too large");
     if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     // replicas in groups so that the PCA buffers stay below ~8 GiB
     int group = s.R;
@@ -722,7 +723,7 @@ void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cac
         a.queue = d_queue;
         a.partials = d_part;
         const size_t lds = levels_lds_bytes(nz, max_D, max_G);
-        if (lds > 160 * 1024) fail(FDMC_E_UNSUPPORTED, "multilevel LDS footprint too large");
+        if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "multilevel LDS footprint too large");
         if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const int grid = (int)std::min<int64_t>(total, (int64_t)c->n_cu * 8);
         void* args[] = {(void*)&a, (void*)&max_D, (void*)&max_G};

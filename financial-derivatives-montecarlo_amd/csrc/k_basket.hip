@@ -22,6 +22,13 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = blockIdx.y;
     const int n = b.n, D = b.D;
+    // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
+    constexpr bool ITAB = SRC == SRC_SOBOL;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) {
+        icdf_tab_load(itab, threadIdx.x, 256);
+        __syncthreads();
+    }
     const bool lds_mode = (SRC == SRC_SOBOL) && D <= SOBOL_LDS_MAX_D;
     uint32_t* rec = (uint32_t*)smem;
     double* red = (double*)(smem + ((size_t)(lds_mode ? D * 20 : 0) * 4 + 15) / 16 * 16);
@@ -81,7 +88,7 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
                 float z[NA];
                 if constexpr (SRC == SRC_SOBOL) {
 #pragma unroll
-                    for (int a = 0; a < NA; a++) z[a] = (a < n) ? icdf_u32(sobol_coord(sl, k * n + a)) * b.sqrt_h : 0.0f;
+                    for (int a = 0; a < NA; a++) z[a] = (a < n) ? icdf_u32_tab(sobol_coord(sl, k * n + a), itab) * b.sqrt_h : 0.0f;
                 } else {
 #pragma unroll
                     for (int a0 = 0; a0 < NA; a0 += 4) {

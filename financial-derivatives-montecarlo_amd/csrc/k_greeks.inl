@@ -58,6 +58,13 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = blockIdx.y;
     const int D = a.D, n = a.n_steps;
+    // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
+    constexpr bool ITAB = NZ != NZ_PRNG;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) {
+        icdf_tab_load(itab, threadIdx.x, 256);
+        __syncthreads();
+    }
     const int G = (NZ == NZ_SOBOL_BB) ? a.G : (n + B - 1) / B;
     const bool lds_mode = (NZ != NZ_PRNG) && D <= SOBOL_LDS_MAX_D;
     uint32_t* rec = (uint32_t*)smem;
@@ -124,14 +131,14 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
         if constexpr (NZ == NZ_SOBOL_BB) {
             const float4* bw = (const float4*)a.bbw;
             sk[0] = 0.0f;
-            sk[G * 256] = bw[0].z * icdf_u32(sobol_coord(sl, 0));
+            sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
             for (int j = 1; j < G; j++) {
                 const int lev = 31 - __builtin_clz(j);
                 const int q = j - (1 << lev);
                 const int st = G >> lev;
                 const int L = q * st, R = L + st, M = L + (st >> 1);
                 const float4 c = bw[j];
-                sk[M * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32(sobol_coord(sl, j));
+                sk[M * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
             }
         }
         // increments of block i (B steps; steps beyond n are zero)
@@ -152,7 +159,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                             const int s = B >> l;
                             const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
                             const float4 c = bw[j0 + q];
-                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32(sobol_coord(sl, j0 + q));
+                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32_tab(sobol_coord(sl, j0 + q), itab);
                         }
                     }
 #pragma unroll
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
 #pragma unroll
                 for (int k = 0; k < B; k++) {
                     const int g = i * B + k;
-                    dw[k] = (g < n) ? icdf_u32(sobol_coord(sl, g)) * a.sqrt_h : 0.0f;
+                    dw[k] = (g < n) ? icdf_u32_tab(sobol_coord(sl, g), itab) * a.sqrt_h : 0.0f;
                 }
             } else {
 #pragma unroll

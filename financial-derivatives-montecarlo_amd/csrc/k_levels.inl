@@ -67,6 +67,13 @@ __global__ __launch_bounds__(256) void k_levels(LevelsLaunch a, int max_D, int m
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63;
     const bool lds_ok = (NZ != NZ_PRNG) && max_D <= SOBOL_LDS_MAX_D;
+    // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
+    constexpr bool ITAB = NZ != NZ_PRNG;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) {
+        icdf_tab_load(itab, threadIdx.x, 256);
+        __syncthreads();
+    }
     uint32_t* rec = (uint32_t*)smem;
     const int rec_words = lds_ok ? max_D * 20 : 0;
     float* skel = (float*)(smem + (size_t)rec_words * 4);
@@ -164,23 +171,23 @@ __global__ __launch_bounds__(256) void k_levels(LevelsLaunch a, int max_D, int m
         } else if constexpr (NZ == NZ_SOBOL_RAW) {
             if constexpr (MK == MK_HESTON) {
                 for (int k = 0; k < nfine; k++)
-                    cp.fineH(P, E, icdf_u32(sobol_coord(sl, 2 * k)), icdf_u32(sobol_coord(sl, 2 * k + 1)), M);
+                    cp.fineH(P, E, icdf_u32_tab(sobol_coord(sl, 2 * k), itab), icdf_u32_tab(sobol_coord(sl, 2 * k + 1), itab), M);
             } else {
-                for (int k = 0; k < nfine; k++) cp.fine1(P, E, icdf_u32(sobol_coord(sl, k)) * E.sqrt_hf, k, M);
+                for (int k = 0; k < nfine; k++) cp.fine1(P, E, icdf_u32_tab(sobol_coord(sl, k), itab) * E.sqrt_hf, k, M);
             }
         } else {  // Brownian bridge over n_fine steps (one-factor models)
             const float4* bw = (const float4*)E.bbw;
             const int G = E.G;
             float* sk = skel + tid;
             sk[0] = 0.0f;
-            sk[G * 256] = bw[0].z * icdf_u32(sobol_coord(sl, 0));
+            sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
             for (int j = 1; j < G; j++) {
                 const int lev = 31 - __builtin_clz(j);
                 const int q = j - (1 << lev);
                 const int st = G >> lev;
                 const int L = q * st, R = L + st, Mi = L + (st >> 1);
                 const float4 c = bw[j];
-                sk[Mi * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32(sobol_coord(sl, j));
+                sk[Mi * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
             }
             if (LB == 0 || nfine < 16) {
                 float left = 0.0f;
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(256) void k_levels(LevelsLaunch a, int max_D, int m
                             const int s = B >> l;
                             const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
                             const float4 c = bw[j0 + q];
-                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32(sobol_coord(sl, j0 + q));
+                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32_tab(sobol_coord(sl, j0 + q), itab);
                         }
                     }
 #pragma unroll

@@ -101,6 +101,13 @@ __global__ __launch_bounds__(256) void k_path(PathLaunch a) {
     const int r = blockIdx.y;
     const int D = a.D;
     constexpr bool SOBOL = (NZ == NZ_SOBOL_RAW || NZ == NZ_SOBOL_BB);
+    // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
+    constexpr bool ITAB = SOBOL;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) {
+        icdf_tab_load(itab, threadIdx.x, 256);
+        __syncthreads();
+    }
     const bool lds_mode = SOBOL && D <= SOBOL_LDS_MAX_D;
     uint32_t* rec = (uint32_t*)smem;
     const int rec_words = lds_mode ? D * 20 : 0;
@@ -192,27 +199,27 @@ __global__ __launch_bounds__(256) void k_path(PathLaunch a) {
         } else if constexpr (NZ == NZ_SOBOL_RAW) {
             if constexpr (MK == MK_HESTON) {
                 for (int k = 0; k < a.n_steps; k++) {
-                    const float d1 = icdf_u32(sobol_coord(sl, 2 * k)) * a.sqrt_h;
-                    const float d2 = icdf_u32(sobol_coord(sl, 2 * k + 1)) * a.sqrt_h;
+                    const float d1 = icdf_u32_tab(sobol_coord(sl, 2 * k), itab) * a.sqrt_h;
+                    const float d2 = icdf_u32_tab(sobol_coord(sl, 2 * k + 1), itab) * a.sqrt_h;
                     ps.stepH(a, d1, a.leg[0].L10 * d1 + a.leg[0].L11 * d2);
                 }
             } else {
                 for (int k = 0; k < a.n_steps; k++)
-                    ps.step(a, icdf_u32(sobol_coord(sl, k)) * a.sqrt_h, k);
+                    ps.step(a, icdf_u32_tab(sobol_coord(sl, k), itab) * a.sqrt_h, k);
             }
         } else {  // NZ_SOBOL_BB (one-factor models)
             const float4* bw = (const float4*)a.bbw;
             const int G = a.G;
             float* sk = skel + tid;
             sk[0] = 0.0f;
-            sk[G * 256] = bw[0].z * icdf_u32(sobol_coord(sl, 0));
+            sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
             for (int j = 1; j < G; j++) {
                 const int lev = 31 - __builtin_clz(j);
                 const int q = j - (1 << lev);
                 const int st = G >> lev;
                 const int L = q * st, R = L + st, M = L + (st >> 1);
                 const float4 c = bw[j];
-                sk[M * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32(sobol_coord(sl, j));
+                sk[M * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
             }
             if constexpr (LB == 0) {
                 float left = 0.0f;
@@ -236,7 +243,7 @@ __global__ __launch_bounds__(256) void k_path(PathLaunch a) {
                             const int s = B >> l;
                             const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
                             const float4 c = bw[j0 + q];
-                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32(sobol_coord(sl, j0 + q));
+                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32_tab(sobol_coord(sl, j0 + q), itab);
                         }
                     }
 #pragma unroll

@@ -39,6 +39,13 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = blockIdx.y;
     const bool lds_mode = D <= SOBOL_LDS_MAX_D;
+    // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
+    constexpr bool ITAB = true;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) {
+        icdf_tab_load(itab, threadIdx.x, 256);
+        __syncthreads();
+    }
     uint32_t* rec = (uint32_t*)smem;
     T* tile = (T*)(smem + (size_t)(lds_mode ? D * 20 : 0) * 4);   // [256][64 + 8]
     constexpr int TW = 72;
@@ -70,7 +77,7 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
     }
     for (int d0 = 0; d0 < ld; d0 += 64) {
         for (int j = 0; j < 64; j++) {
-            const float z = (d0 + j < D) ? icdf_u32(sobol_coord(sl, d0 + j)) : 0.0f;
+            const float z = (d0 + j < D) ? icdf_u32_tab(sobol_coord(sl, d0 + j), itab) : 0.0f;
             tile[tid * TW + j] = to_half<T>(z);
         }
         __syncthreads();
@@ -340,6 +347,7 @@ hipError_t launch_gen_z(const uint32_t* sdir, int D, int ld, int R, int64_t firs
     const int64_t tiles = ((first + count - 1) >> 8) - (first >> 8) + 1;
     if (tiles > 0x7FFFFFFF || R > 65535) return hipErrorInvalidValue;
     const size_t lds = gen_z_lds_bytes(D);
+    if (lds > LDS_DYNAMIC_MAX) return hipErrorInvalidValue;
     const void* fn = bf16 ? (const void*)&k_gen_z<__bf16> : (const void*)&k_gen_z<_Float16>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -80,18 +80,32 @@ template <bool NORMAL>
 __global__ __launch_bounds__(256) void k_sobol_gen(const uint32_t* __restrict__ sdir, int D, int64_t n,
                                                    uint32_t offset, void* __restrict__ out) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
+    constexpr bool ITAB = NORMAL;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) {
+        icdf_tab_load(itab, threadIdx.x, 256);
+        __syncthreads();
+    }
     if (gid >= (int64_t)D * n) return;
     const int d = (int)(gid / n);
     const int64_t p = gid - (int64_t)d * n;
     const uint32_t x = sobol_from_record(sdir + (int64_t)d * SDIR_WORDS, offset + (uint32_t)p);
-    if constexpr (NORMAL) ((float*)out)[gid] = icdf_u32(x);
+    if constexpr (NORMAL) ((float*)out)[gid] = icdf_u32_tab(x, itab);
     else ((uint32_t*)out)[gid] = x;
 }
 
 __global__ __launch_bounds__(256) void k_icdf(const uint32_t* __restrict__ in, int64_t n,
                                               float* __restrict__ out) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid < n) out[gid] = icdf_u32(in[gid]);
+    // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
+    constexpr bool ITAB = true;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) {
+        icdf_tab_load(itab, threadIdx.x, 256);
+        __syncthreads();
+    }
+    if (gid < n) out[gid] = icdf_u32_tab(in[gid], itab);
 }
 
 // Deterministic second-stage reduction: out[s][i] = sum_{w < nw} part[s][w][i],

@@ -103,6 +103,12 @@ struct BasketLaunch {
     double* partials;         // [R][nwg_x][2 + 2 n]
 };
 
+// Static LDS of the inverse-CDF table (icdf_u32_tab) in every kernel that
+// draws normals from Sobol points, and what that leaves of a CU's 160 KiB for
+// a launch's dynamic LDS.
+constexpr size_t ICDF_TAB_LDS_BYTES = (size_t)FDMC_ICDF_TAB_ROWS * 16;
+constexpr size_t LDS_DYNAMIC_MAX = 160 * 1024 - ICDF_TAB_LDS_BYTES;
+
 // Sobol direction records are staged in LDS up to this dimension; above it the
 // path kernels read them from global memory (L2-resident).
 constexpr int SOBOL_LDS_MAX_D = 1024;

@@ -1,10 +1,10 @@
-"""The FP32 inverse normal CDF's generated constants (csrc/icdf_coeffs.h), evaluated
-on the CPU the way csrc/device_math.hpp evaluates them: FP32 Horner steps,
-q = (f + 1/2) 2^-32 folded to the lower half, t = q (1 - q), and either
-u = log2(t) with the two-branch polynomial (icdf_u32) or the piecewise cubic read
-from the bits of t (icdf_u32_tab).  The device's log2 is the hardware's (1 ulp);
-here it is the correctly rounded one, so the bound checked here is a little
-tighter than the GPU test's 4e-7."""
+"""The FP32 inverse normal CDF's generated table (csrc/icdf_coeffs.h), evaluated
+on the CPU the way icdf_u32_tab_scaled_neg (csrc/device_math.hpp) evaluates it:
+q = |(float)(int32)s 2^-32 + 2^-33| (the fold to the lower half), t = q (1 - q),
+row = bits(t) >> 19 & 511, x = bits(t) & 0x7FFFF, FP32 Horner steps, the sign
+taken from s.  numpy rounds each multiply and add where the device fuses them,
+so the bound checked here is looser than the device's error and tighter than
+the GPU test's 4e-7."""
 import os
 import re
 
@@ -16,55 +16,27 @@ HDR = os.path.join(os.path.dirname(__file__), "..", "financial-derivatives-monte
 F32 = np.float32
 
 
-def _consts():
+def _table():
     h = open(HDR).read()
-
-    def arr(name):
-        return np.array([float(x.strip().rstrip("f")) for x in
-                         re.search(r"#define %s \{([^}]*)\}" % name, h).group(1).split(",")], np.float32)
-
-    def val(name):
-        return F32(float(re.search(r"#define %s ([^\s]+?)f?\n" % name, h).group(1)))
-
-    tab = np.array([float(x.strip().rstrip("f")) for x in
-                    re.search(r"#define FDMC_ICDF_TAB \{(.*?)\}", h, re.S).group(1).replace("\\", "").split(",")],
-                   np.float32).reshape(-1, 4)
-    return dict(tab=tab, cw=arr("FDMC_ICDF_CENTRAL_COEFFS"), cu=arr("FDMC_ICDF_CENTRAL_COEFFS_U"),
-                ct=arr("FDMC_ICDF_TAIL_COEFFS"), u0=val("FDMC_ICDF_U0"), us=val("FDMC_ICDF_U_SPLIT"),
-                cc=val("FDMC_ICDF_C_CENTRAL"), ctl=val("FDMC_ICDF_C_TAIL"), ws=val("FDMC_ICDF_W_SPLIT"))
+    body = re.search(r"#define FDMC_ICDF_TAB \{(.*?)\}", h, re.S).group(1).replace("\\", "")
+    tab = np.array([float(x.strip().rstrip("f")) for x in body.split(",")], np.float32).reshape(-1, 4)
+    assert tab.shape[0] == int(re.search(r"#define FDMC_ICDF_TAB_ROWS (\d+)", h).group(1))
+    return tab
 
 
-def _horner(c, x):
-    p = np.full_like(x, c[0])
-    for k in c[1:]:
-        p = (p * x + k).astype(np.float32)
-    return p
-
-
-def _icdf(s, k):
-    upper = (s & 0x80000000) != 0
-    f = np.where(upper, ~s, s).astype(np.uint32)
-    q = (f.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32)
-    t = (-q * q + q).astype(np.float32)
-    u = np.log2(t.astype(np.float64)).astype(np.float32)
-    pc = _horner(k["cu"], (u - k["u0"]).astype(np.float32))
-    w = (-(u + F32(2.0)) * F32(np.log(2.0))).astype(np.float32)
-    pt = _horner(k["ct"], (np.sqrt(w) - k["ctl"]).astype(np.float32))
-    y = ((F32(1.0) - F32(2.0) * q) * np.where(u > k["us"], pc, pt)).astype(np.float32)
-    return np.where(upper, y, -y).astype(np.float64), q.astype(np.float64), upper
-
-
-def _icdf_tab(s, k):
-    upper = (s & 0x80000000) != 0
-    f = np.where(upper, ~s, s).astype(np.uint32)
-    q = (f.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32)
+def _icdf(s, tab):
+    i = s.view(np.int32)
+    q = np.abs((i.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32))
     t = (-q * q + q).astype(np.float32)
     b = t.view(np.uint32)
-    c = k["tab"][(b >> 19) & 0x1FF]
+    c = tab[(b >> 19) & 0x1FF]
     x = (b & 0x7FFFF).astype(np.float32)
-    p = _horner(c[:, ::-1].T, x)
-    y = ((F32(1.0) - F32(2.0) * q) * p).astype(np.float32)
-    return np.where(upper, y, -y).astype(np.float64), q.astype(np.float64), upper
+    p = c[:, 3]
+    for k in (2, 1, 0):
+        p = (p * x + c[:, k]).astype(np.float32)
+    y = ((q * F32(-2.0) + F32(1.0)) * p).astype(np.float32)
+    neg = (y.view(np.uint32) | (s & np.uint32(0x80000000))).view(np.float32)   # -Phi^-1
+    return -neg.astype(np.float64)
 
 
 def _samples():
@@ -74,36 +46,37 @@ def _samples():
                            np.logspace(0, 31.9, 20000, base=2.0).astype(np.uint64)]).astype(np.uint32)
 
 
-def test_fp32_evaluation_error_table_form():
-    k = _consts()
-    s = _samples()
-    y, q, upper = _icdf_tab(s, k)
-    ref = np.where(upper, -ndtri(q), ndtri(q))
-    err = np.abs(y - ref) / np.maximum(1.0, np.abs(ref))
-    assert err.max() < 2.5e-7, (err.max(), s[np.argmax(err)])
-    # every row the kernels can read is fitted (no default row is reached but t = 1/4)
-    b = (-(q * q) + q).astype(np.float32).view(np.uint32)
-    assert set(np.unique((b >> 23) & 0xFF)) <= set(range(94, 126))
-
-
 def test_fp32_evaluation_error():
-    k = _consts()
-    s = np.concatenate([np.random.default_rng(7).integers(0, 2 ** 32, 1_000_000, dtype=np.uint64),
-                        np.arange(0, 4096, dtype=np.uint64), 2 ** 32 - 1 - np.arange(0, 4096, dtype=np.uint64),
-                        np.logspace(0, 31.9, 20000, base=2.0).astype(np.uint64)]).astype(np.uint32)
-    y, q, upper = _icdf(s, k)
-    ref = np.where(upper, -ndtri(q), ndtri(q))
-    err = np.abs(y - ref) / np.maximum(1.0, np.abs(ref))
+    tab = _table()
+    s = _samples()
+    got = _icdf(s, tab)
+    ref = ndtri((s.astype(np.float64) + 0.5) / 2.0 ** 32)
+    err = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
     assert err.max() < 2.5e-7, (err.max(), s[np.argmax(err)])
 
 
-def test_u_form_is_the_w_polynomial():
-    # sum_k d_k (u - U0)^k == sum_k c_k (w - C)^k with w = -ln2 (u + 2), in FP64
-    k = _consts()
-    u = np.linspace(float(k["us"]), -2.0, 2001)
-    w = -np.log(2.0) * (u + 2.0)
-    a = np.polyval(k["cu"].astype(np.float64), u - float(k["u0"]))
-    b = np.polyval(k["cw"].astype(np.float64), w - float(k["cc"]))
-    np.testing.assert_allclose(a, b, rtol=2e-7)
-    # the split is the same point in both variables
-    assert abs(-np.log(2.0) * (float(k["us"]) + 2.0) - float(k["ws"])) < 1e-5
+def test_rows_reached_are_fitted():
+    # t in [2^-33, 1/4]: biased exponents 94..125; only t = 1/4 reads an
+    # exponent-125 row, and that row is the constant sqrt(pi/2)
+    tab = _table()
+    s = _samples()
+    i = s.view(np.int32)
+    q = np.abs((i.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32))
+    b = (-q * q + q).astype(np.float32).view(np.uint32)
+    e = (b >> 23) & 0xFF
+    assert e.min() >= 94 and e.max() <= 125
+    assert np.all(b[e == 125] == np.float32(0.25).view(np.uint32))
+    row125 = tab[((np.uint32(125) << 4) & 0x1FF)]
+    assert abs(row125[0] - np.sqrt(np.pi / 2)) < 1e-7 and np.all(row125[1:] == 0)
+
+
+def test_antisymmetry_to_rounding():
+    # x(~s) = -x(s) up to one FP32 rounding of (float)(int32)s, which can
+    # differ for s and ~s once |(int32)s| >= 2^24; within 2^24 of 0 or 2^32
+    # (the tails) the conversion is exact and so is the antisymmetry
+    tab = _table()
+    s = _samples()
+    a, b = _icdf(s, tab), _icdf(~s, tab)
+    assert np.all(np.abs(a + b) <= 4e-7 * np.maximum(1.0, np.abs(a)))
+    small = (s < 2 ** 24) | (s >= 2 ** 32 - 2 ** 24)
+    assert np.all(a[small] == -b[small])
