@@ -439,6 +439,16 @@ __device__ __forceinline__ void book_tiles_short(const LegParams& P, const BookG
 
 // SKG = 0: bridge skeleton in LDS; 1: in the workgroup's global scratch (less
 // LDS per workgroup, more workgroups per CU).
+// A workgroup-uniform value read from memory, moved into a scalar register.
+__device__ __forceinline__ int sreg(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float sreg(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+__device__ __forceinline__ int64_t sreg(int64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double sreg(double x) { return __longlong_as_double(sreg((int64_t)__double_as_longlong(x))); }
+
 template <int MK, int SKG, int GM>
 __device__ __forceinline__ void book_body(const BookLaunch& a) {
     // static LDS (its size depends only on SKG): addresses fold into the
@@ -486,29 +496,44 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
             if (a.entries[mid].tile_begin <= item) lo = mid;
             else hi = mid - 1;
         }
-        const BookEntry E = a.entries[lo];
+        // the entry and contract are uniform over the workgroup, but the
+        // compiler cannot prove their tables read-only and loads them with
+        // vector loads: readfirstlane moves every field into scalar registers
+        // (fewer VGPRs live across the tile loops)
+        const BookEntry& Ev = a.entries[lo];
+        BookEntry E;
+        E.c = sreg(Ev.c);
+        E.l = sreg(Ev.l);
+        E.r = sreg(Ev.r);
+        E.tpi = sreg(Ev.tpi);
+        E.tile_begin = sreg(Ev.tile_begin);
+        E.ntiles = 0;
+        E.first = sreg(Ev.first);
+        E.count = sreg(Ev.count);
         const BookContract& C = a.contracts[E.c];
+        const double CT = sreg(C.T);
         BookGeom g;
         g.l = E.l;
         g.nf = 1 << E.l;
         g.nc = g.nf >> 1;
         g.G = g.nf >= 16 ? g.nf >> 4 : g.nf;
-        g.hf = (float)(C.T / (double)g.nf);
-        g.hc = g.nc > 0 ? (float)(C.T / (double)g.nc) : 0.0f;
+        g.hf = (float)(CT / (double)g.nf);
+        g.hc = g.nc > 0 ? (float)(CT / (double)g.nc) : 0.0f;
         g.muh_f = a.leg.mu * g.hf;
         g.muh_c = a.leg.mu * g.hc;
-        g.bgk_f = C.bgk[E.l];
-        g.bgk_c = E.l > 0 ? C.bgk[E.l - 1] : 1.0f;
-        g.T = C.Tf;
-        const int kind = C.kind;
+        g.bgk_f = sreg(C.bgk[E.l]);
+        g.bgk_c = E.l > 0 ? sreg(C.bgk[E.l - 1]) : 1.0f;
+        g.T = sreg(C.Tf);
+        const int kind = sreg(C.kind);
         LegParams P = a.leg;
-        P.K = C.K;
-        P.B = C.B;
-        P.discount = C.discount;
+        P.K = sreg(C.K);
+        P.B = sreg(C.B);
+        P.discount = sreg(C.discount);
         const int D = g.nf;
-        const uint32_t* grec = a.sdir[E.l] + ((size_t)C.slot[E.l] * a.R + E.r) * (size_t)D * SDIR_WORDS;
-        const float* sdt = a.bbtab + ((size_t)C.t_idx * BOOK_NL + E.l) * BOOK_NL;
-        const float* vt = a.voltab + (size_t)C.t_idx * BOOK_VT_STRIDE + (size_t)(D - 1);
+        const int slot = sreg(C.slot[E.l]), t_idx = sreg(C.t_idx);
+        const uint32_t* grec = a.sdir[E.l] + ((size_t)slot * a.R + E.r) * (size_t)D * SDIR_WORDS;
+        const float* sdt = a.bbtab + ((size_t)t_idx * BOOK_NL + E.l) * BOOK_NL;
+        const float* vt = a.voltab + (size_t)t_idx * BOOK_VT_STRIDE + (size_t)(D - 1);
         const int64_t tfirst = E.first >> 8;
         const int nt_all = (int)(((E.first + E.count - 1) >> 8) - tfirst + 1);
         const int t0 = (item - E.tile_begin) * E.tpi;
