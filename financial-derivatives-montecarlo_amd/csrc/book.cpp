@@ -36,13 +36,13 @@ struct fdmc_book {
     int* h_active = nullptr;          // pinned
     // launch shape (autotuned on the first run)
     bool tuned = false;
-    int skg = 0, occ = 0, wg_per_cu = 2, items_target = 512, max_grid = 0;
+    int occ = 0, wg_per_cu = 2, items_target = 512, max_grid = 0;
     double tune_s = 0.0;
     // captured passes of the batched mode
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     double graph_eps = -1.0;
-    int graph_skg = -1, graph_occ = -1, graph_grid = -1;
+    int graph_occ = -1, graph_grid = -1;
 
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -137,8 +137,8 @@ int occupancy(const void* fn) {
 
 // Launch-shape autotuner: times one sweep of every entry of the book at a fixed
 // size (512 points per (contract, level, replica)) for each candidate
-// (skeleton in LDS or in global scratch) x (default or three-waves-per-SIMD
-// compilation) x (persistent workgroups per CU) and keeps the fastest.  Only
+// (default or three-waves-per-SIMD compilation) x (persistent workgroups per
+// CU) and keeps the fastest.  Only
 // the launch shape is tuned: the split of entries into work items
 // (items_target) is fixed, so results do not depend on the choice.
 void autotune(fdmc_book* b) {
@@ -154,30 +154,28 @@ void autotune(fdmc_book* b) {
     HIPC(hipEventCreate(&ev0));
     HIPC(hipEventCreate(&ev1));
     float best = 1e30f;
-    int best_skg = 0, best_occ = 0, best_wpc = 1;
+    int best_occ = 0, best_wpc = 1;
     try {
-        for (int occ = 0; occ < 2; occ++)
-            for (int skg = 0; skg < 2; skg++) {
-                const void* fn = book_kernel(b->mk, skg, b->greeks, occ);
-                const int nb = std::min(occupancy(fn), 4);
-                for (int wpc = 1; wpc <= nb; wpc++) {
-                    const int grid = std::min(c->n_cu * wpc, b->max_grid);
-                    BookLaunch a = make_launch(b, false, 0, 1.0);
-                    HIPC(launch_book_sweep(fn, skg, a, grid, c->stream));   // warm-up
-                    HIPC(hipEventRecord(ev0, c->stream));
-                    HIPC(launch_book_sweep(fn, skg, a, grid, c->stream));
-                    HIPC(hipEventRecord(ev1, c->stream));
-                    HIPC(hipEventSynchronize(ev1));
-                    float ms = 0.0f;
-                    HIPC(hipEventElapsedTime(&ms, ev0, ev1));
-                    if (ms < best) {
-                        best = ms;
-                        best_skg = skg;
-                        best_occ = occ;
-                        best_wpc = wpc;
-                    }
+        for (int occ = 0; occ < 2; occ++) {
+            const void* fn = book_kernel(b->mk, b->greeks, occ);
+            const int nb = std::min(occupancy(fn), 4);
+            for (int wpc = 1; wpc <= nb; wpc++) {
+                const int grid = std::min(c->n_cu * wpc, b->max_grid);
+                BookLaunch a = make_launch(b, false, 0, 1.0);
+                HIPC(launch_book_sweep(fn, a, grid, c->stream));   // warm-up
+                HIPC(hipEventRecord(ev0, c->stream));
+                HIPC(launch_book_sweep(fn, a, grid, c->stream));
+                HIPC(hipEventRecord(ev1, c->stream));
+                HIPC(hipEventSynchronize(ev1));
+                float ms = 0.0f;
+                HIPC(hipEventElapsedTime(&ms, ev0, ev1));
+                if (ms < best) {
+                    best = ms;
+                    best_occ = occ;
+                    best_wpc = wpc;
                 }
             }
+        }
     } catch (...) {
         (void)hipEventDestroy(ev0);
         (void)hipEventDestroy(ev1);
@@ -185,7 +183,6 @@ void autotune(fdmc_book* b) {
     }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
-    b->skg = best_skg;
     b->occ = best_occ;
     b->wg_per_cu = best_wpc;
     b->tuned = true;
@@ -225,13 +222,13 @@ void read_results(fdmc_book* b, fdmc_book_result* res, int64_t* total_steps) {
 void run_batched_timed(fdmc_book* b, double eps, fdmc_book_stats* st) {
     fdmc_context* c = b->ctx;
     const int grid = book_grid(b);
-    const void* fn = book_kernel(b->mk, b->skg, b->greeks, b->occ);
+    const void* fn = book_kernel(b->mk, b->greeks, b->occ);
     BookLaunch a = make_launch(b, false, 0, eps);
     const double t0 = now_s();
     HIPC(launch_book_reset(a, c->stream));
     int passes = 0;
     for (int it = 0; it <= b->max_iter; it++) {
-        HIPC(launch_book_pass(fn, b->skg, a, grid, c->stream, c->ev0, c->ev1));
+        HIPC(launch_book_pass(fn, a, grid, c->stream, c->ev0, c->ev1));
         HIPC(launch_book_status(a, c->stream));
         HIPC(hipMemcpyAsync(b->h_active, a.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
@@ -252,14 +249,14 @@ void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
     fdmc_context* c = b->ctx;
     if (c->timing) return run_batched_timed(b, eps, st);
     const int grid = book_grid(b);
-    const void* fn = book_kernel(b->mk, b->skg, b->greeks, b->occ);
+    const void* fn = book_kernel(b->mk, b->greeks, b->occ);
     BookLaunch a = make_launch(b, false, 0, eps);
-    if (!b->graph_exec || b->graph_eps != eps || b->graph_skg != b->skg || b->graph_occ != b->occ ||
+    if (!b->graph_exec || b->graph_eps != eps || b->graph_occ != b->occ ||
         b->graph_grid != grid) {
         b->drop_graph();
         HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
-        for (int i = 0; i < b->ipg && e == hipSuccess; i++) e = launch_book_pass(fn, b->skg, a, grid, c->stream);
+        for (int i = 0; i < b->ipg && e == hipSuccess; i++) e = launch_book_pass(fn, a, grid, c->stream);
         if (e == hipSuccess) e = launch_book_status(a, c->stream);
         hipGraph_t g = nullptr;
         const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
@@ -271,7 +268,6 @@ void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
         b->graph = g;
         HIPC(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
         b->graph_eps = eps;
-        b->graph_skg = b->skg;
         b->graph_occ = b->occ;
         b->graph_grid = grid;
     }
@@ -295,7 +291,7 @@ void run_batched(fdmc_book* b, double eps, fdmc_book_stats* st) {
 void run_plain(fdmc_book* b, double eps, fdmc_book_stats* st) {
     fdmc_context* c = b->ctx;
     const int grid = book_grid(b);
-    const void* fn = book_kernel(b->mk, b->skg, b->greeks, b->occ);
+    const void* fn = book_kernel(b->mk, b->greeks, b->occ);
     const double t0 = now_s();
     BookLaunch all = make_launch(b, false, 0, eps);
     HIPC(launch_book_reset(all, c->stream));
@@ -303,7 +299,7 @@ void run_plain(fdmc_book* b, double eps, fdmc_book_stats* st) {
     for (int k = 0; k < b->n; k++) {
         BookLaunch a = make_launch(b, true, k, eps);
         for (int it = 0; it <= b->max_iter; it++) {
-            HIPC(launch_book_pass(fn, b->skg, a, grid, c->stream, c->timing ? c->ev0 : nullptr,
+            HIPC(launch_book_pass(fn, a, grid, c->stream, c->timing ? c->ev0 : nullptr,
                                   c->timing ? c->ev1 : nullptr));
             HIPC(launch_book_status(a, c->stream));
             HIPC(hipMemcpyAsync(b->h_active, a.active, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -344,7 +340,7 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
         if (cfg->max_iter < 1) fail(FDMC_E_INVALID, "max_iter must be >= 1");
         if (cfg->iters_per_graph < 1 || cfg->iters_per_graph > 64)
             fail(FDMC_E_INVALID, "iters_per_graph must be in [1, 64]");
-        if (cfg->variant < -1 || cfg->variant > 3) fail(FDMC_E_INVALID, "variant must be -1 (autotune) or 0..3");
+        if (cfg->variant < -1 || cfg->variant > 1) fail(FDMC_E_INVALID, "variant must be -1 (autotune), 0 or 1");
         if (cfg->variant >= 0 && (cfg->wg_per_cu < 1 || cfg->wg_per_cu > 4))
             fail(FDMC_E_INVALID, "wg_per_cu must be in [1, 4] with a fixed variant");
         std::vector<double> Ts;
@@ -380,8 +376,7 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
             b->salt_base = cfg->salt_base;
             b->mk = model_kind_dev(*model);
             if (cfg->variant >= 0) {
-                b->skg = cfg->variant & 1;
-                b->occ = cfg->variant >> 1;
+                b->occ = cfg->variant;
                 b->wg_per_cu = cfg->wg_per_cu;
                 b->tuned = true;
             }
@@ -503,7 +498,7 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
             b->entry_sums.ensure((size_t)b->n_entries * BOOK_NACC * sizeof(double));
             b->counters.ensure(64);
             b->max_grid = ctx->n_cu * 4;
-            b->scratch.ensure((size_t)b->max_grid * book_scratch_floats(1) * sizeof(float));
+            b->scratch.ensure((size_t)b->max_grid * book_scratch_floats() * sizeof(float));
             HIPC(hipHostMalloc((void**)&b->h_active, sizeof(int) * 4, hipHostMallocDefault));
             BookLaunch a = make_launch(b, false, 0, 1.0);
             HIPC(launch_book_reset(a, s));
@@ -531,7 +526,7 @@ fdmc_status fdmc_book_run(fdmc_book* b, double eps, int32_t mode, fdmc_book_resu
         read_results(b, results, &st.path_steps);
         st.time_s += now_s() - t0;
         st.tune_s = b->tune_s;
-        st.variant = b->skg + 2 * b->occ;
+        st.variant = b->occ;
         st.items_target = b->items_target;
         st.wg_per_cu = b->wg_per_cu;
         if (stats) *stats = st;
@@ -558,7 +553,7 @@ fdmc_status fdmc_book_level_sums(fdmc_book* b, int32_t c, int32_t l, int64_t off
         }
         BookLaunch a = make_launch(b, true, c, 1.0);
         HIPC(hipMemcpyAsync(a.entries, e.data(), e.size() * sizeof(BookEntry), hipMemcpyHostToDevice, ctx->stream));
-        HIPC(launch_book_sweep(book_kernel(b->mk, b->skg, b->greeks, b->occ), b->skg, a, book_grid(b), ctx->stream));
+        HIPC(launch_book_sweep(book_kernel(b->mk, b->greeks, b->occ), a, book_grid(b), ctx->stream));
         std::vector<double> s((size_t)ne * BOOK_NACC);
         HIPC(hipMemcpyAsync(s.data(), a.entry_sums, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPC(hipStreamSynchronize(ctx->stream));

@@ -44,13 +44,13 @@ size_t basket_lds_bytes(int src, int D, int na);
 
 // k_book.hip
 // gm: 0 price, 1 adjoint Greeks, 2 tangent Greeks; occ 1: the three-waves-per-SIMD compilation
-const void* book_kernel(int mk, int skg, int gm, int occ);
-const void* book_kernel_o3(int mk, int skg, int gm);   // k_book_o3.hip
+const void* book_kernel(int mk, int gm, int occ);
+const void* book_kernel_o3(int mk, int gm);   // k_book_o3.hip
 hipError_t launch_book_reset(const BookLaunch& a, hipStream_t s);
 hipError_t launch_book_plan(const BookLaunch& a, hipStream_t s);
-hipError_t launch_book_sweep(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s,
+hipError_t launch_book_sweep(const void* fn, const BookLaunch& a, int grid, hipStream_t s,
                              hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-hipError_t launch_book_pass(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s,
+hipError_t launch_book_pass(const void* fn, const BookLaunch& a, int grid, hipStream_t s,
                             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_book_status(const BookLaunch& a, hipStream_t s);
 hipError_t launch_book_final(const BookLaunch& a, hipStream_t s);

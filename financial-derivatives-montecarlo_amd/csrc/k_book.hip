@@ -251,13 +251,13 @@ __global__ void k_book_final(BookLaunch a) {
 
 // ---- host launchers ---------------------------------------------------------------
 
-template <int MK, int SKG, int GM>
+template <int MK, int GM>
 struct BookDefault {
-    static const void* fn() { return (const void*)&k_book<MK, SKG, GM>; }
+    static const void* fn() { return (const void*)&k_book<MK, GM>; }
 };
 
-const void* book_kernel(int mk, int skg, int gm, int occ) {
-    return occ ? book_kernel_o3(mk, skg, gm) : book_kernel_any<BookDefault>(mk, skg, gm);
+const void* book_kernel(int mk, int gm, int occ) {
+    return occ ? book_kernel_o3(mk, gm) : book_kernel_any<BookDefault>(mk, gm);
 }
 
 hipError_t launch_book_reset(const BookLaunch& a, hipStream_t s) {
@@ -272,7 +272,7 @@ hipError_t launch_book_plan(const BookLaunch& a, hipStream_t s) {
 
 // scan -> persistent pass -> reduce (no planning or merging: level-sum hook and
 // tuner).  With events e0/e1 the persistent kernel alone is bracketed by them.
-hipError_t launch_book_sweep(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s, hipEvent_t e0,
+hipError_t launch_book_sweep(const void* fn, const BookLaunch& a, int grid, hipStream_t s, hipEvent_t e0,
                              hipEvent_t e1) {
     hipLaunchKernelGGL(k_book_scan, dim3(1), dim3(1024), 0, s, a);
     hipError_t e = hipSuccess;
@@ -285,10 +285,10 @@ hipError_t launch_book_sweep(const void* fn, int skg, const BookLaunch& a, int g
     return hipGetLastError();
 }
 
-hipError_t launch_book_pass(const void* fn, int skg, const BookLaunch& a, int grid, hipStream_t s, hipEvent_t e0,
+hipError_t launch_book_pass(const void* fn, const BookLaunch& a, int grid, hipStream_t s, hipEvent_t e0,
                             hipEvent_t e1) {
     hipLaunchKernelGGL(k_book_plan, dim3((a.n_contracts + 63) / 64), dim3(64), 0, s, a);
-    hipError_t e = launch_book_sweep(fn, skg, a, grid, s, e0, e1);
+    hipError_t e = launch_book_sweep(fn, a, grid, s, e0, e1);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_book_merge, dim3((a.n_contracts * BOOK_NL + 63) / 64), dim3(64), 0, s, a);
     return hipGetLastError();

@@ -165,11 +165,14 @@ struct BookAdj {
 // block, with no per-step storage.
 // PAIR: the step increments are dw[2k] + dw[2k+1] (coarse leg), whose step
 // times are the even fine times (vb holds the block's fine-grid time factors,
-// in registers).
-template <int MK, int N, bool PAIR>
+// in registers).  The direct terms d_k by payoff: the Asian's `each` at every
+// date, the lookback's `cmin` at its running minimum; the final date's `last`
+// is the initial adjoint of the sweep (A.adj at S_n), so no step carries it.
+template <int MK, int PK, int N, bool PAIR>
 __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, const float* dw, int nsteps, int k0,
-                                               const float* vb, float h, float muh, int n, float each, float last,
-                                               float cmin, int kmin, BookAdj& A) {
+                                               const float* vb, float h, float muh, float each, float cmin, int kmin,
+                                               BookAdj& A) {
+    constexpr bool DIRECT = PK == PK_ASIAN || PK == PK_LOOKBACK;
     float jS = 1.0f, jv = 0.0f, jm = 0.0f, j0 = 0.0f;
     float dS = 0.0f, dv = 0.0f, dm = 0.0f, d0 = 0.0f;
 #pragma unroll
@@ -185,12 +188,13 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
             jv = fS * jv + fv;
             jm = fS * jm + fm;
             j0 = fS * j0 + f0;
-            const int idx = gi + 1;
-            const float d = each + (idx == n ? last : 0.0f) + (idx == kmin ? cmin : 0.0f);
-            dS += d * jS;
-            dv += d * jv;
-            dm += d * jm;
-            d0 += d * j0;
+            if constexpr (DIRECT) {
+                const float d = (PK == PK_ASIAN) ? each : (gi + 1 == kmin ? cmin : 0.0f);
+                dS += d * jS;
+                dv += d * jv;
+                dm += d * jm;
+                d0 += d * j0;
+            }
         }
     }
     A.v += dv + A.adj * jv;
@@ -200,8 +204,9 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
 }
 
 // Payoffs of both legs, their Greeks and the accumulation of the level
-// difference.  `sweep(ef, lf, mf, ec, lc, mc, Af, Ac)` runs the adjoint reverse
-// sweep of the fine and coarse legs given the payoffs' direct sensitivities.
+// difference.  `sweep(ef, mf, ec, mc, Af, Ac)` runs the adjoint reverse sweep of
+// the fine and coarse legs given the payoffs' direct sensitivities (each date,
+// running minimum; the final date's is already in Af.adj, Ac.adj).
 template <int MK, int PK, int GM, class Sweep>
 __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
                                           const BookLeg<MK, PK, LIP_TAN(PK, GM)>& F,
@@ -238,8 +243,8 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
         float ef, lf, mf, ec, lc, mc;
         book_direct<PK>(P, F.S, F.run, nf, g.bgk_f, ef, lf, mf);
         book_direct<PK>(P, Cc.S, Cc.run, nc > 0 ? nc : 1, g.bgk_c, ec, lc, mc);
-        BookAdj Af{0.0f, 0.0f, 0.0f, 0.0f}, Ac{0.0f, 0.0f, 0.0f, 0.0f};
-        sweep(ef, lf, mf, ec, lc, mc, Af, Ac);
+        BookAdj Af{lf, 0.0f, 0.0f, 0.0f}, Ac{lc, 0.0f, 0.0f, 0.0f};   // dY/dS_n direct
+        sweep(ef, mf, ec, mc, Af, Ac);
         if (F.kmin == 0) Af.adj += mf;     // S0 itself is the running minimum
         if (Cc.kmin == 0) Ac.adj += mc;
         gf = Tan3{Af.adj + Af.s0, Af.v, Af.m};
@@ -300,49 +305,55 @@ __device__ __forceinline__ void book_path_short(const LegParams& P, const BookGe
         if (k & 1) Cc.step(P, dw[k - 1] + dw[k], vb[k - 1], g.hc, g.muh_c, (k >> 1) + 1);
     }
     book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,
-                          [&](float ef, float lf, float mf, float ec, float lc, float mc, BookAdj& Af, BookAdj& Ac) {
+                          [&](float ef, float mf, float ec, float mc, BookAdj& Af, BookAdj& Ac) {
                               if constexpr (NC > 0)
-                                  book_block_adj<MK, NC, true>(P, P.S0, dw, NC, 0, vb, g.hc, g.muh_c, NC, ec, lc, mc,
-                                                               Cc.kmin, Ac);
-                              book_block_adj<MK, NF, false>(P, P.S0, dw, NF, 0, vb, g.hf, g.muh_f, NF, ef, lf, mf,
-                                                            F.kmin, Af);
+                                  book_block_adj<MK, PK, NC, true>(P, P.S0, dw, NC, 0, vb, g.hc, g.muh_c, ec, mc,
+                                                                   Cc.kmin, Ac);
+                              book_block_adj<MK, PK, NF, false>(P, P.S0, dw, NF, 0, vb, g.hf, g.muh_f, ef, mf,
+                                                                F.kmin, Af);
                           });
 }
 
-// Levels with 16 or more dates: the bridge skeleton at the 16-date block
-// boundaries lives in sk (LDS or the workgroup's scratch), each block's 15
+// Levels of 32 or more dates (G = 2^m blocks of 16, 1 <= m <= 5).  The bridge
+// skeleton (W at the block boundaries) is not stored: the forward pass walks
+// the boundaries left to right keeping, per dyadic size 2^k, W at the right end
+// of the aligned interval that holds the current boundary (rt[k]); moving to
+// boundary i makes the intervals of size 2^k <= 2^ctz(i) start at i, the one of
+// size 2^ctz(i) ends where its parent does, the smaller ones end at new
+// midpoints.  Every skeleton point is generated once, from the same operands
+// as the top-down bridge, so the values are identical.  The adjoint's reverse
+// sweep walks right to left the same way (lt[k]: left ends).  Each block's 15
 // inner points are generated in registers; the adjoint regenerates them block
-// by block from the Sobol index in the reverse sweep, from checkpoints of S.
+// by block from the Sobol index, from checkpoints of S.
 template <int MK, int PK, int GM>
 __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
                                                const float* __restrict__ vt, const SobolLane& sl, const float4* itab,
-                                               float* sk, float* ckf, float* ckc, bool valid,
-                                               double (&acc)[BOOK_NACC]) {
+                                               float* ckf, float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
     constexpr bool ADJ = PK != PK_BARRIER && GM == 1;
     constexpr bool TAN = LIP_TAN(PK, GM);
+    constexpr int MMAX = BOOK_NL - 1 - 4;        // log2 of the most blocks per path
     const int G = g.G;
-    sk[0] = 0.0f;
-    sk[G * 256] = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0), itab, sdt[0], -2.0f * sdt[0]);
-    for (int j = 1; j < G; j++) {
-        const int lev = 31 - __builtin_clz(j);
-        const int q = j - (1 << lev);
-        const int st = G >> lev;
-        const int L = q * st, R = L + st, M = L + (st >> 1);
+    const int m = 31 - __builtin_clz(G);
+    // W at the midpoint of [L, L + 2^(k+1)) from the ends: the skeleton point
+    // of bridge level m-1-k, index L >> (k+1), coordinate 2^(m-1-k) + (L >> (k+1))
+    auto node = [&](int k, int L, float WL, float WR) {
+        const int lev = m - 1 - k;
         const float sd = sdt[lev + 1];
-        sk[M * 256] = 0.5f * (sk[L * 256] + sk[R * 256]) -
-                      icdf_u32_tab_scaled_neg(sobol_coord(sl, j), itab, sd, -2.0f * sd);
-    }
-    const int lgG = 31 - __builtin_clz(G);
+        return 0.5f * (WL + WR) -
+               icdf_u32_tab_scaled_neg(sobol_coord(sl, (1 << lev) + (L >> (k + 1))), itab, sd, -2.0f * sd);
+    };
+    // icdf_u32_tab_scaled_neg gives -sd z: the bridge subtracts it
+    const float WG = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0), itab, sdt[0], -2.0f * sdt[0]);
     float sdb[4], m2sdb[4];
 #pragma unroll
     for (int lv = 0; lv < 4; lv++) {
-        sdb[lv] = sdt[lgG + lv + 1];
+        sdb[lv] = sdt[m + lv + 1];
         m2sdb[lv] = -2.0f * sdb[lv];
     }
-    auto gen_block = [&](int i, float (&dw)[16]) {
+    auto gen_block = [&](int i, float WL, float WR, float (&dw)[16]) {
         float W[17];
-        W[0] = sk[i * 256];
-        W[16] = sk[(i + 1) * 256];
+        W[0] = WL;
+        W[16] = WR;
 #pragma unroll
         for (int lv = 0; lv < 4; lv++) {
             const int j0 = (G << lv) + (i << lv);
@@ -367,10 +378,27 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
 #pragma unroll
         for (int k = 0; k < 16; k++) vb[k] = vt[i * 16 + k];
     };
+    float rt[MMAX + 1];
+    float Wp = 0.0f;                              // W at the current boundary i
+#pragma unroll
+    for (int k = 0; k <= MMAX; k++)
+        if (k == m) rt[k] = WG;
+#pragma unroll
+    for (int k = MMAX - 1; k >= 0; k--)
+        if (k < m) rt[k] = node(k, 0, Wp, rt[k + 1]);
     for (int i = 0; i < G; i++) {
+        if (i > 0) {
+            Wp = rt[0];
+            const int c = __builtin_ctz(i);       // < m
+#pragma unroll
+            for (int k = MMAX - 1; k >= 0; k--) {
+                if (k == c) rt[k] = rt[k + 1];
+                else if (k < c) rt[k] = node(k, i, Wp, rt[k + 1]);
+            }
+        }
         float dw[16], vb[16];
         load_vb(i, vb);
-        gen_block(i, dw);
+        gen_block(i, Wp, rt[0], dw);
         if constexpr (ADJ) {
             ckf[i * 256] = F.S;
             ckc[i * 256] = Cc.S;
@@ -383,15 +411,32 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
         }
     }
     book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,
-                          [&](float ef, float lf, float mf, float ec, float lc, float mc, BookAdj& Af, BookAdj& Ac) {
+                          [&](float ef, float mf, float ec, float mc, BookAdj& Af, BookAdj& Ac) {
+                              float lt[MMAX + 1];
+                              float WR = WG;        // W at the current block's right end i + 1
+#pragma unroll
+                              for (int k = 0; k <= MMAX; k++)
+                                  if (k == m) lt[k] = 0.0f;
+#pragma unroll
+                              for (int k = MMAX - 1; k >= 0; k--)
+                                  if (k < m) lt[k] = node(k, G - (2 << k), lt[k + 1], WR);
                               for (int i = G - 1; i >= 0; i--) {
+                                  if (i < G - 1) {
+                                      WR = lt[0];
+                                      const int p = i + 1, c = __builtin_ctz(p);   // < m
+#pragma unroll
+                                      for (int k = MMAX - 1; k >= 0; k--) {
+                                          if (k == c) lt[k] = lt[k + 1];
+                                          else if (k < c) lt[k] = node(k, p - (2 << k), lt[k + 1], WR);
+                                      }
+                                  }
                                   float dw[16], vb[16];
                                   load_vb(i, vb);
-                                  gen_block(i, dw);
-                                  book_block_adj<MK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vb, g.hc, g.muh_c,
-                                                              g.nc, ec, lc, mc, Cc.kmin, Ac);
-                                  book_block_adj<MK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vb, g.hf, g.muh_f,
-                                                                g.nf, ef, lf, mf, F.kmin, Af);
+                                  gen_block(i, lt[0], WR, dw);
+                                  book_block_adj<MK, PK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vb, g.hc,
+                                                                  g.muh_c, ec, mc, Cc.kmin, Ac);
+                                  book_block_adj<MK, PK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vb, g.hf,
+                                                                    g.muh_f, ef, mf, F.kmin, Af);
                               }
                           });
 }
@@ -437,8 +482,6 @@ __device__ __forceinline__ void book_tiles_short(const LegParams& P, const BookG
     }
 }
 
-// SKG = 0: bridge skeleton in LDS; 1: in the workgroup's global scratch (less
-// LDS per workgroup, more workgroups per CU).
 // A workgroup-uniform value read from memory, moved into a scalar register.
 __device__ __forceinline__ int sreg(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ float sreg(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
@@ -449,24 +492,21 @@ __device__ __forceinline__ int64_t sreg(int64_t x) {
 }
 __device__ __forceinline__ double sreg(double x) { return __longlong_as_double(sreg((int64_t)__double_as_longlong(x))); }
 
-template <int MK, int SKG, int GM>
+template <int MK, int GM>
 __device__ __forceinline__ void book_body(const BookLaunch& a) {
-    // static LDS (its size depends only on SKG): addresses fold into the
-    // instructions' offset fields instead of adding a dynamic base to each
-    __shared__ __align__(16) unsigned char smem[book_lds_static(SKG)];
+    // static LDS: addresses fold into the instructions' offset fields instead
+    // of adding a dynamic base to each
+    __shared__ __align__(16) unsigned char smem[book_lds_static()];
     const int tid = threadIdx.x, lane = tid & 63;
     uint32_t* rec = (uint32_t*)smem;                                     // [BOOK_DMAX][20]
     constexpr size_t REC_BYTES = (size_t)BOOK_DMAX * 20 * 4;
-    constexpr size_t SKEL_BYTES = SKG ? 0 : (size_t)(BOOK_GMAX + 1) * 256 * 4;
-    float4* itab = (float4*)(smem + REC_BYTES + SKEL_BYTES);             // inverse-CDF table
-    double* red = (double*)(smem + REC_BYTES + SKEL_BYTES + (size_t)FDMC_ICDF_TAB_ROWS * 16);
+    float4* itab = (float4*)(smem + REC_BYTES);                          // inverse-CDF table
+    double* red = (double*)(smem + REC_BYTES + (size_t)FDMC_ICDF_TAB_ROWS * 16);
     int* s_item = (int*)(red + 4 * BOOK_NACC);
-    float* wscr = a.scratch + (size_t)blockIdx.x * book_scratch_floats(SKG);
+    // the adjoint's checkpoints of S (fine, coarse) at the 16-step block starts
+    float* wscr = a.scratch + (size_t)blockIdx.x * book_scratch_floats();
     float* ckf = wscr + tid;
     float* ckc = ckf + BOOK_GMAX * 256;
-    float* sk;
-    if constexpr (SKG) sk = wscr + 2 * BOOK_GMAX * 256 + tid;
-    else sk = (float*)(smem + REC_BYTES) + tid;
 
     SobolLane sl;
     {
@@ -597,10 +637,10 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
                 const int64_t idx = ((int64_t)n << 8) + tid;
                 const bool valid = idx >= E.first && idx < E.first + E.count;
                 switch (kind) {
-                case PK_EUROPEAN: book_path_long<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
-                case PK_ASIAN: book_path_long<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
-                case PK_LOOKBACK: book_path_long<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
-                default: book_path_long<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, itab, sk, ckf, ckc, valid, acc); break;
+                case PK_EUROPEAN: book_path_long<MK, PK_EUROPEAN, GM>(P, g, sdt, vt, sl, itab, ckf, ckc, valid, acc); break;
+                case PK_ASIAN: book_path_long<MK, PK_ASIAN, GM>(P, g, sdt, vt, sl, itab, ckf, ckc, valid, acc); break;
+                case PK_LOOKBACK: book_path_long<MK, PK_LOOKBACK, GM>(P, g, sdt, vt, sl, itab, ckf, ckc, valid, acc); break;
+                default: book_path_long<MK, PK_BARRIER, GM>(P, g, sdt, vt, sl, itab, ckf, ckc, valid, acc); break;
                 }
             }
         }
@@ -616,36 +656,33 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
 // Two compilations of the same body: the compiler's own register allocation,
 // and one asked for three waves per SIMD (it may spill to reach it).  The
 // launch-shape autotuner times both.
-template <int MK, int SKG, int GM>
+template <int MK, int GM>
 __global__ __launch_bounds__(256) void k_book(BookLaunch a) {
-    book_body<MK, SKG, GM>(a);
+    book_body<MK, GM>(a);
 }
 
-template <int MK, int SKG, int GM>
+template <int MK, int GM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_book_o3(BookLaunch a) {
-    book_body<MK, SKG, GM>(a);
+    book_body<MK, GM>(a);
 }
 
-// Kernel of a (model, skeleton placement, Greeks mode) for one occupancy choice.
-template <template <int, int, int> class Sel, int MK>
-const void* book_kernel_sel(int skg, int gm) {
-    switch (gm * 2 + skg) {
-    case 0: return Sel<MK, 0, 0>::fn();
-    case 1: return Sel<MK, 1, 0>::fn();
-    case 2: return Sel<MK, 0, 1>::fn();
-    case 3: return Sel<MK, 1, 1>::fn();
-    case 4: return Sel<MK, 0, 2>::fn();
-    case 5: return Sel<MK, 1, 2>::fn();
+// Kernel of a (model, Greeks mode) for one register-allocation choice.
+template <template <int, int> class Sel, int MK>
+const void* book_kernel_sel(int gm) {
+    switch (gm) {
+    case 0: return Sel<MK, 0>::fn();
+    case 1: return Sel<MK, 1>::fn();
+    case 2: return Sel<MK, 2>::fn();
     default: return nullptr;
     }
 }
 
-template <template <int, int, int> class Sel>
-const void* book_kernel_any(int mk, int skg, int gm) {
+template <template <int, int> class Sel>
+const void* book_kernel_any(int mk, int gm) {
     switch (mk) {
-    case MK_GBM: return book_kernel_sel<Sel, MK_GBM>(skg, gm);
-    case MK_DUPIRE: return book_kernel_sel<Sel, MK_DUPIRE>(skg, gm);
-    case MK_LVQ: return book_kernel_sel<Sel, MK_LVQ>(skg, gm);
+    case MK_GBM: return book_kernel_sel<Sel, MK_GBM>(gm);
+    case MK_DUPIRE: return book_kernel_sel<Sel, MK_DUPIRE>(gm);
+    case MK_LVQ: return book_kernel_sel<Sel, MK_LVQ>(gm);
     default: return nullptr;
     }
 }

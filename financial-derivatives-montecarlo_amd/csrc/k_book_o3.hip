@@ -4,11 +4,11 @@
 
 namespace fdmc {
 
-template <int MK, int SKG, int GM>
+template <int MK, int GM>
 struct BookO3 {
-    static const void* fn() { return (const void*)&k_book_o3<MK, SKG, GM>; }
+    static const void* fn() { return (const void*)&k_book_o3<MK, GM>; }
 };
 
-const void* book_kernel_o3(int mk, int skg, int gm) { return book_kernel_any<BookO3>(mk, skg, gm); }
+const void* book_kernel_o3(int mk, int gm) { return book_kernel_any<BookO3>(mk, gm); }
 
 }  // namespace fdmc

@@ -203,21 +203,19 @@ struct BookLaunch {
     int* active;
     double* partials;         // [items][BOOK_NACC]
     double* entry_sums;       // [n_entries][BOOK_NACC]
-    float* scratch;           // per workgroup: checkpoints [2][BOOK_GMAX][256] (+ skeleton)
+    float* scratch;           // per workgroup: checkpoints [2][BOOK_GMAX][256]
     BookOut* out;
 };
 
 // LDS of one book workgroup: direction records [BOOK_DMAX][20] words, the
-// bridge skeleton when it is kept in LDS, the inverse-CDF table, the block
-// reduction and the work-item slot.
-constexpr size_t book_lds_static(int skel_global) {
-    return (size_t)BOOK_DMAX * 20 * 4 + (skel_global ? 0 : (size_t)(BOOK_GMAX + 1) * 256 * 4) +
-           (size_t)FDMC_ICDF_TAB_ROWS * 16 + 4 * BOOK_NACC * 8 + 16;
+// inverse-CDF table, the block reduction and the work-item slot.
+constexpr size_t book_lds_static() {
+    return (size_t)BOOK_DMAX * 20 * 4 + (size_t)FDMC_ICDF_TAB_ROWS * 16 + 4 * BOOK_NACC * 8 + 16;
 }
-static_assert(book_lds_static(0) <= 160 * 1024, "book LDS footprint exceeds a CU's LDS");
+static_assert(book_lds_static() <= 160 * 1024, "book LDS footprint exceeds a CU's LDS");
 
-constexpr size_t book_scratch_floats(int skel_global) {
-    return (size_t)2 * BOOK_GMAX * 256 + (skel_global ? (size_t)(BOOK_GMAX + 1) * 256 : 0);
-}
+// Global scratch of one book workgroup: the adjoint's checkpoints of S (fine
+// and coarse leg) at the block starts, [2][BOOK_GMAX][256] floats.
+constexpr size_t book_scratch_floats() { return (size_t)2 * BOOK_GMAX * 256; }
 
 }  // namespace fdmc

@@ -198,8 +198,8 @@ typedef struct fdmc_book_stats {
     int64_t path_steps;          /* all contracts */
     int32_t passes;              /* device passes (batched) or host iterations (plain) */
     int32_t graph_replays;       /* batched mode */
-    int32_t variant;             /* autotuned kernel: skeleton in LDS (0) or global (1), + 2 for the
-                                    three-waves-per-SIMD compilation */
+    int32_t variant;             /* autotuned kernel: the compiler's register allocation (0) or the
+                                    three-waves-per-SIMD compilation (1) */
     int32_t items_target;        /* noise dimensions per lane and work item chosen by the autotuner */
     int32_t wg_per_cu;           /* persistent workgroups per CU chosen by the autotuner */
     int32_t pad;
