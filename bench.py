@@ -49,7 +49,7 @@ BOOK_KERNEL = "k_book<LOCALVOL_QUAD, adjoint Greeks>"
 # WRITE_SIZE of two rocprofv3 --pmc runs of `bench.py --roofline-only`, scaled by
 # the same counters on a 512 MiB device-to-device copy (reads x2.0, writes x1.0);
 # a counter run cannot be part of the timed process, so the figure is recorded.
-BOOK_TRAFFIC_BYTES_PER_LAUNCH = 4.385e10
+BOOK_TRAFFIC_BYTES_PER_LAUNCH = 1.892e11
 
 
 def log(*a):
