@@ -94,7 +94,10 @@ struct PathState {
 };
 
 template <int MK, int PK, int CV, int CPK, int NZ, int LB>
-__global__ __launch_bounds__(256) void k_path(PathLaunch a) {
+// Compiled for three waves per SIMD (168 VGPRs, under 100 B of spills): 23 %
+// faster than the compiler's own allocation (190 VGPRs, two waves) on the
+// 256-date bridge pass, measured on MI355X.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_path(PathLaunch a) {
     constexpr int NACC = (CV == CV_NONE) ? 2 : 7;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63;
