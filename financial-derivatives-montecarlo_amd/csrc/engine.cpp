@@ -39,6 +39,10 @@ struct fdmc_bb {
     int N;
     double T;
     float* d_w;   // [N][4] {wl, wr, sd, 0}
+    // uniform date grid: every FP32 weight is 1/2 and the sd depends only on
+    // the depth (sd_depth[0] terminal, [k] midpoints of schedule level k - 1)
+    int uniform;
+    float sd_depth[BB_SD_DEPTHS];
 };
 
 struct fdmc_pca {
@@ -286,6 +290,8 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
         lb = s.n_steps >= 16 ? 4 : 0;
         a.G = s.n_steps >> lb;
         a.bbw = s.bb->d_w;
+        a.bb_uniform = s.bb->uniform;
+        for (int k = 0; k < BB_SD_DEPTHS; k++) a.bb_sd[k] = s.bb->sd_depth[k];
         fn = s.greeks ? greeks_kernel_bb(mk, pk, cv, cpk, lb) : path_kernel_bb(mk, pk, cv, cpk, lb);
     } else if (s.mode == FDMC_NOISE_PCA) {
         if (nf != 1) fail(FDMC_E_UNSUPPORTED, "PCA noise needs a one-factor model (methods_cuda.cu:1702-1704)");
@@ -1045,7 +1051,14 @@ fdmc_status fdmc_bb_upload(fdmc_context* c, int32_t N, double T, const int32_t* 
             w[4 * j + 2] = (float)sd[j];
             w[4 * j + 3] = 0.0f;
         }
-        auto* b = new fdmc_bb{c, N, T, nullptr};
+        auto* b = new fdmc_bb{c, N, T, nullptr, 1, {}};
+        b->sd_depth[0] = w[2];
+        for (int j = 1; j < N && b->uniform; j++) {
+            const int depth = 32 - __builtin_clz((unsigned)j);     // schedule level + 1
+            if (depth >= BB_SD_DEPTHS || w[4 * j] != 0.5f || w[4 * j + 1] != 0.5f) b->uniform = 0;
+            else if (j == (1 << (depth - 1))) b->sd_depth[depth] = w[4 * j + 2];
+            else if (w[4 * j + 2] != b->sd_depth[depth]) b->uniform = 0;
+        }
         if (hipMalloc((void**)&b->d_w, w.size() * sizeof(float)) != hipSuccess) {
             delete b;
             fail(FDMC_E_ALLOC, "hipMalloc failed");

@@ -54,18 +54,10 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
         const int64_t base = (tile0 + it) << 8;
         if (base >= last) break;
         if constexpr (SRC == SRC_SOBOL) {
-            const uint32_t b32 = (uint32_t)base, gbase = b32 ^ (b32 >> 1);
+            const uint32_t b32 = (uint32_t)base;
             if (lds_mode) {
                 __syncthreads();
-                for (int d = tid; d < D; d += 256) {
-                    const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-                    uint32_t s = 0;
-                    for (uint32_t x = gbase; x; x &= x - 1u) s ^= gr[32 + __builtin_ctz(x)];
-                    rec[d * 20 + 16] = s ^ gr[16];
-                    rec[d * 20 + 17] = s ^ gr[17];
-                    rec[d * 20 + 18] = s ^ gr[18];
-                    rec[d * 20 + 19] = s ^ gr[19];
-                }
+                sobol_tile_words(rec, grec, D, (uint32_t)(tile0 + it), it == 0, tid);
                 __syncthreads();
             } else {
                 const uint32_t hi = b32 + (sl.w << 6);

@@ -66,6 +66,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
         __syncthreads();
     }
     const int G = (NZ == NZ_SOBOL_BB) ? a.G : (n + B - 1) / B;
+    const int lgG = 31 - __builtin_clz(G > 0 ? G : 1);
     const bool lds_mode = (NZ != NZ_PRNG) && D <= SOBOL_LDS_MAX_D;
     uint32_t* rec = (uint32_t*)smem;
     const int rec_words = (NZ != NZ_PRNG && lds_mode) ? D * 20 : 0;
@@ -105,18 +106,9 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
         if (base >= last) break;
         if constexpr (NZ != NZ_PRNG) {
             const uint32_t b32 = (uint32_t)base;
-            const uint32_t gbase = b32 ^ (b32 >> 1);
             if (lds_mode) {
                 __syncthreads();
-                for (int d = tid; d < D; d += 256) {
-                    const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-                    uint32_t s = 0;
-                    for (uint32_t g = gbase; g; g &= g - 1u) s ^= gr[32 + __builtin_ctz(g)];
-                    rec[d * 20 + 16] = s ^ gr[16];
-                    rec[d * 20 + 17] = s ^ gr[17];
-                    rec[d * 20 + 18] = s ^ gr[18];
-                    rec[d * 20 + 19] = s ^ gr[19];
-                }
+                sobol_tile_words(rec, grec, D, (uint32_t)(tile0 + it), it == 0, tid);
                 __syncthreads();
             } else {
                 const uint32_t hi = b32 + (sl.w << 6);
@@ -128,156 +120,162 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
         float* sk = skel + tid;
         float* ck = ckp + tid;
 
-        if constexpr (NZ == NZ_SOBOL_BB) {
-            const float4* bw = (const float4*)a.bbw;
-            sk[0] = 0.0f;
-            sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
-            for (int j = 1; j < G; j++) {
-                const int lev = 31 - __builtin_clz(j);
-                const int q = j - (1 << lev);
-                const int st = G >> lev;
-                const int L = q * st, R = L + st, M = L + (st >> 1);
-                const float4 c = bw[j];
-                sk[M * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
-            }
-        }
-        // increments of block i (B steps; steps beyond n are zero)
-        auto gen_block = [&](int i, float (&dw)[B]) {
+        // the rest of the tile, for a uniform-grid bridge (UNI) or bb_precompute's weights
+        auto tile = [&](auto uniform_tag) {
+            constexpr bool UNI = decltype(uniform_tag)::value;
             if constexpr (NZ == NZ_SOBOL_BB) {
-                if constexpr (B == 1) {
-                    dw[0] = sk[(i + 1) * 256] - sk[i * 256];
-                } else {
-                    const float4* bw = (const float4*)a.bbw;
-                    float W[B + 1];
-                    W[0] = sk[i * 256];
-                    W[B] = sk[(i + 1) * 256];
-#pragma unroll
-                    for (int l = 0; l < 4; l++) {
-                        const int j0 = (G << l) + (i << l);
-#pragma unroll
-                        for (int q = 0; q < (1 << l); q++) {
-                            const int s = B >> l;
-                            const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                            const float4 c = bw[j0 + q];
-                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32_tab(sobol_coord(sl, j0 + q), itab);
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < B; k++) dw[k] = W[k + 1] - W[k];
+                sk[0] = 0.0f;
+                sk[G * 256] = bb_terminal<UNI>(a, sl, itab);
+                for (int j = 1; j < G; j++) {
+                    const int lev = 31 - __builtin_clz(j);
+                    const int q = j - (1 << lev);
+                    const int st = G >> lev;
+                    const int L = q * st, R = L + st, M = L + (st >> 1);
+                    sk[M * 256] = bb_mid<UNI>(a, sl, itab, sk[L * 256], sk[R * 256], j, lev + 1);
                 }
-            } else if constexpr (NZ == NZ_SOBOL_RAW) {
-#pragma unroll
+            }
+            // increments of block i (B steps; steps beyond n are zero)
+            auto gen_block = [&](int i, float (&dw)[B]) {
+                if constexpr (NZ == NZ_SOBOL_BB) {
+                    if constexpr (B == 1) {
+                        dw[0] = sk[(i + 1) * 256] - sk[i * 256];
+                    } else {
+                        float W[B + 1];
+                        W[0] = sk[i * 256];
+                        W[B] = sk[(i + 1) * 256];
+    #pragma unroll
+                        for (int l = 0; l < 4; l++) {
+                            const int j0 = (G << l) + (i << l);
+    #pragma unroll
+                            for (int q = 0; q < (1 << l); q++) {
+                                const int s = B >> l;
+                                const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
+                                W[Mp] = bb_mid<UNI>(a, sl, itab, W[Lp], W[Rp], j0 + q, lgG + l + 1);
+                            }
+                        }
+    #pragma unroll
+                        for (int k = 0; k < B; k++) dw[k] = W[k + 1] - W[k];
+                    }
+                } else if constexpr (NZ == NZ_SOBOL_RAW) {
+    #pragma unroll
+                    for (int k = 0; k < B; k++) {
+                        const int g = i * B + k;
+                        dw[k] = (g < n) ? icdf_u32_tab(sobol_coord(sl, g), itab) * a.sqrt_h : 0.0f;
+                    }
+                } else {
+    #pragma unroll
+                    for (int b = 0; b < B / 4; b++) {
+                        float z[4];
+                        prng_normals4(a.seed + (uint32_t)r, a.tag, (uint64_t)idx, (uint32_t)(i * (B / 4) + b), z);
+    #pragma unroll
+                        for (int c = 0; c < 4; c++) dw[4 * b + c] = (i * B + 4 * b + c < n) ? z[c] * a.sqrt_h : 0.0f;
+                    }
+                }
+            };
+
+            // ---- forward
+            float S = P.S0, run = run_init<PK>(P), crun = run_init<CPK>(a.leg[1]);
+            int kmin = 0;
+            for (int i = 0; i < G; i++) {
+                float dw[B];
+                gen_block(i, dw);
+                ck[i * 256] = S;
+    #pragma unroll
                 for (int k = 0; k < B; k++) {
                     const int g = i * B + k;
-                    dw[k] = (g < n) ? icdf_u32_tab(sobol_coord(sl, g), itab) * a.sqrt_h : 0.0f;
+                    if (g < n) {
+                        S = step_1f<MK>(P, S, dw[k], (float)g * h, h);
+                        if constexpr (PK == PK_LOOKBACK) {
+                            if (S < run) { run = S; kmin = g + 1; }
+                        } else if constexpr (is_path_dep<PK>()) {
+                            run_update<PK>(run, S);
+                        }
+                        if constexpr (CV == CV_SHARED) run_update<CPK>(crun, S);
+                    }
                 }
-            } else {
-#pragma unroll
-                for (int b = 0; b < B / 4; b++) {
-                    float z[4];
-                    prng_normals4(a.seed + (uint32_t)r, a.tag, (uint64_t)idx, (uint32_t)(i * (B / 4) + b), z);
-#pragma unroll
-                    for (int c = 0; c < 4; c++) dw[4 * b + c] = (i * B + 4 * b + c < n) ? z[c] * a.sqrt_h : 0.0f;
+            }
+            const float ym = payoff<PK>(P, S, run, n, a.bgk[0]);
+            float yc = 0.0f;
+            if constexpr (CV == CV_SHARED) yc = payoff<CPK>(a.leg[1], S, crun, n, a.bgk[1]);
+
+            // ---- direct sensitivities dY/dS_k (k >= 1)
+            const float inv_n = 1.0f / (float)n;
+            float c_each = 0.0f;        // added to every S_k
+            float c_log = 0.0f;         // times 1/S_k for every S_k
+            float c_last = 0.0f;        // added at S_n
+            float c_min = 0.0f;         // added at S_kmin
+            if constexpr (PK == PK_ASIAN) c_each = (run * inv_n > P.K) ? inv_n : 0.0f;
+            if constexpr (PK == PK_GEOM_ASIAN) {
+                const float e = fast_exp(run * inv_n);
+                c_log = (e > P.K) ? e * inv_n : 0.0f;
+            }
+            if constexpr (PK == PK_EUROPEAN) c_last = (S > P.K) ? P.discount : 0.0f;
+            if constexpr (PK == PK_LOOKBACK) { c_last = 1.0f; c_min = -a.bgk[0]; }
+            if constexpr (CV == CV_SHARED && CPK == PK_GEOM_ASIAN) {
+                const float e = fast_exp(crun * inv_n);
+                c_log -= a.beta_cv * ((e > a.leg[1].K) ? e * inv_n : 0.0f);
+            }
+
+            // ---- reverse sweep
+            float adj = 0.0f, vb = 0.0f, mb = 0.0f, s0b = 0.0f;
+            for (int i = G - 1; i >= 0; i--) {
+                float dw[B], Sl[B + 1];
+                gen_block(i, dw);
+                Sl[0] = ck[i * 256];
+    #pragma unroll
+                for (int k = 0; k < B; k++) {
+                    const int g = i * B + k;
+                    Sl[k + 1] = (g < n) ? step_1f<MK>(P, Sl[k], dw[k], (float)g * h, h) : Sl[k];
                 }
+    #pragma unroll
+                for (int k = B - 1; k >= 0; k--) {
+                    const int g = i * B + k;
+                    if (g < n) {
+                        const float Sn = Sl[k + 1];
+                        adj += c_each + c_log / Sn;
+                        if (g + 1 == n) adj += c_last;
+                        if (g + 1 == kmin) adj += c_min;
+                        float fS, fv, fm, f0;
+                        step_partials<MK>(P, Sl[k], dw[k], (float)g * h, h, fS, fv, fm, f0);
+                        vb += adj * fv;
+                        mb += adj * fm;
+                        s0b += adj * f0;
+                        adj *= fS;
+                    }
+                }
+            }
+            if (kmin == 0) adj += c_min;     // running minimum at S0
+            const float delta = adj + s0b;
+            float rho = mb;
+            if constexpr (PK == PK_EUROPEAN) rho -= a.T * ym;   // d/dr of the discount e^{-rT}
+
+            if (valid) {
+                const double dm = (double)ym;
+                acc[0] += dm;
+                acc[1] += dm * dm;
+                if constexpr (CV != CV_NONE) {
+                    const float ycv = ym - a.beta_cv * (yc - a.E_ctrl);
+                    const double dv = (double)ycv, dc = (double)yc;
+                    acc[2] += dv;
+                    acc[3] += dv * dv;
+                    acc[4] += dc;
+                    acc[5] += dc * dc;
+                    acc[6] += dm * dc;
+                }
+                const double gd = delta, gv = vb, gr = rho;
+                acc[NP + 0] += gd;
+                acc[NP + 1] += gd * gd;
+                acc[NP + 2] += gv;
+                acc[NP + 3] += gv * gv;
+                acc[NP + 4] += gr;
+                acc[NP + 5] += gr * gr;
             }
         };
-
-        // ---- forward
-        float S = P.S0, run = run_init<PK>(P), crun = run_init<CPK>(a.leg[1]);
-        int kmin = 0;
-        for (int i = 0; i < G; i++) {
-            float dw[B];
-            gen_block(i, dw);
-            ck[i * 256] = S;
-#pragma unroll
-            for (int k = 0; k < B; k++) {
-                const int g = i * B + k;
-                if (g < n) {
-                    S = step_1f<MK>(P, S, dw[k], (float)g * h, h);
-                    if constexpr (PK == PK_LOOKBACK) {
-                        if (S < run) { run = S; kmin = g + 1; }
-                    } else if constexpr (is_path_dep<PK>()) {
-                        run_update<PK>(run, S);
-                    }
-                    if constexpr (CV == CV_SHARED) run_update<CPK>(crun, S);
-                }
-            }
-        }
-        const float ym = payoff<PK>(P, S, run, n, a.bgk[0]);
-        float yc = 0.0f;
-        if constexpr (CV == CV_SHARED) yc = payoff<CPK>(a.leg[1], S, crun, n, a.bgk[1]);
-
-        // ---- direct sensitivities dY/dS_k (k >= 1)
-        const float inv_n = 1.0f / (float)n;
-        float c_each = 0.0f;        // added to every S_k
-        float c_log = 0.0f;         // times 1/S_k for every S_k
-        float c_last = 0.0f;        // added at S_n
-        float c_min = 0.0f;         // added at S_kmin
-        if constexpr (PK == PK_ASIAN) c_each = (run * inv_n > P.K) ? inv_n : 0.0f;
-        if constexpr (PK == PK_GEOM_ASIAN) {
-            const float e = fast_exp(run * inv_n);
-            c_log = (e > P.K) ? e * inv_n : 0.0f;
-        }
-        if constexpr (PK == PK_EUROPEAN) c_last = (S > P.K) ? P.discount : 0.0f;
-        if constexpr (PK == PK_LOOKBACK) { c_last = 1.0f; c_min = -a.bgk[0]; }
-        if constexpr (CV == CV_SHARED && CPK == PK_GEOM_ASIAN) {
-            const float e = fast_exp(crun * inv_n);
-            c_log -= a.beta_cv * ((e > a.leg[1].K) ? e * inv_n : 0.0f);
-        }
-
-        // ---- reverse sweep
-        float adj = 0.0f, vb = 0.0f, mb = 0.0f, s0b = 0.0f;
-        for (int i = G - 1; i >= 0; i--) {
-            float dw[B], Sl[B + 1];
-            gen_block(i, dw);
-            Sl[0] = ck[i * 256];
-#pragma unroll
-            for (int k = 0; k < B; k++) {
-                const int g = i * B + k;
-                Sl[k + 1] = (g < n) ? step_1f<MK>(P, Sl[k], dw[k], (float)g * h, h) : Sl[k];
-            }
-#pragma unroll
-            for (int k = B - 1; k >= 0; k--) {
-                const int g = i * B + k;
-                if (g < n) {
-                    const float Sn = Sl[k + 1];
-                    adj += c_each + c_log / Sn;
-                    if (g + 1 == n) adj += c_last;
-                    if (g + 1 == kmin) adj += c_min;
-                    float fS, fv, fm, f0;
-                    step_partials<MK>(P, Sl[k], dw[k], (float)g * h, h, fS, fv, fm, f0);
-                    vb += adj * fv;
-                    mb += adj * fm;
-                    s0b += adj * f0;
-                    adj *= fS;
-                }
-            }
-        }
-        if (kmin == 0) adj += c_min;     // running minimum at S0
-        const float delta = adj + s0b;
-        float rho = mb;
-        if constexpr (PK == PK_EUROPEAN) rho -= a.T * ym;   // d/dr of the discount e^{-rT}
-
-        if (valid) {
-            const double dm = (double)ym;
-            acc[0] += dm;
-            acc[1] += dm * dm;
-            if constexpr (CV != CV_NONE) {
-                const float ycv = ym - a.beta_cv * (yc - a.E_ctrl);
-                const double dv = (double)ycv, dc = (double)yc;
-                acc[2] += dv;
-                acc[3] += dv * dv;
-                acc[4] += dc;
-                acc[5] += dc * dc;
-                acc[6] += dm * dc;
-            }
-            const double gd = delta, gv = vb, gr = rho;
-            acc[NP + 0] += gd;
-            acc[NP + 1] += gd * gd;
-            acc[NP + 2] += gv;
-            acc[NP + 3] += gv * gv;
-            acc[NP + 4] += gr;
-            acc[NP + 5] += gr * gr;
+        if constexpr (NZ == NZ_SOBOL_BB) {
+            if (a.bb_uniform) tile(std::true_type{});
+            else tile(std::false_type{});
+        } else {
+            tile(std::false_type{});
         }
     }
     __syncthreads();

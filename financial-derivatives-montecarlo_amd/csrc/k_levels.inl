@@ -126,7 +126,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if constexpr (NZ != NZ_PRNG) {
             sl.grec = E.sdir;
             const uint32_t b32 = (uint32_t)base;
-            const uint32_t gbase = b32 ^ (b32 >> 1);
             if (lds_ok) {
                 if (t > t0) __syncthreads();   // every lane is done with the previous tile's table
                 if (lo != last_e) {
@@ -135,15 +134,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         rec[d * 20 + k] = E.sdir[(int64_t)d * SDIR_WORDS + k];
                     }
                 }
-                for (int d = tid; d < D; d += 256) {
-                    const uint32_t* gr = E.sdir + (int64_t)d * SDIR_WORDS;
-                    uint32_t s = 0;
-                    for (uint32_t g = gbase; g; g &= g - 1u) s ^= gr[32 + __builtin_ctz(g)];
-                    rec[d * 20 + 16] = s ^ gr[16];
-                    rec[d * 20 + 17] = s ^ gr[17];
-                    rec[d * 20 + 18] = s ^ gr[18];
-                    rec[d * 20 + 19] = s ^ gr[19];
-                }
+                sobol_tile_words(rec, E.sdir, D, (uint32_t)(base >> 8), t == t0, tid);
                 __syncthreads();
             } else {
                 const uint32_t hi2 = b32 + (sl.w << 6);

@@ -19,6 +19,8 @@
 // same formula wl W_left + wr W_right + sd z as kernel_bb_transform (:561-594).
 #pragma once
 
+#include <type_traits>
+
 #include "device_math.hpp"
 #include "engine_internal.hpp"
 
@@ -32,6 +34,12 @@ struct SobolLane {
     bool lds_mode;
 };
 
+// LDS mode only (callers whose dimension never exceeds SOBOL_LDS_MAX_D).
+__device__ __forceinline__ uint32_t sobol_coord_lds(const SobolLane& s, int d) {
+    const uint32_t* r = s.rec + d * 20;
+    return r[s.ga] ^ r[8 + s.gb] ^ r[16 + s.w];
+}
+
 __device__ __forceinline__ uint32_t sobol_coord(const SobolLane& s, int d) {
     if (s.lds_mode) {
         const uint32_t* r = s.rec + d * 20;
@@ -41,6 +49,56 @@ __device__ __forceinline__ uint32_t sobol_coord(const SobolLane& s, int d) {
     uint32_t x = r[s.ga] ^ r[8 + s.gb];
     for (uint32_t g = s.hi_gray; g; g &= g - 1u) x ^= r[32 + __builtin_ctz(g)];
     return x;
+}
+
+// Words 16..19 of the LDS direction records for the 256-point tile n (points
+// 256 n .. 256 n + 255, below 2^32): the wave offsets' direction sums XOR the
+// sum of Lv[j] over the set bits of gray(256 n), bits 7..31.  `full`: form the
+// sum (its loads are independent); otherwise the records hold tile n - 1, and
+// gray(256 n) differs from gray(256 (n - 1)) in bits 7 and 8 + ctz(n) only, so
+// two direction numbers are XORed in.  Threads d = tid, tid + 256, ... < D.
+__device__ __forceinline__ void sobol_tile_words(uint32_t* rec, const uint32_t* grec, int D, uint32_t n, bool full,
+                                                 int tid) {
+    const uint32_t b32 = n << 8, gbase = b32 ^ (b32 >> 1);
+    for (int d = tid; d < D; d += 256) {
+        const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
+        uint4* rw = (uint4*)(rec + d * 20 + 16);
+        if (full) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int j = 7; j < 32; j++) s ^= ((gbase >> j) & 1u) ? gr[32 + j] : 0u;
+            *rw = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
+        } else {
+            const uint32_t dl = gr[32 + 7] ^ gr[32 + 8 + __builtin_ctz(n)];
+            const uint4 v = *rw;
+            *rw = make_uint4(v.x ^ dl, v.y ^ dl, v.z ^ dl, v.w ^ dl);
+        }
+    }
+}
+
+// A Brownian-bridge midpoint W_M from the ends of its interval and the normal of
+// coordinate j (LDS direction records: a bridge has at most SOBOL_LDS_MAX_D
+// dates): with bb_precompute's FP32 weights (wl, wr, sd), or on a uniform date
+// grid (UNI) with weights 1/2 and the sd of the depth, read from no table per
+// coordinate.  The path and Greeks kernels both build their bridges from these,
+// so the same points give the same paths.
+template <bool UNI>
+__device__ __forceinline__ float bb_mid(const PathLaunch& a, const SobolLane& sl, const float4* itab, float WL,
+                                        float WR, int j, int depth) {
+    if constexpr (UNI) {
+        const float sd = a.bb_sd[depth];
+        return 0.5f * (WL + WR) - icdf_u32_tab_scaled_neg(sobol_coord_lds(sl, j), itab, sd, -2.0f * sd);
+    } else {
+        const float4 c = ((const float4*)a.bbw)[j];
+        return c.x * WL + c.y * WR + c.z * icdf_u32_tab(sobol_coord_lds(sl, j), itab);
+    }
+}
+
+// W at the last date: sd sqrt(T) times coordinate 0's normal.
+template <bool UNI>
+__device__ __forceinline__ float bb_terminal(const PathLaunch& a, const SobolLane& sl, const float4* itab) {
+    if constexpr (UNI) return -icdf_u32_tab_scaled_neg(sobol_coord_lds(sl, 0), itab, a.bb_sd[0], -2.0f * a.bb_sd[0]);
+    else return ((const float4*)a.bbw)[0].z * icdf_u32_tab(sobol_coord_lds(sl, 0), itab);
 }
 
 template <int MK, int PK>
@@ -148,18 +206,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (base >= last) break;
         if constexpr (SOBOL) {
             const uint32_t b32 = (uint32_t)base;
-            const uint32_t gbase = b32 ^ (b32 >> 1);
             if (lds_mode) {
                 __syncthreads();
-                for (int d = tid; d < D; d += 256) {
-                    const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-                    uint32_t s = 0;
-                    for (uint32_t g = gbase; g; g &= g - 1u) s ^= gr[32 + __builtin_ctz(g)];
-                    rec[d * 20 + 16] = s ^ gr[16];
-                    rec[d * 20 + 17] = s ^ gr[17];
-                    rec[d * 20 + 18] = s ^ gr[18];
-                    rec[d * 20 + 19] = s ^ gr[19];
-                }
+                sobol_tile_words(rec, grec, D, (uint32_t)(tile0 + it), it == 0, tid);
                 __syncthreads();
             } else {
                 const uint32_t hi = b32 + (sl.w << 6);
@@ -211,49 +260,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     ps.step(a, icdf_u32_tab(sobol_coord(sl, k), itab) * a.sqrt_h, k);
             }
         } else {  // NZ_SOBOL_BB (one-factor models)
-            const float4* bw = (const float4*)a.bbw;
-            const int G = a.G;
+            // W_M from the ends of its interval and coordinate j's normal: with
+            // bb_precompute's weights, or on a uniform date grid (weights 1/2, the
+            // sd of the depth; no weight loads per coordinate).  A bridge has at
+            // most SOBOL_LDS_MAX_D dates, so its direction records are in LDS.
+            const int G = a.G, lgG = 31 - __builtin_clz(G);
             float* sk = skel + tid;
-            sk[0] = 0.0f;
-            sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
-            for (int j = 1; j < G; j++) {
-                const int lev = 31 - __builtin_clz(j);
-                const int q = j - (1 << lev);
-                const int st = G >> lev;
-                const int L = q * st, R = L + st, M = L + (st >> 1);
-                const float4 c = bw[j];
-                sk[M * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
-            }
-            if constexpr (LB == 0) {
-                float left = 0.0f;
-                for (int i = 0; i < G; i++) {
-                    const float right = sk[(i + 1) * 256];
-                    ps.step(a, right - left, i);
-                    left = right;
+            auto bridge = [&](auto uniform_tag) {
+                constexpr bool UNI = decltype(uniform_tag)::value;
+                auto mid = [&](float WL, float WR, int j, int depth) {
+                    return bb_mid<UNI>(a, sl, itab, WL, WR, j, depth);
+                };
+                sk[0] = 0.0f;
+                sk[G * 256] = bb_terminal<UNI>(a, sl, itab);
+                for (int j = 1; j < G; j++) {
+                    const int lev = 31 - __builtin_clz(j);
+                    const int q = j - (1 << lev);
+                    const int st = G >> lev;
+                    const int L = q * st, R = L + st, M = L + (st >> 1);
+                    sk[M * 256] = mid(sk[L * 256], sk[R * 256], j, lev + 1);
                 }
-            } else {
-                constexpr int B = 1 << LB;
-                float left = 0.0f;
-                for (int i = 0; i < G; i++) {
-                    float W[B + 1];
-                    W[0] = left;
-                    W[B] = sk[(i + 1) * 256];
-#pragma unroll
-                    for (int l = 0; l < LB; l++) {
-                        const int j0 = (G << l) + (i << l);
-#pragma unroll
-                        for (int q = 0; q < (1 << l); q++) {
-                            const int s = B >> l;
-                            const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                            const float4 c = bw[j0 + q];
-                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32_tab(sobol_coord(sl, j0 + q), itab);
-                        }
+                if constexpr (LB == 0) {
+                    float left = 0.0f;
+                    for (int i = 0; i < G; i++) {
+                        const float right = sk[(i + 1) * 256];
+                        ps.step(a, right - left, i);
+                        left = right;
                     }
+                } else {
+                    constexpr int B = 1 << LB;
+                    float left = 0.0f;
+                    for (int i = 0; i < G; i++) {
+                        float W[B + 1];
+                        W[0] = left;
+                        W[B] = sk[(i + 1) * 256];
 #pragma unroll
-                    for (int k = 0; k < B; k++) ps.step(a, W[k + 1] - W[k], i * B + k);
-                    left = W[B];
+                        for (int l = 0; l < LB; l++) {
+                            const int j0 = (G << l) + (i << l);
+#pragma unroll
+                            for (int q = 0; q < (1 << l); q++) {
+                                const int s = B >> l;
+                                const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
+                                W[Mp] = mid(W[Lp], W[Rp], j0 + q, lgG + l + 1);
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < B; k++) ps.step(a, W[k + 1] - W[k], i * B + k);
+                        left = W[B];
+                    }
                 }
-            }
+            };
+            if (a.bb_uniform) bridge(std::true_type{});
+            else bridge(std::false_type{});
         }
 
         float ym, yc;

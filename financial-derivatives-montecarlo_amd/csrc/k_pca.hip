@@ -55,21 +55,13 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
     sl.rec = rec; sl.grec = grec; sl.ga = g & 7u; sl.gb = g >> 3; sl.w = (uint32_t)(tid >> 6);
     sl.hi_gray = 0; sl.lds_mode = lds_mode;
     const int64_t base = ((first >> 8) + blockIdx.x) << 8;
-    const uint32_t b32 = (uint32_t)base, gbase = b32 ^ (b32 >> 1);
+    const uint32_t b32 = (uint32_t)base;
     if (lds_mode) {
         for (int i = tid; i < D * 16; i += 256) {
             const int d = i >> 4, k = i & 15;
             rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
         }
-        for (int d = tid; d < D; d += 256) {
-            const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-            uint32_t s = 0;
-            for (uint32_t x = gbase; x; x &= x - 1u) s ^= gr[32 + __builtin_ctz(x)];
-            rec[d * 20 + 16] = s ^ gr[16];
-            rec[d * 20 + 17] = s ^ gr[17];
-            rec[d * 20 + 18] = s ^ gr[18];
-            rec[d * 20 + 19] = s ^ gr[19];
-        }
+        sobol_tile_words(rec, grec, D, (uint32_t)(base >> 8), true, tid);
         __syncthreads();
     } else {
         const uint32_t hi = b32 + (sl.w << 6);

@@ -40,6 +40,8 @@ struct LegParams {
 //  g = gray(lane).
 constexpr int SDIR_WORDS = 64;
 
+constexpr int BB_SD_DEPTHS = 12;   // a bridge of at most 1024 steps has 11 depths
+
 struct PathLaunch {
     LegParams leg[2];       // main, control
     float h, sqrt_h, em;    // step, sqrt(step), e^{-kappa h}
@@ -47,10 +49,12 @@ struct PathLaunch {
     float beta_cv, E_ctrl;
     float T;                // maturity (rho of the discount)
     int n_steps, nf, D;
-    int G, log2_G;          // BB: coarse blocks (N / 16, or N when N < 16)
+    int G;                  // BB: coarse blocks (N / 16, or N when N < 16)
     uint32_t seed, tag;     // PRNG stream key
     const uint32_t* sdir;   // [R][D][SDIR_WORDS]
     const float* bbw;       // BB weights [N][4] = {wl, wr, sd, 0} in bb_precompute order
+    int bb_uniform;         // BB on a uniform date grid: every weight 1/2 in FP32, one sd per depth
+    float bb_sd[BB_SD_DEPTHS];  // then: [0] terminal sd, [k] sd of the midpoints of schedule level k - 1
     const float* L;         // basket: n x n lower Cholesky (row-major) or null
     const float* rows;      // NZ_ROWS: precomputed increments [R * count][ld] (PCA)
     int ld;
