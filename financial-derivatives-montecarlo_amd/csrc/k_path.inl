@@ -173,7 +173,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     uint32_t* rec = (uint32_t*)smem;
     const int rec_words = lds_mode ? D * 20 : 0;
     float* skel = (float*)(smem + (size_t)rec_words * 4);
-    const int skel_words = (NZ == NZ_SOBOL_BB) ? (a.G + 1) * 256 : 0;
+    // the bridge skeleton (dates < 16) or the walk stack of the block boundaries
+    const int skel_words = (NZ == NZ_SOBOL_BB) ? (LB == 0 ? a.G + 1 : BB_WALK_DEPTH) * 256 : 0;
     double* red = (double*)(smem + (((size_t)(rec_words + skel_words) * 4 + 15) & ~(size_t)15));
 
     const uint32_t* grec = SOBOL ? a.sdir + (int64_t)r * D * SDIR_WORDS : nullptr;
@@ -265,22 +266,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             // sd of the depth; no weight loads per coordinate).  A bridge has at
             // most SOBOL_LDS_MAX_D dates, so its direction records are in LDS.
             const int G = a.G, lgG = 31 - __builtin_clz(G);
-            float* sk = skel + tid;
             auto bridge = [&](auto uniform_tag) {
                 constexpr bool UNI = decltype(uniform_tag)::value;
                 auto mid = [&](float WL, float WR, int j, int depth) {
                     return bb_mid<UNI>(a, sl, itab, WL, WR, j, depth);
                 };
-                sk[0] = 0.0f;
-                sk[G * 256] = bb_terminal<UNI>(a, sl, itab);
-                for (int j = 1; j < G; j++) {
-                    const int lev = 31 - __builtin_clz(j);
-                    const int q = j - (1 << lev);
-                    const int st = G >> lev;
-                    const int L = q * st, R = L + st, M = L + (st >> 1);
-                    sk[M * 256] = mid(sk[L * 256], sk[R * 256], j, lev + 1);
-                }
                 if constexpr (LB == 0) {
+                    // fewer than 16 dates: the whole skeleton in LDS, one step per entry
+                    float* sk = skel + tid;
+                    sk[0] = 0.0f;
+                    sk[G * 256] = bb_terminal<UNI>(a, sl, itab);
+                    for (int j = 1; j < G; j++) {
+                        const int lev = 31 - __builtin_clz(j);
+                        const int q = j - (1 << lev);
+                        const int st = G >> lev;
+                        const int L = q * st, R = L + st, M = L + (st >> 1);
+                        sk[M * 256] = mid(sk[L * 256], sk[R * 256], j, lev + 1);
+                    }
                     float left = 0.0f;
                     for (int i = 0; i < G; i++) {
                         const float right = sk[(i + 1) * 256];
@@ -288,12 +290,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         left = right;
                     }
                 } else {
+                    // G = 2^m blocks of B dates.  The skeleton (W at the block
+                    // boundaries) is walked left to right instead of stored: rt[k]
+                    // (a per-lane stack in LDS, k <= m) holds W at the right end of
+                    // the aligned 2^k-block interval that contains the current
+                    // boundary; moving to boundary i makes the intervals of size
+                    // <= 2^ctz(i) start there, and their new right ends are bridge
+                    // midpoints generated at that moment, from the same operands as
+                    // the top-down skeleton.
                     constexpr int B = 1 << LB;
-                    float left = 0.0f;
+                    const int m = lgG;
+                    float* rt = skel + tid;              // rt[k * 256]
+                    // the midpoint of [L, L + 2^(k+1)) blocks: skeleton level m-1-k
+                    auto node = [&](int k, int L, float WL, float WR) {
+                        const int lev = m - 1 - k;
+                        return mid(WL, WR, (1 << lev) + (L >> (k + 1)), lev + 1);
+                    };
+                    float Wp = 0.0f;
+                    rt[m * 256] = bb_terminal<UNI>(a, sl, itab);
+                    for (int k = m - 1; k >= 0; k--) rt[k * 256] = node(k, 0, Wp, rt[(k + 1) * 256]);
                     for (int i = 0; i < G; i++) {
+                        if (i > 0) {
+                            Wp = rt[0];
+                            const int c = __builtin_ctz(i);
+                            rt[c * 256] = rt[(c + 1) * 256];
+                            for (int k = c - 1; k >= 0; k--) rt[k * 256] = node(k, i, Wp, rt[(k + 1) * 256]);
+                        }
                         float W[B + 1];
-                        W[0] = left;
-                        W[B] = sk[(i + 1) * 256];
+                        W[0] = Wp;
+                        W[B] = rt[0];
 #pragma unroll
                         for (int l = 0; l < LB; l++) {
                             const int j0 = (G << l) + (i << l);
@@ -306,7 +331,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         }
 #pragma unroll
                         for (int k = 0; k < B; k++) ps.step(a, W[k + 1] - W[k], i * B + k);
-                        left = W[B];
                     }
                 }
             };

@@ -41,6 +41,7 @@ struct LegParams {
 constexpr int SDIR_WORDS = 64;
 
 constexpr int BB_SD_DEPTHS = 12;   // a bridge of at most 1024 steps has 11 depths
+constexpr int BB_WALK_DEPTH = 7;   // skeleton walk of at most 64 16-date blocks: W at 2^0..2^6 blocks
 
 struct PathLaunch {
     LegParams leg[2];       // main, control
@@ -121,7 +122,7 @@ constexpr int SOBOL_LDS_MAX_D = 1024;
 inline size_t path_lds_bytes(int nz, int D, int G) {
     size_t b = 0;
     if (nz != NZ_PRNG && nz != NZ_ROWS && D <= SOBOL_LDS_MAX_D) b += (size_t)D * 20 * 4;
-    if (nz == NZ_SOBOL_BB) b += (size_t)(G + 1) * 256 * 4;
+    if (nz == NZ_SOBOL_BB) b += (size_t)(D < 16 ? G + 1 : BB_WALK_DEPTH) * 256 * 4;   // skeleton or walk stack
     b = (b + 15) & ~(size_t)15;
     return b + 4 * 7 * sizeof(double);
 }
