@@ -16,6 +16,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "engine_host.hpp"
@@ -594,11 +595,14 @@ struct MultiSpec {
     const fdmc_payoff* payoff = nullptr;
     int M = 2;
     bool qmc = false;                 // Sobol (MLQMC) or Philox (MLMC)
-    int mode = FDMC_NOISE_RAW;        // MLQMC: RAW or BB
+    int mode = FDMC_NOISE_RAW;        // MLQMC: RAW, BB or PCA
     int R = 1;                        // replicas per level (MLQMC)
     uint32_t salt_base = 42;          // MLQMC salt = salt_base + l R + r; MLMC: PRNG seed
     const fdmc_bb* const* bb_list = nullptr;
     int n_bb = 0;
+    const fdmc_pca* const* pca_list = nullptr;
+    int n_pca = 0;
+    bool pca_bf16 = false;            // PCA GEMM operands: FP16 (as the reference) or BF16
 };
 
 // Scrambled records of all R replicas of level l (cached per context for the
@@ -619,14 +623,141 @@ const uint32_t* level_sdir(fdmc_context* c, std::map<int, DevBuf>& cache, const 
     return (const uint32_t*)b.p;
 }
 
-// Runs every request in one persistent launch per noise kind; out[i] receives
-// {sum dY, sum dY^2, sum Yf, sum Yf^2} of request i.
+// One persistent multilevel launch over `ent` (any order; sorted here longest
+// first); entry i's {sum dY, sum dY^2, sum Yf, sum Yf^2} go to out[which[i]].
+void launch_levels(fdmc_context* c, const MultiSpec& ms, int nz, const std::vector<LevelEntry>& ent,
+                   const std::vector<int>& which, std::vector<std::array<double, 4>>& out) {
+    const fdmc_model& m = *ms.model;
+    const fdmc_payoff& p = *ms.payoff;
+    const int nf = noise_dim(m);
+    const int mk = model_kind_dev(m), pk = payoff_kind_dev(p);
+    std::vector<int> order(ent.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ent[x].D > ent[y].D; });
+    std::vector<LevelEntry> sorted;
+    std::vector<int> sw;
+    int total = 0, max_D = 0, max_G = 0;
+    for (int o : order) {
+        LevelEntry e = ent[o];
+        const int64_t t0 = e.first >> 8, t1 = (e.first + e.count - 1) >> 8;
+        const int64_t nt = t1 - t0 + 1;
+        if (nt > (1LL << 30)) fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
+        // about 512 noise dimensions of work per lane and item, and at least
+        // 64 items per entry where it has that many tiles
+        e.tpi = (int)std::max<int64_t>(1, std::min<int64_t>(512 / std::max(e.D, 1), nt / 64));
+        const int64_t items = (nt + e.tpi - 1) / e.tpi;
+        if (total + items > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
+        e.tile_begin = total;
+        e.ntiles = (int)items;
+        total += e.ntiles;
+        max_D = std::max(max_D, e.D);
+        max_G = std::max(max_G, e.G);
+        sorted.push_back(e);
+        sw.push_back(which[o]);
+    }
+    if ((nz == NZ_SOBOL_BB || nz == NZ_ROWS) && nf != 1) fail(FDMC_E_UNSUPPORTED, "BB/PCA need a one-factor model");
+    const void* fn = nz == NZ_PRNG ? levels_kernel_prng(mk, pk)
+                     : nz == NZ_SOBOL_BB ? levels_kernel_bb(mk, pk)
+                     : nz == NZ_ROWS ? levels_kernel_rows(mk, pk) : levels_kernel_raw(mk, pk);
+    if (!fn) fail(FDMC_E_UNSUPPORTED, "model/payoff combination not supported by the multilevel kernel");
+
+    const size_t ent_bytes = sorted.size() * sizeof(LevelEntry);
+    const size_t part_bytes = (size_t)total * 4 * sizeof(double);
+    const size_t sums_bytes = sorted.size() * 4 * sizeof(double);
+    char* scratch = (char*)c->partials.ensure(part_bytes + ent_bytes + sums_bytes + 256);
+    double* d_part = (double*)scratch;
+    LevelEntry* d_ent = (LevelEntry*)(scratch + ((part_bytes + 63) & ~(size_t)63));
+    double* d_out = (double*)((char*)d_ent + ((ent_bytes + 63) & ~(size_t)63));
+    unsigned int* d_queue = (unsigned int*)c->sums.ensure(64);
+    HIPC(hipMemcpyAsync(d_ent, sorted.data(), ent_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPC(launch_zero_u32(d_queue, 1, c->stream));
+
+    LevelsLaunch a{};
+    a.leg = make_leg(m, p);
+    a.M = ms.M;
+    a.nf = nf;
+    a.n_entries = (int)sorted.size();
+    a.total_items = total;
+    a.entries = d_ent;
+    a.queue = d_queue;
+    a.partials = d_part;
+    const size_t lds = levels_lds_bytes(nz, max_D, max_G);
+    if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "multilevel LDS footprint too large");
+    if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int grid = (int)std::min<int64_t>(total, (int64_t)c->n_cu * 8);
+    void* args[] = {(void*)&a, (void*)&max_D, (void*)&max_G};
+    if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
+    HIPC(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, c->stream));
+    if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
+    HIPC(launch_reduce_entries(d_part, d_ent, a.n_entries, 4, d_out, c->stream));
+    ensure_host_sums(c, sorted.size() * 4);
+    HIPC(hipMemcpyAsync(c->h_sums, d_out, sums_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        float ms_ = 0.0f;
+        HIPC(hipEventElapsedTime(&ms_, c->ev0, c->ev1));
+        c->timed_ms += ms_;
+        c->timed_launches++;
+    }
+    for (size_t i = 0; i < sorted.size(); i++)
+        for (int k = 0; k < 4; k++) out[sw[i]][k] = c->h_sums[i * 4 + k];
+}
+
+// PCA requests (methods_cuda.cu:1907-1913): the normals of consecutive
+// replicas of one level over the same point range go through one GEMM (FP16 or
+// BF16 operands, FP32 accumulation) into the context's row buffer, then one
+// multilevel launch reads the fine increments from those rows.
+void run_levels_pca(fdmc_context* c, const MultiSpec& ms, const std::vector<LevelReq>& reqs,
+                    const std::vector<LevelEntry>& ent, const std::vector<int>& which,
+                    std::vector<std::array<double, 4>>& out) {
+    std::vector<int> idx(ent.size());
+    for (size_t i = 0; i < idx.size(); i++) idx[i] = (int)i;
+    auto key = [&](int i) {
+        const LevelReq& q = reqs[which[i]];
+        return std::make_tuple(q.level, q.first, q.count, q.replica);
+    };
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return key(x) < key(y); });
+    size_t i = 0;
+    while (i < idx.size()) {
+        const LevelEntry& e0 = ent[idx[i]];
+        const LevelReq& q0 = reqs[which[idx[i]]];
+        const fdmc_pca* pca = ms.pca_list[q0.level];
+        // Z (2 B) and dW (4 B) per padded element: groups of at most 8 GiB
+        const double per = (double)q0.count * pca->mp * 6.0;
+        if (per > 16.0 * (1 << 30)) fail(FDMC_E_RANGE, "PCA level pass above 16 GiB of buffers: lower the sample count");
+        const size_t gmax = (size_t)std::max(1.0, 8.0 * (1 << 30) / per);
+        size_t j = i + 1;
+        while (j < idx.size() && j - i < gmax) {
+            const LevelReq& q = reqs[which[idx[j]]];
+            if (q.level != q0.level || q.first != q0.first || q.count != q0.count ||
+                q.replica != q0.replica + (int)(j - i))
+                break;
+            j++;
+        }
+        const int g = (int)(j - i);
+        const int ld = pca_rows(c, pca, e0.sdir, e0.D, g, q0.first, q0.count, ms.pca_bf16);
+        std::vector<LevelEntry> ge;
+        std::vector<int> gw;
+        for (size_t k = i; k < j; k++) {
+            LevelEntry e = ent[idx[k]];
+            e.rows = (const float*)c->pca_dw.p + (size_t)(k - i) * (size_t)q0.count * (size_t)ld;
+            e.ld = ld;
+            ge.push_back(e);
+            gw.push_back(which[idx[k]]);
+        }
+        launch_levels(c, ms, NZ_ROWS, ge, gw, out);
+        i = j;
+    }
+}
+
+// Runs every request, one persistent launch per noise kind (PCA: per level
+// and replica group); out[i] receives {sum dY, sum dY^2, sum Yf, sum Yf^2} of
+// request i.
 void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cache,
                 const std::vector<LevelReq>& reqs, std::vector<std::array<double, 4>>& out) {
     const fdmc_model& m = *ms.model;
     const fdmc_payoff& p = *ms.payoff;
     const int nf = noise_dim(m);
-    const int mk = model_kind_dev(m), pk = payoff_kind_dev(p);
     out.assign(reqs.size(), {0.0, 0.0, 0.0, 0.0});
     // split by noise kind: Sobol (D <= 20000) and pseudo-random (MLMC or QMC fallback)
     for (int pass = 0; pass < 2; pass++) {
@@ -671,83 +802,25 @@ void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cac
                     if (nfine > SOBOL_LDS_MAX_D) fail(FDMC_E_UNSUPPORTED, "BB supports at most 1024 steps");
                     e.bbw = b->d_w;
                     e.G = nfine >= 16 ? nfine / 16 : nfine;
+                } else if (ms.mode == FDMC_NOISE_PCA) {  // methods_cuda.cu:1781-1789
+                    if (q.level >= ms.n_pca || !ms.pca_list[q.level])
+                        fail(FDMC_E_INVALID, "missing pca_list[" + std::to_string(q.level) + "]");
+                    const fdmc_pca* pc = ms.pca_list[q.level];
+                    if (pc->m != nfine)
+                        fail(FDMC_E_INVALID, "pca_list[" + std::to_string(q.level) + "] has m=" +
+                                                 std::to_string(pc->m) + ", expected " + std::to_string(nfine));
                 }
             }
             ent.push_back(e);
             which.push_back((int)i);
         }
         if (ent.empty()) continue;
-        // longest paths first
-        std::vector<int> order(ent.size());
-        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ent[x].D > ent[y].D; });
-        std::vector<LevelEntry> sorted;
-        std::vector<int> sw;
-        int total = 0, max_D = 0, max_G = 0;
-        for (int o : order) {
-            LevelEntry e = ent[o];
-            const int64_t t0 = e.first >> 8, t1 = (e.first + e.count - 1) >> 8;
-            const int64_t nt = t1 - t0 + 1;
-            if (nt > (1LL << 30)) fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
-            // about 512 noise dimensions of work per lane and item, and at least
-            // 64 items per entry where it has that many tiles
-            e.tpi = (int)std::max<int64_t>(1, std::min<int64_t>(512 / std::max(e.D, 1), nt / 64));
-            const int64_t items = (nt + e.tpi - 1) / e.tpi;
-            if (total + items > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many tiles in one multilevel launch");
-            e.tile_begin = total;
-            e.ntiles = (int)items;
-            total += e.ntiles;
-            max_D = std::max(max_D, e.D);
-            max_G = std::max(max_G, e.G);
-            sorted.push_back(e);
-            sw.push_back(which[o]);
+        if (pass == 0 && ms.mode == FDMC_NOISE_PCA) {
+            run_levels_pca(c, ms, reqs, ent, which, out);
+            continue;
         }
         const int nz = pass == 1 ? NZ_PRNG : (ms.mode == FDMC_NOISE_BB ? NZ_SOBOL_BB : NZ_SOBOL_RAW);
-        if (nz == NZ_SOBOL_BB && nf != 1) fail(FDMC_E_UNSUPPORTED, "BB needs a one-factor model");
-        const void* fn = nz == NZ_PRNG ? levels_kernel_prng(mk, pk)
-                         : nz == NZ_SOBOL_BB ? levels_kernel_bb(mk, pk) : levels_kernel_raw(mk, pk);
-        if (!fn) fail(FDMC_E_UNSUPPORTED, "model/payoff combination not supported by the multilevel kernel");
-
-        const size_t ent_bytes = sorted.size() * sizeof(LevelEntry);
-        const size_t part_bytes = (size_t)total * 4 * sizeof(double);
-        const size_t sums_bytes = sorted.size() * 4 * sizeof(double);
-        char* scratch = (char*)c->partials.ensure(part_bytes + ent_bytes + sums_bytes + 256);
-        double* d_part = (double*)scratch;
-        LevelEntry* d_ent = (LevelEntry*)(scratch + ((part_bytes + 63) & ~(size_t)63));
-        double* d_out = (double*)((char*)d_ent + ((ent_bytes + 63) & ~(size_t)63));
-        unsigned int* d_queue = (unsigned int*)c->sums.ensure(64);
-        HIPC(hipMemcpyAsync(d_ent, sorted.data(), ent_bytes, hipMemcpyHostToDevice, c->stream));
-        HIPC(launch_zero_u32(d_queue, 1, c->stream));
-
-        LevelsLaunch a{};
-        a.leg = make_leg(m, p);
-        a.M = ms.M;
-        a.nf = nf;
-        a.n_entries = (int)sorted.size();
-        a.total_items = total;
-        a.entries = d_ent;
-        a.queue = d_queue;
-        a.partials = d_part;
-        const size_t lds = levels_lds_bytes(nz, max_D, max_G);
-        if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "multilevel LDS footprint too large");
-        if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        const int grid = (int)std::min<int64_t>(total, (int64_t)c->n_cu * 8);
-        void* args[] = {(void*)&a, (void*)&max_D, (void*)&max_G};
-        if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
-        HIPC(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, c->stream));
-        if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
-        HIPC(launch_reduce_entries(d_part, d_ent, a.n_entries, 4, d_out, c->stream));
-        ensure_host_sums(c, sorted.size() * 4);
-        HIPC(hipMemcpyAsync(c->h_sums, d_out, sums_bytes, hipMemcpyDeviceToHost, c->stream));
-        HIPC(hipStreamSynchronize(c->stream));
-        if (c->timing) {
-            float ms_ = 0.0f;
-            HIPC(hipEventElapsedTime(&ms_, c->ev0, c->ev1));
-            c->timed_ms += ms_;
-            c->timed_launches++;
-        }
-        for (size_t i = 0; i < sorted.size(); i++)
-            for (int k = 0; k < 4; k++) out[sw[i]][k] = c->h_sums[i * 4 + k];
+        launch_levels(c, ms, nz, ent, which, out);
     }
 }
 
@@ -1371,15 +1444,21 @@ fdmc_status fdmc_levels_accumulate(fdmc_context* c, const fdmc_model* model, con
                                    int32_t M, int32_t qmc, int32_t mode, int32_t R, uint32_t salt_or_seed,
                                    int32_t n_req, const int32_t* levels, const int32_t* replicas,
                                    const int64_t* first, const int64_t* count,
-                                   const fdmc_bb* const* bb_list, int32_t n_bb, double* sums_out) {
+                                   const fdmc_bb* const* bb_list, int32_t n_bb,
+                                   const fdmc_pca* const* pca_list, int32_t n_pca, double* sums_out) {
     return guarded([&] {
         if (!c || !levels || !first || !count || !sums_out || n_req <= 0) fail(FDMC_E_INVALID, "null argument");
         validate_model(model);
         validate_payoff(payoff);
         if (is_basket(*model)) fail(FDMC_E_UNSUPPORTED, "basket models are not supported by the multilevel kernel");
+        const bool bf16 = (mode & FDMC_PCA_BF16) != 0;
+        mode &= ~FDMC_PCA_BF16;
+        if (mode != FDMC_NOISE_RAW && mode != FDMC_NOISE_BB && mode != FDMC_NOISE_PCA)
+            fail(FDMC_E_INVALID, "unknown noise mode");
         MultiSpec ms;
         ms.model = model; ms.payoff = payoff; ms.M = M; ms.qmc = qmc != 0; ms.mode = mode; ms.R = std::max(R, 1);
         ms.salt_base = salt_or_seed; ms.bb_list = bb_list; ms.n_bb = n_bb;
+        ms.pca_list = pca_list; ms.n_pca = n_pca; ms.pca_bf16 = bf16;
         if (M < 2) fail(FDMC_E_INVALID, "M must be >= 2");
         std::vector<LevelReq> rq;
         for (int i = 0; i < n_req; i++) {
@@ -1409,13 +1488,10 @@ fdmc_status fdmc_run_mlqmc(fdmc_context* c, const fdmc_model* model, const fdmc_
             fail(FDMC_E_UNSUPPORTED, "run_mlqmc: basket models are not supported");
         if (mode != FDMC_NOISE_RAW && noise_dim(*model) != 1)  // methods_cuda.cu:1702-1704
             fail(FDMC_E_UNSUPPORTED, "run_mlqmc: Brownian bridge/PCA need a one-factor model");
-        if (mode == FDMC_NOISE_PCA) {
-            (void)pcal; (void)n_pca;
-            fail(FDMC_E_UNSUPPORTED, "run_mlqmc: PCA noise mode is not available yet");
-        }
         MultiSpec ms;
         ms.model = model; ms.payoff = payoff; ms.M = ml->M; ms.qmc = true; ms.mode = mode; ms.R = qc->R;
         ms.salt_base = qc->salt_base; ms.bb_list = bbl; ms.n_bb = n_bb;
+        ms.pca_list = pcal; ms.n_pca = n_pca; ms.pca_bf16 = qc->pca_precision != 0;
         mlqmc_loop(c, ms, eps, ml->max_L, ml->pilot_n, out);
     });
 }

@@ -26,6 +26,7 @@ hipError_t launch_zero_u32(unsigned int* p, int n, hipStream_t st);
 const void* levels_kernel_raw(int mk, int pk);
 const void* levels_kernel_bb(int mk, int pk);
 const void* levels_kernel_prng(int mk, int pk);
+const void* levels_kernel_rows(int mk, int pk);
 
 // k_greeks_{raw,bb,prng}.hip: price + adjoint Greeks kernel, or null.
 const void* greeks_kernel_raw(int mk, int pk, int cv, int cpk, int lb);

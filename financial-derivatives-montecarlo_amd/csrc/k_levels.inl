@@ -69,9 +69,10 @@ template <int MK, int PK, int NZ, int LB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_levels(LevelsLaunch a, int max_D, int max_G) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63;
-    const bool lds_ok = (NZ != NZ_PRNG) && max_D <= SOBOL_LDS_MAX_D;
+    constexpr bool SOBOL = NZ == NZ_SOBOL_RAW || NZ == NZ_SOBOL_BB;
+    const bool lds_ok = SOBOL && max_D <= SOBOL_LDS_MAX_D;
     // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
-    constexpr bool ITAB = NZ != NZ_PRNG;
+    constexpr bool ITAB = SOBOL;
     __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
     if constexpr (ITAB) {
         icdf_tab_load(itab, threadIdx.x, 256);
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const int64_t idx = base + tid;
         const bool valid = idx >= E.first && idx < E.first + E.count;
 
-        if constexpr (NZ != NZ_PRNG) {
+        if constexpr (SOBOL) {
             sl.grec = E.sdir;
             const uint32_t b32 = (uint32_t)base;
             if (lds_ok) {
@@ -146,7 +147,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         Coupled<MK, PK> cp;
         cp.init(P);
         const int M = a.M, nfine = E.n_fine;
-        if constexpr (NZ == NZ_PRNG) {
+        if constexpr (NZ == NZ_ROWS) {
+            // fine increments precomputed by the PCA GEMM (methods_cuda.cu:1907-1913)
+            const float* src = E.rows + (int64_t)(valid ? idx - E.first : 0) * E.ld;
+            for (int k = 0; k < nfine; k++) cp.fine1(P, E, src[k], k, M);
+        } else if constexpr (NZ == NZ_PRNG) {
             float z[4];
             if constexpr (MK == MK_HESTON) {
                 for (int b = 0; 2 * b < nfine; b++) {

@@ -1,0 +1,17 @@
+// k_levels_rows.hip -- instantiations of the multilevel kernel for fine
+// increments precomputed by the PCA GEMM (NZ_ROWS, one-factor models).
+#include "k_levels.inl"
+
+namespace fdmc {
+
+const void* levels_kernel_rows(int mk, int pk) {
+#define X(MKv, PKv) \
+    if (mk == MKv && pk == PKv) return (const void*)&k_levels<MKv, PKv, NZ_ROWS, 0>;
+#define XP(MKv) X(MKv, PK_EUROPEAN) X(MKv, PK_ASIAN) X(MKv, PK_GEOM_ASIAN) X(MKv, PK_LOOKBACK) X(MKv, PK_BARRIER)
+    XP(MK_GBM) XP(MK_DUPIRE) XP(MK_LVQ)
+#undef XP
+#undef X
+    return nullptr;
+}
+
+}  // namespace fdmc

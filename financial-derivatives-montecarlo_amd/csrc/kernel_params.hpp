@@ -75,10 +75,12 @@ struct LevelEntry {
     int level, n_fine, n_coarse, D, G;
     int tile_begin, ntiles, tpi;   // first work item, work items, 256-sample tiles per item
     int64_t first, count;   // sample (PRNG path id / Sobol point index) range
-    float h_f, h_c, sqrt_hf, em_f, em_c, bgk_f, bgk_c, pad1;
+    float h_f, h_c, sqrt_hf, em_f, em_c, bgk_f, bgk_c;
+    int ld;                 // row stride of `rows`
     uint32_t seed, tag;     // PRNG stream (MLMC)
     const uint32_t* sdir;   // Sobol records of this (level, replica), or null
     const float* bbw;       // BB weights for N = n_fine, or null
+    const float* rows;      // NZ_ROWS: fine increments of sample first + i at rows[i * ld], or null
 };
 
 struct LevelsLaunch {
@@ -142,7 +144,7 @@ inline size_t greeks_lds_bytes(int nz, int D, int G, int n_steps) {
 // Dynamic LDS bytes of one k_levels launch (max over its entries).
 inline size_t levels_lds_bytes(int nz, int max_D, int max_G) {
     size_t b = 0;
-    if (nz != NZ_PRNG && max_D <= SOBOL_LDS_MAX_D) b += (size_t)max_D * 20 * 4;
+    if (nz != NZ_PRNG && nz != NZ_ROWS && max_D <= SOBOL_LDS_MAX_D) b += (size_t)max_D * 20 * 4;
     if (nz == NZ_SOBOL_BB) b += (size_t)(max_G + 1) * 256 * 4;
     b = (b + 15) & ~(size_t)15;
     return b + 4 * 4 * sizeof(double) + 16;

@@ -409,7 +409,8 @@ _SIGS = {
     "fdmc_levels_accumulate": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_int32, C.c_int32,
                                          C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
-                                         C.POINTER(_P), C.c_int32, C.POINTER(C.c_double)]),
+                                         C.POINTER(_P), C.c_int32, C.POINTER(_P), C.c_int32,
+                                         C.POINTER(C.c_double)]),
     "fdmc_greeks_accumulate": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.POINTER(CModel),
                                          C.POINTER(CPayoff), C.c_double, C.c_double, C.c_int32, C.c_int32,
                                          C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P,
@@ -619,18 +620,21 @@ class Engine:
         return [[sums[r * na + i] for i in range(na)] for r in range(R)]
 
     def levels_accumulate(self, model, payoff, M, reqs, qmc=False, mode=NOISE_RAW, R=1, salt_or_seed=42,
-                          bb_list=()):
-        """reqs: list of (level, replica, first, count); returns n x 4 sums."""
+                          bb_list=(), pca_list=()):
+        """reqs: list of (level, replica, first, count); returns n x 4 sums.
+        mode may carry PCA_BF16 with NOISE_PCA."""
         n = len(reqs)
         lv = (C.c_int32 * n)(*[q[0] for q in reqs])
         rp = (C.c_int32 * n)(*[q[1] for q in reqs])
         fi = (C.c_int64 * n)(*[q[2] for q in reqs])
         co = (C.c_int64 * n)(*[q[3] for q in reqs])
         bbs = (_P * max(1, len(bb_list)))(*[x.handle if x else None for x in bb_list])
+        pcs = (_P * max(1, len(pca_list)))(*[x.handle if x else None for x in pca_list])
         out = (C.c_double * (4 * n))()
         m, p = model.to_c(), payoff.to_c()
         _check(self.lib, self.lib.fdmc_levels_accumulate(self.ctx, C.byref(m), C.byref(p), M, 1 if qmc else 0, mode,
-                                                         R, salt_or_seed, n, lv, rp, fi, co, bbs, len(bb_list), out))
+                                                         R, salt_or_seed, n, lv, rp, fi, co, bbs, len(bb_list),
+                                                         pcs, len(pca_list), out))
         return [[out[4 * i + k] for k in range(4)] for i in range(n)]
 
     def greeks_accumulate(self, model, payoff, n_steps, R, salt_or_seed, offset, n_points, mode=NOISE_RAW,

@@ -300,14 +300,18 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* ctx, const fdmc_model* model,
  * and 1800-1924) for n_req (level, replica, sample range) requests, run as one
  * persistent launch.  qmc = 0: Philox paths (seed = salt_or_seed, stream
  * 0x100 + level); qmc = 1: Sobol points of salt salt_or_seed + level*R + replica
- * (mode RAW or BB with bb_list[level]).  sums_out: n_req x {sum dY, sum dY^2,
- * sum Yf, sum Yf^2} (methods_cuda.cu:514-517). */
+ * (mode RAW, BB with bb_list[level], or PCA with pca_list[level], m = M^level,
+ * optionally | FDMC_PCA_BF16; PCA runs one GEMM and one launch per level and
+ * group of consecutive replicas with equal ranges).  sums_out: n_req x
+ * {sum dY, sum dY^2, sum Yf, sum Yf^2} (methods_cuda.cu:514-517). */
 fdmc_status fdmc_levels_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                    const fdmc_payoff* payoff, int32_t M, int32_t qmc,
                                    int32_t mode, int32_t R, uint32_t salt_or_seed, int32_t n_req,
                                    const int32_t* levels, const int32_t* replicas,
                                    const int64_t* first, const int64_t* count,
-                                   const fdmc_bb* const* bb_list, int32_t n_bb, double* sums_out);
+                                   const fdmc_bb* const* bb_list, int32_t n_bb,
+                                   const fdmc_pca* const* pca_list, int32_t n_pca,
+                                   double* sums_out);
 
 /* ---- adjoint Greeks (additive; the reference prices only) ----
  * Fused price + reverse-mode pathwise Greeks for one-factor GBM / Dupire /

@@ -150,10 +150,12 @@ int orc_run_mlmc(const fdmc_model* model, const fdmc_payoff* payoff, double eps,
                  const fdmc_mlmc_config* cfg, fdmc_result* out);
 int orc_mlqmc_level_sums(const fdmc_model* model, const fdmc_payoff* payoff, int M, int l,
                          int R, uint32_t salt_base, int64_t offset, int64_t n, int mode,
-                         const orc_bb* bb, double* sums /* R x 4 */);
+                         const orc_bb* bb, const double* pca_M /* column-major, PCA mode */,
+                         double* sums /* R x 4 */);
 int orc_run_mlqmc(const fdmc_model* model, const fdmc_payoff* payoff, double eps,
                   const fdmc_mlmc_config* ml, const fdmc_qmc_config* qc, int mode,
-                  const orc_bb* bbs, int n_bb, fdmc_result* out);
+                  const orc_bb* bbs, int n_bb, const double* const* pcas, int n_pca,
+                  fdmc_result* out);
 int orc_cv_pilot(const fdmc_model* main_model, const fdmc_model* ctrl_model,
                  const fdmc_payoff* main_payoff, const fdmc_payoff* ctrl_payoff,
                  double E_ctrl, int n_steps, int N_pilot, uint32_t seed,

@@ -342,12 +342,20 @@ static int level_steps(int M, int l) {
 }
 
 static void coupled_sums_from_noise(const orc_params* P, int nf, int nfine, int M, int mode,
-                                    const orc_bb* bb, float* z, float* dW, double* s) {
+                                    const orc_bb* bb, const double* pca_M, float* z, float* dW,
+                                    double* s) {
     int D = nfine * nf;
     const float* in = z;
     if (P->model_kind != FDMC_MODEL_HESTON) {
         if (mode == FDMC_NOISE_BB) {
             orc_bb_apply_f32(bb, z, dW);
+        } else if (mode == FDMC_NOISE_PCA) {
+            /* phase1_pca's dW = M z in FP64 (methods_cuda.cu:1907-1913), rounded to FP32 */
+            for (int i = 0; i < D; i++) {
+                double a = 0.0;
+                for (int j = 0; j < D; j++) a += pca_M[(size_t)j * D + i] * (double)z[j];
+                dW[i] = (float)a;
+            }
         } else {
             float sq = sqrtf((float)((double)P->T / nfine));
             for (int i = 0; i < D; i++) dW[i] = z[i] * sq;
@@ -375,7 +383,7 @@ int orc_mlmc_level_sums(const fdmc_model* model, const fdmc_payoff* payoff, int 
 #pragma omp for schedule(static)
         for (int64_t p = 0; p < n; p++) {
             orc_prng_normals(seed, TAG_MLMC + (uint32_t)l, (uint64_t)(first + p), D, z);
-            coupled_sums_from_noise(&P, nf, nfine, M, FDMC_NOISE_RAW, NULL, z, dW, loc);
+            coupled_sums_from_noise(&P, nf, nfine, M, FDMC_NOISE_RAW, NULL, NULL, z, dW, loc);
         }
 #pragma omp critical
         for (int i = 0; i < 4; i++) tot[i] += loc[i];
@@ -457,7 +465,7 @@ int orc_run_mlmc(const fdmc_model* model, const fdmc_payoff* payoff, double eps,
 
 int orc_mlqmc_level_sums(const fdmc_model* model, const fdmc_payoff* payoff, int M, int l,
                          int R, uint32_t salt_base, int64_t offset, int64_t n, int mode,
-                         const orc_bb* bb, double* sums) {
+                         const orc_bb* bb, const double* pca_M, double* sums) {
     orc_params P;
     int nfine = level_steps(M, l), nf = noise_dim(model);
     if (orc_make_params(model, payoff, 1, &P)) return -1;
@@ -465,6 +473,10 @@ int orc_mlqmc_level_sums(const fdmc_model* model, const fdmc_payoff* payoff, int
     int use_sobol = D <= D_MAX_SOBOL;
     if (mode == FDMC_NOISE_BB && (nf != 1 || !bb || bb->N != nfine)) {
         orc_set_error("MLQMC BB needs a one-factor model and bb->N == M^l");
+        return -1;
+    }
+    if (mode == FDMC_NOISE_PCA && (nf != 1 || !pca_M)) {
+        orc_set_error("MLQMC PCA needs a one-factor model and an M^l x M^l matrix");
         return -1;
     }
     for (int r = 0; r < R; r++) {
@@ -482,7 +494,7 @@ int orc_mlqmc_level_sums(const fdmc_model* model, const fdmc_payoff* payoff, int
                 uint32_t idx = (uint32_t)(offset + p);
                 if (use_sobol) sobol_normals_point(&t, idx, z);
                 else orc_prng_normals(salt, TAG_QMC_FALLBACK, idx, D, z);
-                coupled_sums_from_noise(&P, nf, nfine, M, mode, bb, z, dW, loc);
+                coupled_sums_from_noise(&P, nf, nfine, M, mode, bb, pca_M, z, dW, loc);
             }
 #pragma omp critical
             for (int i = 0; i < 4; i++) tot[i] += loc[i];
@@ -497,14 +509,14 @@ int orc_mlqmc_level_sums(const fdmc_model* model, const fdmc_payoff* payoff, int
 /* run_mlqmc_cuda (methods_cuda.cu:1680-2019). */
 int orc_run_mlqmc(const fdmc_model* model, const fdmc_payoff* payoff, double eps,
                   const fdmc_mlmc_config* ml, const fdmc_qmc_config* qc, int mode,
-                  const orc_bb* bbs, int n_bb, fdmc_result* out) {
+                  const orc_bb* bbs, int n_bb, const double* const* pcas, int n_pca,
+                  fdmc_result* out) {
     double t0 = now_s();
     int M = ml->M, maxL = ml->max_L, L = 2, R = qc->R;
     if (is_basket(model)) { orc_set_error("MLQMC does not support baskets"); return -1; }
     if (mode != FDMC_NOISE_RAW && noise_dim(model) != 1) {
         orc_set_error("MLQMC BB/PCA need a one-factor model"); return -1;
     }
-    if (mode == FDMC_NOISE_PCA) { orc_set_error("oracle MLQMC PCA not provided"); return -1; }
     double E[32] = {0}, S2[32] = {0};
     int64_t N[32] = {0}, off[32] = {0};
     double* s = (double*)malloc(sizeof(double) * 4 * (size_t)R);
@@ -514,9 +526,11 @@ int orc_run_mlqmc(const fdmc_model* model, const fdmc_payoff* payoff, double eps
 #define LEVEL_RUN(l, n, em, vv)                                                                  \
     do {                                                                                         \
         const orc_bb* bl = (mode == FDMC_NOISE_BB && (l) < n_bb) ? &bbs[l] : NULL;              \
+        const double* pl = (mode == FDMC_NOISE_PCA && (l) < n_pca) ? pcas[l] : NULL;            \
         if (mode == FDMC_NOISE_BB && !bl) { orc_set_error("missing bb_list entry"); return -1; } \
+        if (mode == FDMC_NOISE_PCA && !pl) { orc_set_error("missing pca_list entry"); return -1; } \
         if (orc_mlqmc_level_sums(model, payoff, M, (l), R, qc->salt_base, off[l], (n), mode, bl, \
-                                 s)) return -1;                                                  \
+                                 pl, s)) return -1;                                              \
         off[l] += (n);                                                                           \
         for (int r = 0; r < R; r++) rm[r] = s[4 * r] / (double)(n);                              \
         mean_var(rm, R, &(em), &(vv));                                                           \

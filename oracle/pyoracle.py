@@ -91,10 +91,11 @@ _SIGS = {
                                C.POINTER(fdmc.CMLMCConfig), C.POINTER(fdmc.CResult)]),
     "orc_mlqmc_level_sums": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff), C.c_int, C.c_int,
                                        C.c_int, C.c_uint32, C.c_int64, C.c_int64, C.c_int, C.POINTER(COrcBB),
-                                       _f64p]),
+                                       _f64p, _f64p]),
     "orc_run_mlqmc": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff), C.c_double,
                                 C.POINTER(fdmc.CMLMCConfig), C.POINTER(fdmc.CQMCConfig), C.c_int,
-                                C.POINTER(COrcBB), C.c_int, C.POINTER(fdmc.CResult)]),
+                                C.POINTER(COrcBB), C.c_int, C.POINTER(_f64p), C.c_int,
+                                C.POINTER(fdmc.CResult)]),
     "orc_cv_pilot": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff),
                                C.POINTER(fdmc.CPayoff), C.c_double, C.c_int, C.c_int, C.c_uint32,
                                C.POINTER(fdmc.CCVPilot)]),
@@ -339,15 +340,22 @@ def run_mlmc(model, payoff, eps, cfg):
     return fdmc._result(r)
 
 
-def mlqmc_level_sums(model, payoff, M, l, R, salt_base, offset, n, mode=0, bb=None):
+def _colmajor(M):
+    return np.ascontiguousarray(np.asarray(M, np.float64).T).ravel()
+
+
+def mlqmc_level_sums(model, payoff, M, l, R, salt_base, offset, n, mode=0, bb=None, pca_M=None):
+    """pca_M: the level's D x D increment matrix (dW = pca_M z), row-major."""
     m, p = model.to_c(), payoff.to_c()
     s = np.zeros(4 * R)
+    Mc = _colmajor(pca_M) if pca_M is not None else None
     _chk(lib().orc_mlqmc_level_sums(C.byref(m), C.byref(p), M, l, R, salt_base, offset, n, mode, _bbp(bb),
-                                    _ptr(s, C.c_double)))
+                                    _ptr(Mc, C.c_double) if Mc is not None else None, _ptr(s, C.c_double)))
     return s.reshape(R, 4)
 
 
-def run_mlqmc(model, payoff, eps, ml, qc, mode=0, bbs=None):
+def run_mlqmc(model, payoff, eps, ml, qc, mode=0, bbs=None, pcas=None):
+    """pcas: per-level increment matrices (row-major), level l of size M^l."""
     r = fdmc.CResult()
     m, p, a, b = model.to_c(), payoff.to_c(), ml.to_c(), qc.to_c()
     arr = None
@@ -355,7 +363,14 @@ def run_mlqmc(model, payoff, eps, ml, qc, mode=0, bbs=None):
     if bbs:
         n_bb = len(bbs)
         arr = (COrcBB * n_bb)(*[x.c for x in bbs])
-    _chk(lib().orc_run_mlqmc(C.byref(m), C.byref(p), eps, C.byref(a), C.byref(b), mode, arr, n_bb, C.byref(r)))
+    parr, n_pca, keep = None, 0, []
+    if pcas:
+        keep = [_colmajor(x) for x in pcas]
+        n_pca = len(keep)
+        parr = (_f64p * n_pca)(*[_ptr(x, C.c_double) for x in keep])
+    _chk(lib().orc_run_mlqmc(C.byref(m), C.byref(p), eps, C.byref(a), C.byref(b), mode, arr, n_bb, parr, n_pca,
+                             C.byref(r)))
+    del keep
     return fdmc._result(r)
 
 

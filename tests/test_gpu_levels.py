@@ -74,6 +74,62 @@ def test_mlqmc_level_sums_parity(engine, model, payoff, M, mode):
             _check_sums(g, rows[r], 513, False)
 
 
+@pytest.mark.parametrize("model,payoff,M,bf16", [
+    (GBM, ASIAN, 2, False),
+    (LVQ, EURO, 4, False),
+    (DUP, LOOK, 2, True),
+], ids=["gbm-asian-fp16", "lvq-euro-fp16", "dupire-look-bf16"])
+def test_mlqmc_pca_level_sums_parity(engine, model, payoff, M, bf16):
+    """PCA noise per level (methods_cuda.cu:1907-1913): the fine increments are
+    dW = M_l z on matrix cores with FP16/BF16 operands; the oracle applies M_l in
+    FP64.  Replicas with their own ranges run one GEMM each, replicas sharing a
+    range one GEMM together."""
+    R, L = 3, 4
+    levels = (0, 2, L)
+    mats = {l: O.pca_compute(M ** l, 1.0) for l in range(L + 1)}
+    pcl = [engine.pca_upload(M ** l, mats[l].T.ravel()) for l in range(L + 1)]
+    reqs = [(l, r, 100 * r, 513) for l in levels for r in range(R)] + \
+           [(l, r, 1000, 600) for l in levels for r in range(R)]
+    mode = fdmc.NOISE_PCA | (fdmc.PCA_BF16 if bf16 else 0)
+    got = engine.levels_accumulate(model, payoff, M, reqs, qmc=True, mode=mode, R=R, salt_or_seed=42,
+                                   pca_list=pcl)
+    for x in pcl:
+        x.free()
+    tol = 1e-2 if bf16 else 2e-3     # operand quantisation of the increments
+    for i, (l, r, first, n) in enumerate(reqs):
+        ref = O.mlqmc_level_sums(model, payoff, M, l, R, 42, first, n, fdmc.NOISE_PCA, pca_M=mats[l])[r] / n
+        g = np.asarray(got[i]) / n
+        scale = max(1.0, abs(ref[2]))
+        assert abs(g[0] - ref[0]) <= tol * scale, (l, r, g, ref)
+        assert abs(g[2] - ref[2]) <= tol * scale, (l, r, g, ref)
+        assert abs(g[3] - ref[3]) <= 2 * tol * max(1.0, abs(ref[3])), (l, r, g, ref)
+
+
+def test_run_mlqmc_pca_matches_oracle_loop_and_bridge(engine):
+    ml = fdmc.MLMCConfig(M=2, max_L=6, pilot_n=256)
+    qc = fdmc.QMCConfig(R=8)
+    mats = [O.pca_compute(2 ** l, 1.0) for l in range(7)]
+    pcl = [engine.pca_upload(2 ** l, m.T.ravel()) for l, m in enumerate(mats)]
+    g = engine.run_mlqmc_cuda(GBM, ASIAN, 5e-3, ml, qc, fdmc.NOISE_PCA, pca_list=pcl)
+    for x in pcl:
+        x.free()
+    bbl = [engine.bb_upload(fdmc.bb_precompute(2 ** l, 1.0)) for l in range(7)]
+    gb = engine.run_mlqmc_cuda(GBM, ASIAN, 5e-3, ml, qc, fdmc.NOISE_BB, bbl)
+    for b in bbl:
+        b.free()
+    r = O.run_mlqmc(GBM, ASIAN, 5e-3, ml, qc, fdmc.NOISE_PCA, pcas=mats)
+    # FP16 increments move the variance estimates that drive the allocation; this
+    # run ends at the level cap, where the loop's iterations compound that: the
+    # work (path-steps) agrees to a few percent, sample counts (dominated by the
+    # cheap coarse levels) to about 15 %, prices within the error
+    assert g.levels == r.levels
+    assert abs(g.path_steps - r.path_steps) <= 0.05 * r.path_steps
+    assert abs(g.n_samples - r.n_samples) <= 0.25 * r.n_samples
+    assert abs(g.price - r.price) < 1.0 * r.std_error
+    assert abs(g.price - gb.price) < 3 * math.hypot(g.std_error, gb.std_error)
+    assert g.std_error < 5e-3 / math.sqrt(2) * 1.01
+
+
 def test_run_mlmc_matches_oracle_loop(engine):
     cfg = fdmc.MLMCConfig(M=2, max_L=6, pilot_n=2000, seed=7)
     g = engine.run_mlmc_cuda(GBM, EURO, 2e-2, cfg)

@@ -295,6 +295,34 @@ def test_mlmc_gbm_european():
     assert abs(r.price - 10.450584) < 3 * r.std_error + 5e-2
 
 
+def _bridge_matrix(n, T):
+    """The Brownian bridge as a dense increment matrix: column j = bridge of e_j (FP64)."""
+    bb = O.OracleBB(n, T)
+    return bb, np.stack([bb.apply_f64(e) for e in np.eye(n)], axis=1)
+
+
+def test_mlqmc_pca_level_sums_equal_bridge_through_its_matrix():
+    # PCA mode with the bridge written as a matrix must give the bridge's level
+    # sums up to FP32 rounding of the increments; the transposed matrix must not
+    g, A = fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Asian(100.0)
+    for M, l in ((2, 4), (4, 2)):
+        bb, Mb = _bridge_matrix(M ** l, 1.0)
+        ref = O.mlqmc_level_sums(g, A, M, l, 2, 42, 7, 300, fdmc.NOISE_BB, bb=bb)
+        got = O.mlqmc_level_sums(g, A, M, l, 2, 42, 7, 300, fdmc.NOISE_PCA, pca_M=Mb)
+        assert np.allclose(got, ref, rtol=2e-5, atol=1e-3), (got, ref)
+        bad = O.mlqmc_level_sums(g, A, M, l, 2, 42, 7, 300, fdmc.NOISE_PCA, pca_M=Mb.T)
+        assert not np.allclose(bad[:, 2], ref[:, 2], rtol=1e-3)
+
+
+def test_mlqmc_pca_loop_matches_bridge_loop():
+    g, A = fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.Asian(100.0)
+    ml, qc = fdmc.MLMCConfig(M=2, max_L=6, pilot_n=256), fdmc.QMCConfig(R=8)
+    rp = O.run_mlqmc(g, A, 1e-2, ml, qc, fdmc.NOISE_PCA, pcas=[O.pca_compute(2 ** l, 1.0) for l in range(7)])
+    rb = O.run_mlqmc(g, A, 1e-2, ml, qc, fdmc.NOISE_BB, [O.OracleBB(2 ** l, 1.0) for l in range(7)])
+    assert abs(rp.price - rb.price) < 3 * math.hypot(rp.std_error, rb.std_error)
+    assert rp.std_error < 1e-2 / math.sqrt(2) * 1.01
+
+
 def test_cv_pilot_beta_for_asian_geometric():
     g = fdmc.GBMParams(100, 0.05, 0.2, 1.0)
     E = fdmc.geom_asian_analytic(100, 100, 1, 0.05, 0.2, 64)
