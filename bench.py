@@ -132,8 +132,9 @@ def c2_asian(eng, steps=5):
 
 
 def c3_gemm(eng, reps=5):
-    """Config-3 PCA transform: 2^18 points x 16 replicas, dW = Z M^T with
-    K = N = 1024 on MFMA (FP16 and BF16); HIP-event time of the GEMM alone."""
+    """Config-3 PCA basket pass: 2^18 points x 16 replicas, dW = Z M^T with
+    K = N = 1024 on MFMA (FP16 and BF16), the GBM Euler factors multiplied in
+    the GEMM's epilogue; HIP-event time of the GEMM launch(es)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import fdmc
     from c3_setup import c3_parameters

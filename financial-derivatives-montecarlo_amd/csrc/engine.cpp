@@ -375,6 +375,25 @@ int pca_rows(fdmc_context* c, const fdmc_pca* pca, const uint32_t* sdir, int D, 
     return ld;
 }
 
+// PCA noise of a GBM basket for R replicas: Z as in pca_rows, then the GEMM
+// with the product epilogue (ProdEpi) into c->pca_dw as [ld / 128][mrows][n]
+// Euler-factor products.  Returns mrows, the rows per 128-column slab.
+int64_t pca_prod(fdmc_context* c, const fdmc_pca* pca, const uint32_t* sdir, int D, int R, int64_t first,
+                 int64_t count, bool bf16, const ProdEpi& epi) {
+    if (!pca || pca->m != D) fail(FDMC_E_INVALID, "PCA data must have m == n_steps * noise dimension");
+    const int ld = pca->mp;
+    const int64_t rows = (int64_t)R * count, mrows = (rows + 255) / 256 * 256;
+    if (mrows > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many PCA rows in one pass");
+    void* z = c->pca_z.ensure((size_t)mrows * ld * 2);
+    ProdEpi e = epi;
+    e.P = (float*)c->pca_dw.ensure((size_t)(ld / 128) * mrows * e.n * sizeof(float));
+    HIPC(launch_gen_z(sdir, D, ld, R, first, count, bf16, z, c->stream));
+    if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
+    HIPC(launch_gemm_nt(z, bf16 ? pca->d_bf16 : pca->d_f16, nullptr, (int)mrows, ld, ld, bf16, c->stream, &e));
+    if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
+    return mrows;
+}
+
 int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     const fdmc_model& m = *s.model;
     const fdmc_payoff& p = *s.payoff;
@@ -418,7 +437,16 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
         b.seed = s.salt_base;
         b.tag = s.prng ? s.tag : TAG_QMC_FALLBACK;
     } else if (s.mode == FDMC_NOISE_PCA) {
-        src = SRC_ROWS;
+        // GBM assets, a power-of-two count and the GEMM's 256-column tiling: the
+        // Euler factors multiply inside the GEMM's epilogue (no dW in memory)
+        const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
+        const bool tiled = s.pca && s.pca->mp % 256 == 0;
+        if (m.kind == FDMC_MODEL_MULTI_GBM && pow2 && tiled) {
+            src = SRC_PROD;
+            na = n;
+        } else {
+            src = SRC_ROWS;
+        }
     } else if (s.mode == FDMC_NOISE_RAW) {
         src = SRC_SOBOL;
     } else {
@@ -455,11 +483,15 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
     const size_t lds = basket_lds_bytes(src, D, na);
     if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "basket LDS footprint too large");
     if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    // replicas in groups so that the PCA buffers stay below ~8 GiB
+    // replicas in groups so that the PCA buffers stay below ~8 GiB (dW rows)
+    // or ~16 GiB (normals and slab products only)
     int group = s.R;
     if (src == SRC_ROWS) {
         const double per = (double)s.count * (D + 128) * 6.0;
         group = (int)std::max(1.0, std::min((double)s.R, 8.0 * (1 << 30) / per));
+    } else if (src == SRC_PROD) {
+        const double per = (double)s.count * s.pca->mp * (2.0 + 4.0 * n / 128.0);
+        group = (int)std::max(1.0, std::min((double)s.R, 16.0 * (1 << 30) / per));
     }
     for (int r0 = 0; r0 < s.R; r0 += group) {
         const int g = std::min(group, s.R - r0);
@@ -468,13 +500,24 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
         if (src == SRC_ROWS) {
             bg.ld = pca_rows(c, s.pca, sdir + (size_t)r0 * D * SDIR_WORDS, D, g, s.first, s.count, s.pca_bf16 != 0);
             bg.dW = (const float*)c->pca_dw.p;
+        } else if (src == SRC_PROD) {
+            ProdEpi e{};
+            e.c0 = 1.0f + b.mu * b.h;
+            for (int a = 0; a < n; a++) e.sig[a] = b.sig[a];
+            e.n = n;
+            e.D = D;
+            bg.prow = pca_prod(c, s.pca, sdir + (size_t)r0 * D * SDIR_WORDS, D, g, s.first, s.count,
+                               s.pca_bf16 != 0, e);
+            bg.npc = s.pca->mp / 128;
+            bg.ld = s.pca->mp;
+            bg.dW = (const float*)c->pca_dw.p;
         } else {
             bg.sdir = sdir ? sdir + (size_t)r0 * D * SDIR_WORDS : nullptr;
             bg.seed = b.seed + (uint32_t)r0;
         }
         void* args[] = {(void*)&bg};
         HIPC(hipLaunchKernel(fn, dim3((unsigned)nwg, (unsigned)g), dim3(256), args, lds, c->stream));
-        if (src == SRC_ROWS && c->timing) {
+        if ((src == SRC_ROWS || src == SRC_PROD) && c->timing) {
             HIPC(hipStreamSynchronize(c->stream));
             float ms = 0.0f;
             HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));

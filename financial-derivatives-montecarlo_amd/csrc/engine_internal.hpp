@@ -34,11 +34,11 @@ const void* greeks_kernel_bb(int mk, int pk, int cv, int cpk, int lb);
 const void* greeks_kernel_prng(int mk, int pk, int cv, int cpk, int lb);
 
 // k_pca.hip
-size_t gen_z_lds_bytes(int D);
 hipError_t launch_gen_z(const uint32_t* sdir, int D, int ld, int R, int64_t first, int64_t count, bool bf16,
                         void* out, hipStream_t st);
+// epi: product epilogue (baskets of GBM assets) instead of storing C
 hipError_t launch_gemm_nt(const void* A, const void* Bt, float* C, int Mrows, int N, int K, bool bf16,
-                          hipStream_t st);
+                          hipStream_t st, const ProdEpi* epi = nullptr);
 // k_basket.hip
 const void* basket_kernel(int mk, int src, int na);
 size_t basket_lds_bytes(int src, int D, int na);

@@ -7,7 +7,11 @@
 //   SRC_ROWS : dW rows [row][k * n + a] produced by the PCA GEMM (FP32), or
 //   SRC_SOBOL/SRC_PRNG : normals generated in the lane, scaled by sqrt(h) and
 //              correlated with the lower Cholesky factor L (dW_k = L z_k sqrt h,
-//              the batched GEMM of correlate_multi_dupire, :1133-1148).
+//              the batched GEMM of correlate_multi_dupire, :1133-1148), or
+//   SRC_PROD : GBM assets through the PCA GEMM's product epilogue: the Euler
+//              step S <- S (1 + mu h + sig dW) multiplies, so S_T^a is S0^a
+//              times the products of factors the GEMM left per 128-column slab
+//              (dW itself never reaches memory).
 // Payoff max(sum_a w_a S_T^a - K, 0) e^{-rT}.  For GBM assets the pathwise
 // adjoint of the Euler scheme gives dS_T^a / dS0^a = S_T^a / S0^a, hence the
 // per-asset deltas disc 1{B > K} w_a S_T^a / S0^a accumulated beside the price.
@@ -70,6 +74,13 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
         float S[NA];
 #pragma unroll
         for (int a = 0; a < NA; a++) S[a] = b.S0[a];
+        if constexpr (SRC == SRC_PROD) {
+            for (int pc = 0; pc < b.npc; pc++) {
+                const float* src = b.dW + ((int64_t)pc * b.prow + row) * NA;   // n == NA here
+#pragma unroll
+                for (int a = 0; a < NA; a++) S[a] *= src[a];
+            }
+        } else
         for (int k = 0; k < b.n_steps; k++) {
             float dw[NA];
             if constexpr (SRC == SRC_ROWS) {
@@ -144,6 +155,9 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
 
 template <int MK, int SRC>
 static const void* pick_na(int na) {
+    if constexpr (SRC == SRC_PROD) {
+        if (na == 2) return (const void*)&k_basket<2, MK, SRC>;
+    }
     switch (na) {
     case 4: return (const void*)&k_basket<4, MK, SRC>;
     case 8: return (const void*)&k_basket<8, MK, SRC>;
@@ -154,6 +168,8 @@ static const void* pick_na(int na) {
 }
 
 const void* basket_kernel(int mk, int src, int na) {
+    if (mk == MK_MULTI_GBM && src == SRC_PROD) return pick_na<MK_MULTI_GBM, SRC_PROD>(na);
+    if (src == SRC_PROD) return nullptr;
     if (mk == MK_MULTI_GBM) {
         if (src == SRC_ROWS) return pick_na<MK_MULTI_GBM, SRC_ROWS>(na);
         if (src == SRC_SOBOL) return pick_na<MK_MULTI_GBM, SRC_SOBOL>(na);

@@ -92,8 +92,21 @@ struct LevelsLaunch {
 };
 
 // Multi-asset basket launch (k_basket.hip).
-enum BasketSrc : int { SRC_ROWS = 0, SRC_SOBOL = 1, SRC_PRNG = 2 };
+enum BasketSrc : int { SRC_ROWS = 0, SRC_SOBOL = 1, SRC_PRNG = 2, SRC_PROD = 3 };
 constexpr int BASKET_MAX = 32;
+
+// Product epilogue of the PCA GEMM for baskets of GBM assets (k_gemm_nt256):
+// the Euler factor of step k and asset a is c0 + sig[a] dW[row][k n + a]
+// (c0 = 1 + mu h), and S_T^a = S0^a times the product of all of them, in any
+// order; each 128-column slab (128 / n dates) leaves its product per row and
+// asset in P[slab][row][a].  n is a power of two in [2, 32]; columns >= D
+// (padding) contribute 1.
+struct ProdEpi {
+    float c0;
+    float sig[BASKET_MAX];
+    int n, D;
+    float* P;                 // [ld / 128][Mrows][n]
+};
 
 struct BasketLaunch {
     int n, n_steps, D, greeks;
@@ -101,8 +114,10 @@ struct BasketLaunch {
     float sigma0, alpha, beta_m1, inv_S0;     // MULTI_DUPIRE (vol normalised by the first S0)
     float S0[BASKET_MAX], sig[BASKET_MAX], w[BASKET_MAX];
     const float* L;           // n x n lower Cholesky (row-major) or null
-    const float* dW;          // SRC_ROWS: [R * count][ld]
+    const float* dW;          // SRC_ROWS: [R * count][ld]; SRC_PROD: the GEMM's products [npc][prow][n]
     int ld;                   // row stride of dW (D rounded up to 128)
+    int npc;                  // SRC_PROD: 128-column slabs (ld / 128)
+    int64_t prow;             // SRC_PROD: rows per slab
     const uint32_t* sdir;     // SRC_SOBOL: [R][D][SDIR_WORDS]
     uint32_t seed, tag;       // SRC_PRNG
     int64_t first, count;

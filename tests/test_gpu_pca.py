@@ -85,6 +85,39 @@ def test_basket_pca_matches_oracle_fp64_gemm(engine, c3):
         assert np.allclose(got[:, 0] / n, ref[:, 0] / n, rtol=tol), (bf16, got[:, 0] / n, ref[:, 0] / n)
 
 
+@pytest.mark.parametrize("n,steps", [(8, 32), (4, 48), (2, 96), (16, 24), (32, 8)],
+                         ids=["8x32-fused", "4x48-fused-padded", "2x96-fused-padded", "16x24-rows", "32x8-fused"])
+def test_basket_pca_product_epilogue_and_rows_match_oracle(engine, n, steps):
+    """GBM baskets whose padded dimension is a multiple of 256 price through the
+    GEMM's product epilogue (the Euler factors multiply per 128-column slab,
+    padding columns count as 1); the others through stored dW rows.  Both
+    against the oracle's FP64 PCA and sequential Euler steps."""
+    sig, C = c3_parameters(n, seed=n)
+    M = O.pca_kron_compute(steps, 1.0, C)
+    D = n * steps
+    pca = engine.pca_upload(D, M.T.ravel())
+    m = fdmc.MultiGBMParams(n, [100.0 + a for a in range(n)], list(sig), 0.05, 1.0, L=cholesky_lower(C).ravel())
+    pay = fdmc.Basket(100.0, 0.05, 1.0, n)
+    R, cnt = 3, 1500
+    for bf16, tol in ((False, 2e-3), (True, 1e-2)):
+        mode = fdmc.NOISE_PCA | (fdmc.PCA_BF16 if bf16 else 0)
+        got = np.array(engine.qmc_accumulate(m, pay, steps, R, 42, 300, cnt, mode, dev_pca=pca))
+        ref = O.qmc_accumulate(m, pay, steps, R, 42, 300, cnt, fdmc.NOISE_PCA, pca_M=M)
+        assert np.allclose(got[:, 0] / cnt, ref[:, 0] / cnt, rtol=tol, atol=1e-3), (bf16, got[:, 0], ref[:, 0])
+        assert np.allclose(got[:, 1] / cnt, ref[:, 1] / cnt, rtol=2 * tol, atol=1e-2), (bf16, got[:, 1], ref[:, 1])
+    pca.free()
+
+
+def test_basket_pca_dupire_rows_match_oracle(engine, c3):
+    """Local-vol assets step sequentially, so they read the stored dW rows."""
+    sig, C, M, pca = c3
+    m = fdmc.MultiDupireParams(16, 0.05, 0.2, 0.5, 0.7, 1.0, L=cholesky_lower(C).ravel())
+    R, n = 2, 1024
+    got = np.array(engine.qmc_accumulate(m, PAY, 64, R, 42, 0, n, fdmc.NOISE_PCA, dev_pca=pca))
+    ref = O.qmc_accumulate(m, PAY, 64, R, 42, 0, n, fdmc.NOISE_PCA, pca_M=M)
+    assert np.allclose(got[:, 0] / n, ref[:, 0] / n, rtol=2e-3, atol=1e-3), (got[:, 0] / n, ref[:, 0] / n)
+
+
 def test_basket_prng_matches_oracle(engine, c3):
     sig, C, M, pca = c3
     m = _basket(sig, C)
