@@ -21,7 +21,7 @@ ARCH = "gfx950"
 SOURCES = ["engine.cpp", "k_sobol.hip", "k_path_raw.hip", "k_path_bb.hip", "k_path_prng.hip",
            "k_levels_raw.hip", "k_levels_bb.hip", "k_levels_prng.hip", "k_levels_rows.hip",
            "k_greeks_raw.hip", "k_greeks_bb.hip", "k_greeks_prng.hip",
-           "k_pca.hip", "k_basket.hip", "k_path_rows.hip", "k_book.hip", "k_book_o3.hip", "book.cpp",
+           "k_pca.hip", "k_basket.hip", "k_is.hip", "k_path_rows.hip", "k_book.hip", "k_book_o3.hip", "book.cpp",
            "../cpp/utils.cpp", "../cpp/methods_hip.cpp"]
 EXAMPLES = ["asian_table", "book_table"]
 

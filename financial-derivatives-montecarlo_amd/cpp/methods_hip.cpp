@@ -232,8 +232,14 @@ MCResult run_qmc_cv_cuda(const ModelVariant& main_model, const ModelVariant& ctr
     return res(r);
 }
 
-MCResult run_is_cuda(const GBMParams&, const European&, double, double, const MCConfig&) {
-    throw std::runtime_error("run_is_cuda: importance sampling is not part of this engine");
+MCResult run_is_cuda(const GBMParams& model, const European& payoff, double z_star, double eps,
+                     const MCConfig& cfg) {
+    CModel m = to_c(ModelVariant{model});
+    CPayoff p = to_c(PayoffVariant{payoff});
+    fdmc_mc_config c = to_c(cfg);
+    fdmc_result r{};
+    check(fdmc_run_is(ctx(), &m.m, &p.p, z_star, eps, &c, &r));
+    return res(r);
 }
 
 std::pair<double, double> run_mc_fixed(const ModelVariant& model, const PayoffVariant& payoff, int n_steps,

@@ -82,7 +82,7 @@ MCResult run_qmc_cv_cuda(const ModelVariant& main_model, const ModelVariant& ctr
                          double beta, double eps, int n_steps, const QMCConfig& cfg = {},
                          NoiseMode mode = NoiseMode::Raw, DeviceBBData* dev_bb = nullptr);
 
-// Importance sampling is not part of this engine; calling it throws.
+// Importance-sampled GBM European call (drift shift z_star over the horizon).
 MCResult run_is_cuda(const GBMParams& model, const European& payoff, double z_star, double eps,
                      const MCConfig& cfg = {});
 

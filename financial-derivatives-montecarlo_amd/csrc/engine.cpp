@@ -32,7 +32,7 @@ thread_local std::string fdmc_host::g_last_error;
 namespace {
 constexpr int D_MAX_SOBOL = 20000;  // models.hpp:9
 constexpr uint32_t TAG_QMC_FALLBACK = 0x51u, TAG_MLMC = 0x100u, TAG_CV_PILOT = 0x200u,
-                   TAG_CV_MAIN = 0x201u;
+                   TAG_CV_MAIN = 0x201u, TAG_IS = 0x300u;
 }  // namespace
 
 struct fdmc_bb {
@@ -1401,6 +1401,76 @@ fdmc_status fdmc_run_mc(fdmc_context* c, const fdmc_model* model, const fdmc_pay
         run_pass(c, s, v);
         const double m = v[0] / (double)N;
         const double var = std::max(0.0, v[1] / (double)N - m * m) / (double)N;
+        out->price = m;
+        out->std_error = std::sqrt(var);
+        out->n_samples = N;
+        out->path_steps = (N + cfg->pilot_n) * (int64_t)n_steps;
+        out->time_s = now_s() - t0;
+    });
+}
+
+// Sums {Y, Y^2} of n importance-sampled paths of the Philox stream (seed, TAG_IS).
+static void is_sums(fdmc_context* c, ISLaunch a, uint32_t seed, int64_t n, double out[2]) {
+    a.seed = seed;
+    a.tag = TAG_IS;
+    a.count = n;
+    const int64_t ntiles = (n + 255) >> 8;
+    const int64_t target = (int64_t)c->n_cu * 32;
+    int64_t tpw = std::max<int64_t>(1, (ntiles + target - 1) / target);
+    if (tpw > (1 << 20)) tpw = 1 << 20;
+    const int64_t nwg = (ntiles + tpw - 1) / tpw;
+    if (nwg > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many workgroups");
+    a.tiles_per_wg = (int)tpw;
+    a.partials = (double*)c->partials.ensure((size_t)nwg * 2 * sizeof(double));
+    double* d_sums = (double*)c->sums.ensure(2 * sizeof(double));
+    HIPC(launch_is_gbm(a, (int)nwg, c->stream));
+    HIPC(launch_reduce_partials(a.partials, 1, (int)nwg, 2, d_sums, c->stream));
+    ensure_host_sums(c, 2);
+    HIPC(hipMemcpyAsync(c->h_sums, d_sums, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    out[0] = c->h_sums[0];
+    out[1] = c->h_sums[1];
+}
+
+// run_is_cuda (methods_cuda.cu:2317-2365): n = max(4, ceil(T / eps)) steps,
+// per-step shift z* / sqrt(n); pilot of pilot_n paths (seed), then
+// N = max(ceil(2 Var / eps^2), pilot_n) rounded up to even paths (seed + 1).
+fdmc_status fdmc_run_is(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double z_star,
+                        double eps, const fdmc_mc_config* cfg, fdmc_result* out) {
+    return guarded([&] {
+        if (!c || !model || !payoff || !cfg || !out) fail(FDMC_E_INVALID, "null argument");
+        if (model->kind != FDMC_MODEL_GBM) fail(FDMC_E_UNSUPPORTED, "run_is: GBM models only (methods_cuda.cuh:127)");
+        if (payoff->kind != FDMC_PAYOFF_EUROPEAN) fail(FDMC_E_UNSUPPORTED, "run_is: European payoffs only");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (!(eps > 0.0)) fail(FDMC_E_INVALID, "eps must be positive");
+        if (cfg->pilot_n <= 1) fail(FDMC_E_INVALID, "pilot_n must be > 1");
+        if (!std::isfinite(z_star)) fail(FDMC_E_INVALID, "z_star must be finite");
+        const double t0 = now_s();
+        *out = fdmc_result{};
+        const double nd = std::max(4.0, std::ceil(model->T / eps));
+        if (nd > (double)(1 << 30)) fail(FDMC_E_RANGE, "T / eps gives too many steps");
+        const int n_steps = (int)nd;
+        ISLaunch a{};
+        a.S0 = (float)model->S0;
+        a.mu = (float)model->mu;
+        a.sigma = (float)model->sigma;
+        a.h = (float)model->T / (float)n_steps;
+        a.sqrt_h = std::sqrt(a.h);
+        a.K = (float)payoff->K;
+        a.discount = (float)std::exp(-payoff->r * payoff->T);
+        a.zs = (float)(z_star / std::sqrt((double)n_steps));
+        a.n_steps = n_steps;
+        double s[2];
+        is_sums(c, a, cfg->seed, cfg->pilot_n, s);
+        const double pm = s[0] / cfg->pilot_n;
+        const double pvar = std::max(0.0, s[1] / cfg->pilot_n - pm * pm);
+        int64_t N = (int64_t)std::ceil(2.0 * pvar / (eps * eps));
+        N = std::max<int64_t>(N, cfg->pilot_n);
+        N = (N + 1) & ~(int64_t)1;
+        is_sums(c, a, cfg->seed + 1u, N, s);
+        const double m = s[0] / (double)N;
+        const double var = std::max(0.0, s[1] / (double)N - m * m) / (double)N;
         out->price = m;
         out->std_error = std::sqrt(var);
         out->n_samples = N;

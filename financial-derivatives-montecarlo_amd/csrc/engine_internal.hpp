@@ -39,6 +39,8 @@ hipError_t launch_gen_z(const uint32_t* sdir, int D, int ld, int R, int64_t firs
 // epi: product epilogue (baskets of GBM assets) instead of storing C
 hipError_t launch_gemm_nt(const void* A, const void* Bt, float* C, int Mrows, int N, int K, bool bf16,
                           hipStream_t st, const ProdEpi* epi = nullptr);
+// k_is.hip
+hipError_t launch_is_gbm(const ISLaunch& a, int nwg, hipStream_t st);
 // k_basket.hip
 const void* basket_kernel(int mk, int src, int na);
 size_t basket_lds_bytes(int src, int D, int na);

@@ -125,6 +125,15 @@ struct BasketLaunch {
     double* partials;         // [R][nwg_x][2 + 2 n]
 };
 
+// Importance-sampled GBM European call (k_is.hip).
+struct ISLaunch {
+    float S0, mu, sigma, h, sqrt_h, K, discount, zs;   // zs: per-step drift shift z* / sqrt(n)
+    int n_steps, tiles_per_wg;
+    uint32_t seed, tag;       // Philox stream
+    int64_t count;            // paths 0 .. count - 1
+    double* partials;         // [nwg][2]
+};
+
 // Static LDS of the inverse-CDF table (icdf_u32_tab) in every kernel that
 // draws normals from Sobol points, and what that leaves of a CU's 160 KiB for
 // a launch's dynamic LDS.

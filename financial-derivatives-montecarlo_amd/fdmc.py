@@ -384,6 +384,8 @@ _SIGS = {
     "fdmc_pca_free": (C.c_int, [_P]),
     "fdmc_run_mc": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_double, C.c_int32,
                               C.POINTER(CMCConfig), C.POINTER(CResult)]),
+    "fdmc_run_is": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_double, C.c_double,
+                              C.POINTER(CMCConfig), C.POINTER(CResult)]),
     "fdmc_run_mc_fixed": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_int32, C.c_int64,
                                     C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "fdmc_run_mlmc": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_double,
@@ -542,6 +544,15 @@ class Engine:
         r = CResult()
         m, p, c = model.to_c(), payoff.to_c(), cfg.to_c()
         _check(self.lib, self.lib.fdmc_run_mc(self.ctx, C.byref(m), C.byref(p), eps, n_steps, C.byref(c),
+                                              C.byref(r)))
+        return _result(r)
+
+    def run_is_cuda(self, model, payoff, z_star, eps, cfg=None):
+        """Importance-sampled GBM European call (run_is_cuda)."""
+        cfg = cfg or MCConfig()
+        r = CResult()
+        m, p, c = model.to_c(), payoff.to_c(), cfg.to_c()
+        _check(self.lib, self.lib.fdmc_run_is(self.ctx, C.byref(m), C.byref(p), z_star, eps, C.byref(c),
                                               C.byref(r)))
         return _result(r)
 

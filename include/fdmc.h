@@ -244,6 +244,12 @@ fdmc_status fdmc_pca_free(fdmc_pca* pca);
 fdmc_status fdmc_run_mc(fdmc_context* ctx, const fdmc_model* model, const fdmc_payoff* payoff,
                         double eps, int32_t n_steps, const fdmc_mc_config* cfg,
                         fdmc_result* out);
+/* run_is_cuda (methods_cuda.cuh:127-128): importance-sampled GBM European call;
+ * increments (z + z_star / sqrt(n)) sqrt(h) over n = max(4, ceil(T / eps)) steps,
+ * payoffs weighted by the likelihood ratio.  Pilot of cfg->pilot_n paths
+ * (Philox seed cfg->seed), then max(ceil(2 Var / eps^2), pilot_n) paths (seed + 1). */
+fdmc_status fdmc_run_is(fdmc_context* ctx, const fdmc_model* model, const fdmc_payoff* payoff,
+                        double z_star, double eps, const fdmc_mc_config* cfg, fdmc_result* out);
 /* run_mc_fixed (methods_cuda.cuh:131-134): mean and variance of the mean. */
 fdmc_status fdmc_run_mc_fixed(fdmc_context* ctx, const fdmc_model* model,
                               const fdmc_payoff* payoff, int32_t n_steps, int64_t n_paths,

@@ -143,6 +143,12 @@ int orc_run_mc_fixed(const fdmc_model* model, const fdmc_payoff* payoff, int n_s
                      int64_t n_paths, uint32_t seed, double* mean, double* var_of_mean);
 int orc_run_mc(const fdmc_model* model, const fdmc_payoff* payoff, double eps, int n_steps,
                const fdmc_mc_config* cfg, fdmc_result* out);
+/* kernel_is_gbm sums {Y, Y^2} over paths [0, n) of Philox stream (seed, tag);
+ * zs is the per-step shift. */
+int orc_is_sums(const fdmc_model* model, const fdmc_payoff* payoff, int n_steps, float zs, uint32_t seed,
+                uint32_t tag, int64_t n, double sums[2]);
+int orc_run_is(const fdmc_model* model, const fdmc_payoff* payoff, double z_star, double eps,
+               const fdmc_mc_config* cfg, fdmc_result* out);
 /* Level sums {dY, dY^2, Yf, Yf^2} of level l over paths [first, first+n). */
 int orc_mlmc_level_sums(const fdmc_model* model, const fdmc_payoff* payoff, int M, int l,
                         uint32_t seed, int64_t first, int64_t n, double sums[4]);

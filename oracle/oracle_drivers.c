@@ -9,7 +9,8 @@
  *    path = point index), as the reference falls back to XORWOW (:1521, :1767).
  *  - MC / MLMC / CV: Philox streams keyed by (seed, tag, path):
  *    run_mc pilot (seed, 0), main (seed + 1, 0); MLMC level l (seed, 0x100 + l);
- *    cv pilot (seed, 0x200); run_mc_cv main (seed + 100, 0x201).
+ *    cv pilot (seed, 0x200); run_mc_cv main (seed + 100, 0x201);
+ *    run_is pilot (seed, 0x300), main (seed + 1, 0x300).
  */
 #include "oracle.h"
 
@@ -28,6 +29,7 @@ void orc_set_error(const char* msg);
 #define TAG_MLMC 0x100u
 #define TAG_CV_PILOT 0x200u
 #define TAG_CV_MAIN 0x201u
+#define TAG_IS 0x300u
 
 int orc_num_threads(void) {
 #ifdef _OPENMP
@@ -310,6 +312,71 @@ int orc_run_mc_fixed(const fdmc_model* model, const fdmc_payoff* payoff, int n_s
     double v = s[1] / n_paths - m * m;
     *mean = m;
     *var = (v > 0 ? v : 0.0) / n_paths;
+    return 0;
+}
+
+/* kernel_is_gbm (methods_cuda.cu:685-707) over paths [0, n) of the Philox
+ * stream (seed, tag): z ~ N(0,1), dW = (z + zs) sqrt(h), GBM Euler steps,
+ * payoff max(S_T - K, 0) e^{-rT} times exp(-zs sum z - n_steps zs^2 / 2). */
+int orc_is_sums(const fdmc_model* model, const fdmc_payoff* payoff, int n_steps, float zs, uint32_t seed,
+                uint32_t tag, int64_t n, double sums[2]) {
+    orc_params P;
+    if (model->kind != FDMC_MODEL_GBM || payoff->kind != FDMC_PAYOFF_EUROPEAN) {
+        orc_set_error("IS: GBM European only");
+        return -1;
+    }
+    if (orc_make_params(model, payoff, n_steps, &P)) return -1;
+    float sq = sqrtf(P.h);
+    double tot[2] = {0, 0};
+#pragma omp parallel
+    {
+        double loc[2] = {0, 0};
+        float* z = (float*)malloc(sizeof(float) * (size_t)n_steps);
+        float* dW = (float*)malloc(sizeof(float) * (size_t)n_steps);
+#pragma omp for schedule(static)
+        for (int64_t p = 0; p < n; p++) {
+            orc_prng_normals(seed, tag, (uint64_t)p, n_steps, z);
+            float zsum = 0.0f;
+            for (int k = 0; k < n_steps; k++) {
+                zsum += z[k];
+                dW[k] = (z[k] + zs) * sq;
+            }
+            float lr = expf(-zs * zsum - 0.5f * zs * zs * (float)n_steps);
+            float y = orc_path_single(&P, dW, n_steps) * lr;
+            loc[0] += y;
+            loc[1] += (double)y * y;
+        }
+#pragma omp critical
+        { tot[0] += loc[0]; tot[1] += loc[1]; }
+        free(z); free(dW);
+    }
+    sums[0] = tot[0];
+    sums[1] = tot[1];
+    return 0;
+}
+
+/* run_is_cuda (methods_cuda.cu:2317-2365). */
+int orc_run_is(const fdmc_model* model, const fdmc_payoff* payoff, double z_star, double eps,
+               const fdmc_mc_config* cfg, fdmc_result* out) {
+    double t0 = now_s(), s[2];
+    memset(out, 0, sizeof(*out));
+    double nd = ceil(model->T / eps);
+    int n_steps = nd < 4.0 ? 4 : (int)nd;
+    float zs = (float)(z_star / sqrt((double)n_steps));
+    if (orc_is_sums(model, payoff, n_steps, zs, cfg->seed, TAG_IS, cfg->pilot_n, s)) return -1;
+    double pm = s[0] / cfg->pilot_n;
+    double pv = s[1] / cfg->pilot_n - pm * pm;
+    int64_t N = (int64_t)ceil(2.0 * (pv > 0 ? pv : 0.0) / (eps * eps));
+    if (N < cfg->pilot_n) N = cfg->pilot_n;
+    N = (N + 1) & ~(int64_t)1;
+    if (orc_is_sums(model, payoff, n_steps, zs, cfg->seed + 1u, TAG_IS, N, s)) return -1;
+    double m = s[0] / N;
+    double v = s[1] / N - m * m;
+    out->price = m;
+    out->std_error = sqrt((v > 0 ? v : 0.0) / N);
+    out->n_samples = N;
+    out->path_steps = (int64_t)(N + cfg->pilot_n) * n_steps;
+    out->time_s = now_s() - t0;
     return 0;
 }
 

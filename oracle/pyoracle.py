@@ -96,6 +96,10 @@ _SIGS = {
                                 C.POINTER(fdmc.CMLMCConfig), C.POINTER(fdmc.CQMCConfig), C.c_int,
                                 C.POINTER(COrcBB), C.c_int, C.POINTER(_f64p), C.c_int,
                                 C.POINTER(fdmc.CResult)]),
+    "orc_is_sums": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff), C.c_int, C.c_float, C.c_uint32,
+                              C.c_uint32, C.c_int64, _f64p]),
+    "orc_run_is": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff), C.c_double, C.c_double,
+                             C.POINTER(fdmc.CMCConfig), C.POINTER(fdmc.CResult)]),
     "orc_cv_pilot": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff),
                                C.POINTER(fdmc.CPayoff), C.c_double, C.c_int, C.c_int, C.c_uint32,
                                C.POINTER(fdmc.CCVPilot)]),
@@ -323,6 +327,20 @@ def run_mc(model, payoff, eps, n_steps, cfg):
     r = fdmc.CResult()
     m, p, c = model.to_c(), payoff.to_c(), cfg.to_c()
     _chk(lib().orc_run_mc(C.byref(m), C.byref(p), eps, n_steps, C.byref(c), C.byref(r)))
+    return fdmc._result(r)
+
+
+def is_sums(model, payoff, n_steps, zs, seed, tag, n):
+    m, p = model.to_c(), payoff.to_c()
+    s = np.zeros(2)
+    _chk(lib().orc_is_sums(C.byref(m), C.byref(p), n_steps, zs, seed, tag, n, _ptr(s, C.c_double)))
+    return s
+
+
+def run_is(model, payoff, z_star, eps, cfg):
+    r = fdmc.CResult()
+    m, p, c = model.to_c(), payoff.to_c(), cfg.to_c()
+    _chk(lib().orc_run_is(C.byref(m), C.byref(p), z_star, eps, C.byref(c), C.byref(r)))
     return fdmc._result(r)
 
 

@@ -46,6 +46,18 @@ REPORT = [
      "kind": "mc500k", "n_steps": 256,
      "model": {"kind": "dupire", "S0": 100, "mu": 0.05, "sigma0": 0.2, "alpha": 0.5, "beta_d": 0.7, "T": 1},
      "payoff": {"kind": "european", "K": 100, "r": 0.05, "T": 1}},
+    {"example": "ejemplo11_otm_is", "source": "informes_finales/informe_completo.txt:605", "price": 0.028643,
+     "kind": "black_scholes", "model": {"kind": "gbm", "S0": 100, "mu": 0.05, "sigma": 0.2, "T": 1},
+     "payoff": {"kind": "european", "K": 180, "r": 0.05, "T": 1}},
+    # run_is_cuda rows of ejemplo11 (z* = 2.78, default MCConfig): price, std error, N
+    {"example": "ejemplo11_otm_is", "source": "informes_finales/informe_completo.txt:611", "kind": "is_run",
+     "eps": 0.002, "price": 0.0286, "std_error": 0.0004, "n_samples": 10000,
+     "model": {"kind": "gbm", "S0": 100, "mu": 0.05, "sigma": 0.2, "T": 1},
+     "payoff": {"kind": "european", "K": 180, "r": 0.05, "T": 1}},
+    {"example": "ejemplo11_otm_is", "source": "informes_finales/informe_completo.txt:629", "kind": "is_run",
+     "eps": 0.0002, "price": 0.0288, "std_error": 0.0002, "n_samples": 32222,
+     "model": {"kind": "gbm", "S0": 100, "mu": 0.05, "sigma": 0.2, "T": 1},
+     "payoff": {"kind": "european", "K": 180, "r": 0.05, "T": 1}},
 ]
 
 

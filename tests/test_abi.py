@@ -22,7 +22,8 @@ def declared_symbols():
 def test_header_declares_reference_entry_points():
     syms = declared_symbols()
     for s in ("fdmc_run_mc", "fdmc_run_qmc", "fdmc_run_mlmc", "fdmc_run_mlqmc", "fdmc_bb_upload",
-              "fdmc_pca_upload", "fdmc_cv_pilot", "fdmc_run_mc_cv", "fdmc_run_qmc_cv", "fdmc_run_mc_fixed"):
+              "fdmc_pca_upload", "fdmc_cv_pilot", "fdmc_run_mc_cv", "fdmc_run_qmc_cv", "fdmc_run_mc_fixed",
+              "fdmc_run_is"):
         assert s in syms
 
 

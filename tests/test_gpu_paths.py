@@ -128,6 +128,25 @@ def test_run_qmc_matches_oracle_loop(engine):
     assert abs(g.std_error - r.std_error) < 0.05 * r.std_error
 
 
+@pytest.mark.parametrize("eps", [2e-3, 3e-4])
+def test_run_is_matches_oracle_and_black_scholes(engine, eps):
+    """run_is_cuda (methods_cuda.cu:2317-2365) on ejemplo11's deep-OTM call
+    (K = 180, z* = 2.79): the same Philox paths as the oracle's restatement of
+    kernel_is_gbm, and Black-Scholes within 3 standard errors."""
+    g, e = fdmc.GBMParams(100.0, 0.05, 0.2, 1.0), fdmc.European(180.0, 0.05, 1.0)
+    zs = (math.log(1.8) - (0.05 - 0.02)) / 0.2
+    r = engine.run_is_cuda(g, e, zs, eps, fdmc.MCConfig())
+    o = O.run_is(g, e, zs, eps, fdmc.MCConfig())
+    # N = ceil(2 Var / eps^2) of the pilot rounds up to even: one step of rounding apart at most
+    assert abs(r.n_samples - o.n_samples) <= 2
+    if r.n_samples == o.n_samples:
+        assert abs(r.price - o.price) <= 2e-5 * o.price, (r.price, o.price)
+        assert abs(r.std_error - o.std_error) <= 1e-4 * o.std_error
+    bs = fdmc.bs_call(100, 180, 1, 0.05, 0.2)
+    assert abs(r.price - bs) < 3 * r.std_error + 1e-4
+    assert r.std_error < eps / math.sqrt(2) * 1.08
+
+
 def test_c1_mc_european_vs_black_scholes(engine):
     """Config 1 at eps = 1e-2: within 3 standard errors of BS after the Euler bias."""
     r = engine.run_mc_cuda(GBM, EURO, 1e-2, 64, fdmc.MCConfig())

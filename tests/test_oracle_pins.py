@@ -323,6 +323,74 @@ def test_mlqmc_pca_loop_matches_bridge_loop():
     assert rp.std_error < 1e-2 / math.sqrt(2) * 1.01
 
 
+# ---------------------------------------------------------------- importance sampling
+def _z_star(S0, K, mu, sigma, T):
+    # ejemplo11_otm_is.cpp:25-27: the shift that centres the terminal log-price on the strike
+    return (math.log(K / S0) - (mu - 0.5 * sigma ** 2) * T) / (sigma * math.sqrt(T))
+
+
+def test_is_with_zero_shift_is_plain_mc():
+    g, e = fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.European(100, 0.05, 1.0)
+    s = O.is_sums(g, e, 16, 0.0, 7, 0, 3000)
+    m, v = O.run_mc_fixed(g, e, 16, 3000, 7)
+    assert s[0] / 3000 == pytest.approx(m, rel=1e-12)
+    assert (s[1] / 3000 - (s[0] / 3000) ** 2) / 3000 == pytest.approx(v, rel=1e-9)
+
+
+@pytest.mark.parametrize("zs", [0.1, 0.5])
+def test_is_likelihood_ratio_keeps_the_euler_mean(zs):
+    # K = 0: the payoff is S_T e^{-rT}; under any shift the weighted mean is
+    # E[S_T] e^{-rT} = S0 (1 + mu h)^n e^{-rT} of the Euler scheme
+    g, e = fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.European(0.0, 0.05, 1.0)
+    n, N = 8, 200000
+    s = O.is_sums(g, e, n, zs, 11, 0x300, N)
+    m = s[0] / N
+    se = math.sqrt(max(s[1] / N - m * m, 0) / N)
+    exact = 100 * (1 + 0.05 / n) ** n * math.exp(-0.05)
+    assert abs(m - exact) < 4 * se, (m, exact, se)
+    bad = O.is_sums(g, e, n, -zs, 11, 0x300, N)[0] / N   # the shift's sign matters for the draws only
+    assert abs(bad - exact) < 6 * se
+
+
+def _is_exact_moments(S0, K, mu, sigma, T, zs):
+    """Mean and variance of the importance-sampled payoff in the continuous-time
+    limit, by quadrature over the terminal normal: E_q[Y^2] = E_p[f^2 LR]."""
+    f = lambda x: max(S0 * math.exp((mu - 0.5 * sigma ** 2) * T + sigma * math.sqrt(T) * x) - K, 0.0) \
+        * math.exp(-mu * T)
+    m = integrate.quad(lambda x: f(x) * stats.norm.pdf(x), -10, 12, points=[zs])[0]
+    m2 = integrate.quad(lambda x: f(x) ** 2 * math.exp(-zs * x + zs * zs / 2) * stats.norm.pdf(x), -10, 12,
+                        points=[zs])[0]
+    return m, m2 - m * m
+
+
+def test_is_deep_otm_matches_black_scholes_and_report():
+    g, e = fdmc.GBMParams(100, 0.05, 0.2, 1.0), fdmc.European(180, 0.05, 1.0)
+    zs = _z_star(100, 180, 0.05, 0.2, 1.0)
+    assert ("%f" % zs)[:4] == "2.78"                 # printed truncated: informe_completo.txt:611
+    bs = O.bs_call(100, 180, 1.0, 0.05, 0.2)
+    rows = [x for x in REPORT if x["example"] == "ejemplo11_otm_is"]
+    assert abs(bs - [x for x in rows if x["kind"] == "black_scholes"][0]["price"]) < 5e-7
+    m_ex, v_ex = _is_exact_moments(100, 180, 0.05, 0.2, 1.0, zs)
+    assert abs(m_ex - bs) < 1e-9
+    # the oracle's per-path variance (10000-path pilot of the default MCConfig) is the exact one
+    n = 500
+    s = O.is_sums(g, e, n, zs / math.sqrt(n), 123, 0x300, 10000)
+    assert abs((s[1] / 1e4 - (s[0] / 1e4) ** 2) / v_ex - 1) < 0.05
+    for row in (x for x in rows if x["kind"] == "is_run"):
+        r = O.run_is(g, e, zs, row["eps"], fdmc.MCConfig())
+        # N = max(pilot_n, 2 Var / eps^2); at eps = 2e-4 the reference printed about
+        # half of that (its pilot variance is half the exact 1.28e-3), so its
+        # std error there is ~eps rather than eps / sqrt(2)
+        want = 2 * v_ex / row["eps"] ** 2
+        if want < 10000:
+            assert r.n_samples == 10000
+        else:
+            assert abs(r.n_samples / want - 1) < 0.08, (r.n_samples, want)
+        assert r.std_error < row["eps"] / math.sqrt(2) * 1.08
+        assert abs(r.price - bs) < 3 * r.std_error + 1e-4, (r.price, bs, r.std_error)
+        assert abs(r.price - row["price"]) < 3 * math.hypot(r.std_error, row["std_error"]) + 1e-4
+
+
 def test_cv_pilot_beta_for_asian_geometric():
     g = fdmc.GBMParams(100, 0.05, 0.2, 1.0)
     E = fdmc.geom_asian_analytic(100, 100, 1, 0.05, 0.2, 64)
