@@ -394,11 +394,96 @@ int64_t pca_prod(fdmc_context* c, const fdmc_pca* pca, const uint32_t* sdir, int
     return mrows;
 }
 
+// Baskets of more than BASKET_MAX assets (k_basket_wide): raw Sobol or Philox
+// normals, uncorrelated or Cholesky-correlated (n <= WIDE_CORR_MAX); prices only,
+// so 2 accumulators per replica {sum Y, sum Y^2}.
+int run_basket_wide(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
+    const fdmc_model& m = *s.model;
+    const fdmc_payoff& p = *s.payoff;
+    const int n = m.n_assets;
+    if (s.greeks) fail(FDMC_E_UNSUPPORTED, "basket deltas are available up to 32 assets");
+    const int64_t D64 = (int64_t)s.n_steps * n;
+    if (D64 > 0x7FFFFFFF) fail(FDMC_E_RANGE, "n_steps * n_assets too large");
+    const bool prng = s.prng || D64 > D_MAX_SOBOL;
+    if (!prng && s.mode != FDMC_NOISE_RAW)
+        fail(FDMC_E_UNSUPPORTED, "baskets of more than 32 assets take Raw noise (as the reference's examples 07/08)");
+    const bool corr = !m.uncorrelated && m.L_assets;
+    if (corr && n > WIDE_CORR_MAX) fail(FDMC_E_UNSUPPORTED, "correlated baskets: at most 256 assets");
+    BasketWide b{};
+    b.n = n;
+    b.n_steps = s.n_steps;
+    b.D = (int)D64;
+    const float T = (float)m.T;
+    b.h = T / (float)s.n_steps;
+    b.sqrt_h = std::sqrt(b.h);
+    b.mu = (float)m.mu;
+    b.K = (float)p.K;
+    b.discount = (float)std::exp(-p.r * p.T);
+    b.sigma0 = (float)m.sigma0;
+    b.alpha = (float)m.alpha;
+    b.beta_m1 = (float)m.beta_d - 1.0f;
+    b.inv_S0 = 1.0f / (float)m.S0_assets[0];
+    // per-asset parameters and L in one device buffer: [S0 n][sig n][w n][L n x n]
+    std::vector<float> par((size_t)3 * n + (corr ? (size_t)n * n : 0));
+    for (int a = 0; a < n; a++) {
+        par[a] = (float)m.S0_assets[a];
+        par[n + a] = m.sigma_assets ? (float)m.sigma_assets[a] : 0.0f;
+        par[2 * n + a] = p.weights ? (float)p.weights[a] : 1.0f / (float)n;
+    }
+    if (corr)
+        for (size_t i = 0; i < (size_t)n * n; i++) par[3 * (size_t)n + i] = (float)m.L_assets[i];
+    float* dpar = (float*)c->basket_L.ensure(par.size() * sizeof(float));
+    HIPC(hipMemcpyAsync(dpar, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    b.S0 = dpar;
+    b.sig = dpar + n;
+    b.w = dpar + 2 * n;
+    b.L = corr ? dpar + 3 * n : nullptr;
+    const int mk = m.kind == FDMC_MODEL_MULTI_GBM ? MK_MULTI_GBM : MK_MULTI_DUPIRE;
+    const int src = prng ? SRC_PRNG : SRC_SOBOL;
+    if (prng) {
+        b.seed = s.salt_base;
+        b.tag = s.prng ? s.tag : TAG_QMC_FALLBACK;
+    } else {
+        std::vector<uint32_t> salts((size_t)s.R);
+        for (int r = 0; r < s.R; r++) salts[r] = s.salt_base + (uint32_t)r;
+        b.sdir = get_sdir(c, salts, b.D);
+    }
+    const void* fn = basket_wide_kernel(mk, src, corr);
+    if (!fn) fail(FDMC_E_UNSUPPORTED, "wide basket kernel not available");
+    const int wg = basket_wide_wg(corr);
+    const int64_t t_first = s.first / wg, t_last = (s.first + s.count - 1) / wg;
+    const int64_t ntiles = t_last - t_first + 1;
+    const int64_t per_rep = std::max<int64_t>(1, (int64_t)c->n_cu * (corr ? 8 : 16) / s.R);
+    const int64_t tpw = std::max<int64_t>(1, (ntiles + per_rep - 1) / per_rep);
+    const int64_t nwg = (ntiles + tpw - 1) / tpw;
+    if (nwg > 0x7FFFFFFF || tpw > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many workgroups");
+    b.first = s.first;
+    b.count = s.count;
+    b.tiles_per_wg = (int)tpw;
+    b.nwg_x = (int)nwg;
+    b.partials = (double*)c->partials.ensure((size_t)s.R * nwg * 2 * sizeof(double));
+    double* d_sums = (double*)c->sums.ensure((size_t)s.R * 2 * sizeof(double));
+    const size_t lds = basket_wide_lds_bytes(n, corr);
+    if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "wide basket LDS footprint too large");
+    if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* args[] = {(void*)&b};
+    HIPC(hipLaunchKernel(fn, dim3((unsigned)nwg, (unsigned)s.R), dim3(wg), args, lds, c->stream));
+    HIPC(launch_reduce_partials(b.partials, s.R, (int)nwg, 2, d_sums, c->stream));
+    ensure_host_sums(c, (size_t)s.R * 2);
+    HIPC(hipMemcpyAsync(c->h_sums, d_sums, (size_t)s.R * 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    out.assign(c->h_sums, c->h_sums + (size_t)s.R * 2);
+    return 2;
+}
+
 int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     const fdmc_model& m = *s.model;
     const fdmc_payoff& p = *s.payoff;
     const int n = m.n_assets;
-    if (n > BASKET_MAX) fail(FDMC_E_UNSUPPORTED, "baskets of more than 32 assets are not supported");
+    if (p.kind != FDMC_PAYOFF_BASKET && p.kind != FDMC_PAYOFF_EUROPEAN)
+        fail(FDMC_E_UNSUPPORTED, "basket models take a Basket (or European) payoff");
+    if (s.cmodel) fail(FDMC_E_UNSUPPORTED, "control variates are not available for baskets");
+    if (n > BASKET_MAX) return run_basket_wide(c, s, out);
     if (p.kind != FDMC_PAYOFF_BASKET && p.kind != FDMC_PAYOFF_EUROPEAN)
         fail(FDMC_E_UNSUPPORTED, "basket models take a Basket (or European) payoff");
     if (s.cmodel) fail(FDMC_E_UNSUPPORTED, "control variates are not available for baskets");

@@ -44,6 +44,9 @@ hipError_t launch_is_gbm(const ISLaunch& a, int nwg, hipStream_t st);
 // k_basket.hip
 const void* basket_kernel(int mk, int src, int na);
 size_t basket_lds_bytes(int src, int D, int na);
+const void* basket_wide_kernel(int mk, int src, bool corr);
+int basket_wide_wg(bool corr);
+size_t basket_wide_lds_bytes(int n, bool corr);
 
 // k_book.hip
 // gm: 0 price, 1 adjoint Greeks, 2 tangent Greeks; occ 1: the three-waves-per-SIMD compilation

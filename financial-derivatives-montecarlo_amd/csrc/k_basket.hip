@@ -153,6 +153,132 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
     }
 }
 
+// ---- baskets of more than BASKET_MAX assets ----------------------------------------
+// Reference examples 07 (1000 uncorrelated local-vol assets, D = 20000) and 08
+// (100 correlated assets).  A lane is one path; normal j = k n + a of a path is
+// Sobol coordinate j of its point (direction records read from global memory,
+// L2/MALL resident) or Philox normal j of the path.
+//  - uncorrelated (128-lane workgroups): assets one after the other -- an asset's
+//    path needs only its own increments, so a lane carries only the basket sum;
+//  - correlated (64-lane workgroups, n <= WIDE_CORR_MAX): dates one after the
+//    other; the date's n scaled normals and the lane's n asset prices sit in LDS
+//    ([n][64] floats each), dW = L z row by row with L read uniformly.
+template <int MK>
+__device__ __forceinline__ float wide_step(const BasketWide& b, float S, float sg, float dw, int k) {
+    if constexpr (MK == MK_MULTI_GBM) {
+        return S + b.mu * S * b.h + sg * S * dw;
+    } else {
+        const float t = (float)k * b.h;
+        const float sl = b.sigma0 * fast_exp(-b.alpha * t) * fast_exp2(b.beta_m1 * fast_log2(S * b.inv_S0));
+        return S + b.mu * S * b.h + sl * S * dw;
+    }
+}
+
+template <int MK, int SRC, bool CORR>
+__global__ __launch_bounds__(CORR ? 64 : WIDE_WG) void k_basket_wide(BasketWide b) {
+    constexpr int WG = CORR ? 64 : WIDE_WG;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = blockIdx.y;
+    const int n = b.n, D = b.D;
+    constexpr bool ITAB = SRC == SRC_SOBOL;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) icdf_tab_load(itab, tid, WG);
+    __shared__ double red[(WG / 64) * 2];
+    float* zs = (float*)smem;                       // CORR: [n][WG] scaled normals of the date
+    float* Ss = zs + (size_t)n * WG;                // CORR: [n][WG] asset prices
+    __syncthreads();
+    SobolLane sl;
+    {
+        const uint32_t g = (uint32_t)(lane ^ (lane >> 1));
+        sl.rec = nullptr;
+        sl.grec = (SRC == SRC_SOBOL) ? b.sdir + (int64_t)r * D * SDIR_WORDS : nullptr;
+        sl.ga = g & 7u; sl.gb = g >> 3; sl.w = (uint32_t)(tid >> 6);
+        sl.hi_gray = 0; sl.lds_mode = false;
+    }
+    double acc[2] = {0.0, 0.0};
+    const int64_t last = b.first + b.count;
+    const int64_t tile0 = (b.first / WG) + (int64_t)blockIdx.x * b.tiles_per_wg;
+    for (int it = 0; it < b.tiles_per_wg; it++) {
+        const int64_t base = (tile0 + it) * WG;
+        if (base >= last) break;   // the same for every lane
+        const int64_t idx = base + tid;
+        const bool valid = idx >= b.first && idx < last;
+        if constexpr (SRC == SRC_SOBOL) {
+            const uint32_t hi = (uint32_t)base + (sl.w << 6);
+            sl.hi_gray = hi ^ (hi >> 1);
+        }
+        float B = 0.0f;
+        if constexpr (!CORR) {
+            for (int a = 0; a < n; a++) {
+                float S = b.S0[a];
+                const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
+                for (int k = 0; k < b.n_steps; k++) {
+                    const int j = k * n + a;
+                    float z;
+                    if constexpr (SRC == SRC_SOBOL) {
+                        z = icdf_u32_tab(sobol_coord(sl, j), itab);
+                    } else {
+                        float q[4];
+                        prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j >> 2), q);
+                        z = q[j & 3];
+                    }
+                    S = wide_step<MK>(b, S, sg, z * b.sqrt_h, k);
+                }
+                B += b.w[a] * S;
+            }
+        } else {
+            for (int a = 0; a < n; a++) Ss[a * WG + tid] = b.S0[a];
+            for (int k = 0; k < b.n_steps; k++) {
+                float q[4];
+                for (int c = 0; c < n; c++) {
+                    const int j = k * n + c;
+                    float z;
+                    if constexpr (SRC == SRC_SOBOL) {
+                        z = icdf_u32_tab(sobol_coord(sl, j), itab);
+                    } else {
+                        if (c == 0 || (j & 3) == 0)
+                            prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j >> 2), q);
+                        z = q[j & 3];
+                    }
+                    zs[c * WG + tid] = z * b.sqrt_h;
+                }
+                for (int a = 0; a < n; a++) {
+                    const float* La = b.L + (int64_t)a * n;
+                    float dw = 0.0f;
+                    for (int c = 0; c <= a; c++) dw += La[c] * zs[c * WG + tid];
+                    const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
+                    Ss[a * WG + tid] = wide_step<MK>(b, Ss[a * WG + tid], sg, dw, k);
+                }
+            }
+            for (int a = 0; a < n; a++) B += b.w[a] * Ss[a * WG + tid];
+        }
+        const float y = fmaxf(B - b.K, 0.0f) * b.discount;
+        if (valid) {
+            acc[0] += (double)y;
+            acc[1] += (double)y * (double)y;
+        }
+    }
+    block_sum<2, WG / 64>(acc, red);
+    if (tid == 0) {
+        double* out = b.partials + ((int64_t)r * b.nwg_x + blockIdx.x) * 2;
+        out[0] = acc[0];
+        out[1] = acc[1];
+    }
+}
+
+const void* basket_wide_kernel(int mk, int src, bool corr) {
+#define W(MKv, SRCv)                                                                              \
+    if (mk == MKv && src == SRCv)                                                                 \
+        return corr ? (const void*)&k_basket_wide<MKv, SRCv, true> : (const void*)&k_basket_wide<MKv, SRCv, false>;
+    W(MK_MULTI_GBM, SRC_SOBOL) W(MK_MULTI_GBM, SRC_PRNG) W(MK_MULTI_DUPIRE, SRC_SOBOL) W(MK_MULTI_DUPIRE, SRC_PRNG)
+#undef W
+    return nullptr;
+}
+
+int basket_wide_wg(bool corr) { return corr ? 64 : WIDE_WG; }
+size_t basket_wide_lds_bytes(int n, bool corr) { return corr ? (size_t)2 * n * 64 * sizeof(float) : 0; }
+
 template <int MK, int SRC>
 static const void* pick_na(int na) {
     if constexpr (SRC == SRC_PROD) {

@@ -125,6 +125,25 @@ struct BasketLaunch {
     double* partials;         // [R][nwg_x][2 + 2 n]
 };
 
+// Baskets of more than BASKET_MAX assets (k_basket.hip, k_basket_wide): per-asset
+// parameters in device arrays.
+constexpr int WIDE_WG = 128;        // lanes (paths) per workgroup (uncorrelated; correlated: 64)
+constexpr int WIDE_CORR_MAX = 256;  // correlated: the date's normals and prices sit in LDS (512 B per asset)
+struct BasketWide {
+    int n, n_steps, D;
+    float h, sqrt_h, mu, K, discount;
+    float sigma0, alpha, beta_m1, inv_S0;     // MULTI_DUPIRE
+    const float* S0;          // [n]
+    const float* sig;         // [n] (MULTI_GBM)
+    const float* w;           // [n] weights
+    const float* L;           // n x n lower Cholesky (row-major) or null
+    const uint32_t* sdir;     // SRC_SOBOL: [R][D][SDIR_WORDS]
+    uint32_t seed, tag;       // SRC_PRNG
+    int64_t first, count;
+    int tiles_per_wg, nwg_x;  // tiles of one workgroup's paths
+    double* partials;         // [R][nwg_x][2]
+};
+
 // Importance-sampled GBM European call (k_is.hip).
 struct ISLaunch {
     float S0, mu, sigma, h, sqrt_h, K, discount, zs;   // zs: per-step drift shift z* / sqrt(n)

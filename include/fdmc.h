@@ -292,7 +292,9 @@ fdmc_status fdmc_run_qmc_cv(fdmc_context* ctx, const fdmc_model* main_model,
  * R replicas over points [offset, offset + n_points) of each replica's scrambled
  * sequence (salt = salt_base + r).  sums_out: R x n_acc doubles per replica:
  *   n_acc = 2 {sum Y, sum Y^2} without a control, or 7 with one
- *   {Y, Y^2, Ycv, Ycv^2, Yc, Yc^2, Y*Yc} (methods_cuda.cu:628-639).
+ *   {Y, Y^2, Ycv, Ycv^2, Yc, Yc^2, Y*Yc} (methods_cuda.cu:628-639); baskets of
+ *   n <= 32 assets give 2 + 2 n (the delta slots are zero here), larger ones 2:
+ *   a buffer of R x 66 doubles always suffices.
  * ctrl_model/ctrl_payoff may be NULL (no control variate). */
 fdmc_status fdmc_qmc_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                 const fdmc_payoff* payoff, const fdmc_model* ctrl_model,

@@ -46,6 +46,11 @@ REPORT = [
      "kind": "mc500k", "n_steps": 256,
      "model": {"kind": "dupire", "S0": 100, "mu": 0.05, "sigma0": 0.2, "alpha": 0.5, "beta_d": 0.7, "T": 1},
      "payoff": {"kind": "european", "K": 100, "r": 0.05, "T": 1}},
+    {"example": "ejemplo07_basket_uncorr", "source": "informes_finales/informe_completo.txt:414", "price": 4.872409,
+     "kind": "mc5k", "n_steps": 20,
+     "model": {"kind": "multi_dupire", "n": 1000, "S0": 100, "mu": 0.05, "sigma0": 0.2, "alpha": 0.5, "beta_d": 0.7,
+               "T": 1, "uncorrelated": True},
+     "payoff": {"kind": "basket", "K": 100, "r": 0.05, "T": 1, "n": 1000}},
     {"example": "ejemplo11_otm_is", "source": "informes_finales/informe_completo.txt:605", "price": 0.028643,
      "kind": "black_scholes", "model": {"kind": "gbm", "S0": 100, "mu": 0.05, "sigma": 0.2, "T": 1},
      "payoff": {"kind": "european", "K": 180, "r": 0.05, "T": 1}},
