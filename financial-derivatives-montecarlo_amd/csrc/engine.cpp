@@ -407,6 +407,8 @@ int run_basket_wide(fdmc_context* c, const PassSpec& s, std::vector<double>& out
     const bool prng = s.prng || D64 > D_MAX_SOBOL;
     if (!prng && s.mode != FDMC_NOISE_RAW)
         fail(FDMC_E_UNSUPPORTED, "baskets of more than 32 assets take Raw noise (as the reference's examples 07/08)");
+    if (!prng && s.first + s.count > (int64_t)4294967296LL)
+        fail(FDMC_E_RANGE, "Sobol point index beyond 32 bits (offset + n > 2^32)");
     const bool corr = !m.uncorrelated && m.L_assets;
     if (corr && n > WIDE_CORR_MAX) fail(FDMC_E_UNSUPPORTED, "correlated baskets: at most 256 assets");
     BasketWide b{};
@@ -493,6 +495,8 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
     while (na < n) na <<= 1;
     const int64_t D64 = (int64_t)s.n_steps * n;
     const bool prng = s.prng || D64 > D_MAX_SOBOL;
+    if (!prng && s.first + s.count > (int64_t)4294967296LL)
+        fail(FDMC_E_RANGE, "Sobol point index beyond 32 bits (offset + n > 2^32)");
     const int D = (int)D64;
     BasketLaunch b{};
     b.n = n;
