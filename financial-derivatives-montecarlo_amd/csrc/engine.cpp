@@ -486,9 +486,6 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
         fail(FDMC_E_UNSUPPORTED, "basket models take a Basket (or European) payoff");
     if (s.cmodel) fail(FDMC_E_UNSUPPORTED, "control variates are not available for baskets");
     if (n > BASKET_MAX) return run_basket_wide(c, s, out);
-    if (p.kind != FDMC_PAYOFF_BASKET && p.kind != FDMC_PAYOFF_EUROPEAN)
-        fail(FDMC_E_UNSUPPORTED, "basket models take a Basket (or European) payoff");
-    if (s.cmodel) fail(FDMC_E_UNSUPPORTED, "control variates are not available for baskets");
     if (s.greeks && m.kind != FDMC_MODEL_MULTI_GBM)
         fail(FDMC_E_UNSUPPORTED, "basket deltas need GBM assets");
     int na = 4;

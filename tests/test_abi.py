@@ -55,6 +55,28 @@ def test_null_context_is_rejected_on_host():
     assert b"null" in lib.fdmc_last_error()
 
 
+def test_null_context_is_rejected_by_every_estimator():
+    lib = fdmc.load_library()
+    m, p, r = fdmc.GBMParams().to_c(), fdmc.European(180, 0.05, 1).to_c(), fdmc.CResult()
+    mc, ml, qc = fdmc.MCConfig().to_c(), fdmc.MLMCConfig().to_c(), fdmc.QMCConfig().to_c()
+    calls = [
+        lambda: lib.fdmc_run_mc(None, C.byref(m), C.byref(p), 1e-2, 64, C.byref(mc), C.byref(r)),
+        lambda: lib.fdmc_run_is(None, C.byref(m), C.byref(p), 2.8, 2e-3, C.byref(mc), C.byref(r)),
+        lambda: lib.fdmc_run_mlmc(None, C.byref(m), C.byref(p), 1e-2, C.byref(ml), C.byref(r)),
+        lambda: lib.fdmc_run_mlqmc(None, C.byref(m), C.byref(p), 1e-2, C.byref(ml), C.byref(qc), 2, None, 0,
+                                   None, 0, C.byref(r)),
+    ]
+    for call in calls:
+        lib.fdmc_last_error()
+        assert call() == 1
+        assert b"null" in lib.fdmc_last_error()
+    lv, rp = (C.c_int32 * 1)(0), (C.c_int32 * 1)(0)
+    fi, co = (C.c_int64 * 1)(0), (C.c_int64 * 1)(256)
+    out = (C.c_double * 4)()
+    assert lib.fdmc_levels_accumulate(None, C.byref(m), C.byref(p), 2, 1, 2, 1, 42, 1, lv, rp, fi, co, None, 0,
+                                      None, 0, out) == 1
+
+
 def test_context_create_without_device_reports_device_error():
     try:
         import torch
