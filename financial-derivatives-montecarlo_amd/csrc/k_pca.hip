@@ -13,7 +13,10 @@
 //               double buffering; tiles are assigned so the 8 column tiles of
 //               one row block run on the same XCD (its L2 keeps the A rows).
 //   k_gemm_nt256: the same with 256 x 256 tiles (4 waves of 128 x 128), used
-//               whenever the shape allows it.
+//               whenever the shape allows it; with EPI = true it stores no C
+//               and instead multiplies the Euler factors of a GBM basket per
+//               row, asset and 128-column slab (ProdEpi), so dW never reaches
+//               memory.
 #include "k_path.inl"
 
 namespace fdmc {
