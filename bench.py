@@ -175,9 +175,25 @@ def c3_gemm(eng, reps=5):
         tf = flops / (ms * 1e-3) / 1e12
         out["bf16" if bf16 else "fp16"] = {"gemm_ms": ms / n, "gemm_launches_per_pass": n // reps, "tflops": tf,
                                            "frac_mfma_peak": tf / MFMA_PEAK_TFLOPS, "pass_ms": 1e3 * wall,
-                                           "price": sum(r[0] for r in sums) / (16 << 18)}
+                                           "price": sum(r[0] for r in sums) / (16 << 18),
+                                           "gemm_rel_err_vs_fp64": gemm_error(eng, pca, M, bf16)}
     pca.free()
     return out
+
+
+def gemm_error(eng, pca, M, bf16, n=4096):
+    """Relative Frobenius error of the MFMA transform dW = Z M^T (4096 points of
+    replica salt 42) against an FP64 GEMM of the engine's own FP32 normals."""
+    import numpy as np
+    import torch
+
+    z = torch.empty((1024, n), dtype=torch.float32, device="cuda")      # [dim][point]
+    eng.sobol_normal(42, 1024, n, 0, z.data_ptr())
+    dw = torch.empty((n, 1024), dtype=torch.float32, device="cuda")
+    eng.pca_transform(pca, 42, 0, n, bf16, dw.data_ptr())
+    torch.cuda.synchronize()
+    ref = z.cpu().numpy().T.astype(np.float64) @ M.T
+    return float(np.linalg.norm(dw.cpu().numpy() - ref) / np.linalg.norm(ref))
 
 
 def c4_heston_mlmc(eng):
@@ -191,8 +207,13 @@ def c4_heston_mlmc(eng):
     for _ in range(3):
         r = eng.run_mlmc_cuda(hes, fdmc.European(100, 0.05, 1), 5e-4, cfg)
         ts.append(r.time_s)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from closed_forms import heston_call
+
+    cf = heston_call(100.0, 100.0, 1.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04)
     return {"eps": 5e-4, "time_to_eps_s": sorted(ts)[1], "price": r.price, "std_error": r.std_error,
-            "levels": r.levels, "path_steps": r.path_steps, "path_steps_per_s": r.path_steps / sorted(ts)[1]}
+            "levels": r.levels, "path_steps": r.path_steps, "path_steps_per_s": r.path_steps / sorted(ts)[1],
+            "closed_form": cf, "err_vs_closed_form": r.price - cf}
 
 
 def book_flops(contracts, res):
@@ -243,9 +264,10 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    torch = None
-    if args.copy_mib:
-        import torch   # before the engine: both use the same HIP runtime library
+    # device buffers of the copy calibration and the GEMM error check come from
+    # torch, which must initialise the HIP runtime before the engine does
+    import torch
+    torch.cuda.init()
 
     import fdmc
 
