@@ -297,6 +297,7 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     } else if (s.mode == FDMC_NOISE_PCA) {
         if (nf != 1) fail(FDMC_E_UNSUPPORTED, "PCA noise needs a one-factor model (methods_cuda.cu:1702-1704)");
         if (s.greeks) fail(FDMC_E_UNSUPPORTED, "adjoint Greeks are available with Raw and BB noise");
+        if (!s.pca || s.pca->m != D) fail(FDMC_E_INVALID, "PCA data must have m == n_steps * noise dimension");
         nz = NZ_ROWS;
         fn = path_kernel_rows(mk, pk, cv, cpk, 0);
     } else {
@@ -523,6 +524,7 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
         b.seed = s.salt_base;
         b.tag = s.prng ? s.tag : TAG_QMC_FALLBACK;
     } else if (s.mode == FDMC_NOISE_PCA) {
+        if (!s.pca || s.pca->m != D) fail(FDMC_E_INVALID, "PCA data must have m == n_steps * noise dimension");
         // GBM assets, a power-of-two count and the GEMM's 256-column tiling: the
         // Euler factors multiply inside the GEMM's epilogue (no dW in memory)
         const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
@@ -920,7 +922,6 @@ void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cac
             } else {
                 if (q.first + q.count > (int64_t)4294967296LL)
                     fail(FDMC_E_RANGE, "MLQMC Sobol offset beyond 32 bits (methods_cuda.cu:1847-1851)");
-                e.sdir = level_sdir(c, cache, ms, q.level, e.D) + (size_t)q.replica * e.D * SDIR_WORDS;
                 if (ms.mode == FDMC_NOISE_BB) {
                     if (q.level >= ms.n_bb || !ms.bb_list[q.level])
                         fail(FDMC_E_INVALID, "missing bb_list[" + std::to_string(q.level) + "]");
@@ -943,6 +944,12 @@ void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cac
             ent.push_back(e);
             which.push_back((int)i);
         }
+        // Sobol pass: direction records only once every request has been validated
+        if (pass == 0)
+            for (size_t k = 0; k < ent.size(); k++) {
+                const LevelReq& q = reqs[which[k]];
+                ent[k].sdir = level_sdir(c, cache, ms, q.level, ent[k].D) + (size_t)q.replica * ent[k].D * SDIR_WORDS;
+            }
         if (ent.empty()) continue;
         if (pass == 0 && ms.mode == FDMC_NOISE_PCA) {
             run_levels_pca(c, ms, reqs, ent, which, out);
