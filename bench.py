@@ -42,6 +42,13 @@ BOOK_REV_FLOPS = 44
 C2_FLOPS_PER_PATH_STEP = 31     # C2 forward: Euler 3 + BB 6 + Asian 1 + geo 1 + ICDF 20
 VALU_PEAK_TFLOPS = 157.3        # MI355X FP32 vector peak (packed)
 MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense FP16/BF16 matrix peak
+# transcendentals (exp, log, sqrt, rcp, sin, cos) issue at a quarter of the FP32
+# rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz / 4
+TRANS_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 4 / 1e12
+# algorithmic transcendentals per fine path-step of the book, frozen with the
+# flop counts: the local vol's log and the log of an inverse normal CDF (this
+# build's inverse CDF is a table with no log: the counters measure 1.65)
+BOOK_TRANS_PER_PATH_STEP = 2
 METRIC = "time-to-eps=1e-3 price+Greeks, batched vs plain (s); path-steps/s vs VALU peak"
 EPS = 1e-3
 BOOK_KERNEL = "k_book<LOCALVOL_QUAD, adjoint Greeks>"
@@ -72,7 +79,14 @@ def cpu_baseline(budget_s=15.0, eps=1e-2):
         ps += O.book_contract_run(model, book[n], n, eps, cfg).path_steps
         n += 1
     dt = time.perf_counter() - t
-    return {"value": ps / dt, "unit": "path-steps/s", "cores": O.num_threads(), "kind": "port",
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": ps / dt, "unit": "path-steps/s", "cores": O.num_threads(), "kind": "port", "cpu": cpu,
+            "host_cpus": os.cpu_count(),
             "sample": "contracts 0..%d of the book at eps=%g (price + Greeks), %.3g path-steps in %.1f s"
                       % (n - 1, eps, ps, dt)}
 
@@ -241,6 +255,11 @@ def book_roofline(eng, book, contracts):
             "kernel_avg_ms": k_avg_ms, "kernel_launches": launches, "kernel_total_ms": kms,
             "path_steps": st.path_steps, "flops": flops, "flops_per_path_step": flops / st.path_steps,
             "flops_forward_reverse": [BOOK_FWD_FLOPS, BOOK_REV_FLOPS],
+            "transcendental": {"per_path_step": BOOK_TRANS_PER_PATH_STEP,
+                               "achieved_tops": st.path_steps * BOOK_TRANS_PER_PATH_STEP / (kms * 1e-3) / 1e12,
+                               "peak_tops": TRANS_PEAK_TOPS,
+                               "frac": st.path_steps * BOOK_TRANS_PER_PATH_STEP / (kms * 1e-3) / 1e12
+                                       / TRANS_PEAK_TOPS},
             "variant": st.variant, "wg_per_cu": st.wg_per_cu}
 
 
