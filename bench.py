@@ -15,9 +15,11 @@ Printed line: value = path-steps per run / wall time per run of the timed
 steps; time_to_eps = batched (all contracts per pass) vs plain (one contract at
 a time, a host round trip per pass) on the same kernels, plus the forward-
 tangent variant of the Greeks; roofline of the persistent book kernel from HIP
-events around every launch of one event-timed run (80 algorithmic FP32 flops
-per fine path-step); the CPU oracle on a bounded sample of the same book.
-The earlier configurations are reported as extra fields (c2_asian_rqmc,
+events around every launch of one event-timed run (36 algorithmic FP32 flops
+per fine path-step forward, 44 more on the reverse sweeps of the contracts with
+pathwise Greeks: 42.8 on average); the CPU oracle on a bounded sample of the
+same book.
+The earlier configurations are reported as extra fields (c1_mc, c2_asian_rqmc,
 c3_pca_gemm, c4_heston_mlmc).
 
 --roofline-only runs just the event-timed book run with a fixed launch shape
@@ -210,6 +212,23 @@ def gemm_error(eng, pca, M, bf16, n=4096):
     return float(np.linalg.norm(dw.cpu().numpy() - ref) / np.linalg.norm(ref))
 
 
+def c1_mc(eng):
+    """Config 1: plain MC European call under GBM, 64 Euler steps, pilot 1e4 (seed
+    123), N = ceil(2 Var / eps^2) paths at eps = 1e-2, against Black-Scholes."""
+    import fdmc
+
+    g, e = fdmc.GBMParams(100.0, 0.05, 0.2, 1.0), fdmc.European(100.0, 0.05, 1.0)
+    eng.run_mc_cuda(g, e, 1e-2, 64, fdmc.MCConfig())
+    ts = []
+    for _ in range(3):
+        r = eng.run_mc_cuda(g, e, 1e-2, 64, fdmc.MCConfig())
+        ts.append(r.time_s)
+    t = sorted(ts)[1]
+    bs = fdmc.bs_call(100, 100, 1, 0.05, 0.2)
+    return {"eps": 1e-2, "time_to_eps_s": t, "price": r.price, "std_error": r.std_error, "n_paths": r.n_samples,
+            "path_steps_per_s": r.path_steps / t, "black_scholes": bs, "err_vs_black_scholes": r.price - bs}
+
+
 def c4_heston_mlmc(eng):
     """Config 4: Heston MLMC, M = 4, L <= 8, eps = 5e-4, pilot 1e4, European call."""
     import fdmc
@@ -270,11 +289,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--contracts", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the c2/c3/c4 fields")
+    ap.add_argument("--no-extras", action="store_true", help="skip the c1/c2/c3/c4 fields")
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=2)
-    ap.add_argument("--only", choices=["c2", "c3", "c4"], default=None,
+    ap.add_argument("--only", choices=["c1", "c2", "c3", "c4"], default=None,
                     help="run just one of the extra configurations and print its fields")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="roofline-only: also copy this many MiB device to device (HBM counter calibration)")
@@ -293,7 +312,7 @@ def main():
     model, contracts = fdmc.local_vol_book(args.contracts, 2026)
     eng = fdmc.Engine(0)
     if args.only:
-        fn = {"c2": c2_asian, "c3": c3_gemm, "c4": c4_heston_mlmc}[args.only]
+        fn = {"c1": c1_mc, "c2": c2_asian, "c3": c3_gemm, "c4": c4_heston_mlmc}[args.only]
         out = {args.only: fn(eng)}
         eng.close()
         print(json.dumps(out))
@@ -376,6 +395,8 @@ def main():
                         "contracts_with_greeks": len(lip)},
     }
     if not args.no_extras:
+        log("c1 mc")
+        out["c1_mc"] = c1_mc(eng)
         log("c2 asian")
         out["c2_asian_rqmc"] = c2_asian(eng)
         log("c3 gemm")
