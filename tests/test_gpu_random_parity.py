@@ -1,0 +1,72 @@
+"""Seeded random parity sweep of the fused path kernels: models (GBM, Dupire,
+quadratic local vol, Heston), payoffs (European, Asian, geometric Asian,
+lookback, barrier), noise (raw Sobol, Brownian bridge), odd step counts,
+replicas, unaligned point ranges and parameters drawn from a fixed numpy
+generator, each pass against the oracle's restatement of kernel_mc
+(methods_cuda.cu:414-437) on the same scrambled Sobol points, to the
+tolerances of the fixed-case tests (replica means within 2e-5 relative for
+Lipschitz payoffs, a few max-payoff/N more for a barrier's indicator flips).
+"""
+import numpy as np
+import pytest
+
+import fdmc
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(n=24, seed=20261204):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        T = float(rng.choice([0.25, 0.5, 1.0, 2.0]))
+        mu = float(rng.uniform(0.0, 0.08))
+        kind = int(rng.integers(4))
+        if kind == 0:
+            model = fdmc.GBMParams(100.0, mu, float(rng.uniform(0.1, 0.5)), T)
+        elif kind == 1:
+            model = fdmc.DupireLocalParams(100.0, mu, float(rng.uniform(0.1, 0.3)), float(rng.uniform(0.0, 1.0)),
+                                           float(rng.uniform(0.5, 1.0)), T)
+        elif kind == 2:
+            model = fdmc.LocalVolQuadParams(100.0, mu, float(rng.uniform(0.1, 0.3)), float(rng.uniform(0.0, 0.2)),
+                                            float(rng.uniform(0.0, 2.0)), 100.0, T)
+        else:
+            model = fdmc.HestonParams(100.0, mu, float(rng.uniform(0.5, 3.0)), float(rng.uniform(0.02, 0.09)),
+                                      float(rng.uniform(0.1, 0.7)), float(rng.uniform(-0.9, 0.2)),
+                                      float(rng.uniform(0.02, 0.09)), T).compute_cholesky()
+        K = float(rng.uniform(70.0, 130.0))
+        pk = int(rng.integers(5))
+        payoff = [fdmc.European(K, mu, T), fdmc.Asian(K), fdmc.GeomAsian(K), fdmc.Lookback(0.2),
+                  fdmc.Barrier(K, K * float(rng.uniform(1.1, 1.5)), 0.2, mu, T)][pk]
+        bb = kind != 3 and rng.random() < 0.5
+        n_steps = int(rng.choice([1, 2, 4, 8, 16, 32, 64, 128, 256])) if bb else \
+            int(rng.choice([1, 2, 3, 5, 7, 16, 33, 64, 100, 255]))
+        R = int(rng.integers(1, 4))
+        first = int(rng.integers(0, 5000))
+        count = int(rng.integers(64, 1500))
+        out.append(("c%02d-%s-%s-%s-%d" % (i, ["gbm", "dup", "lvq", "hes"][kind],
+                                           ["eur", "asi", "geo", "lkb", "bar"][pk], "bb" if bb else "raw", n_steps),
+                    model, payoff, n_steps, fdmc.NOISE_BB if bb else fdmc.NOISE_RAW, R, first, count))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name,model,payoff,n_steps,mode,R,first,count", CASES, ids=[c[0] for c in CASES])
+def test_random_pass_parity(engine, name, model, payoff, n_steps, mode, R, first, count):
+    bb = engine.bb_upload(fdmc.bb_precompute(n_steps, model.T)) if mode == fdmc.NOISE_BB else None
+    got = np.array(engine.qmc_accumulate(model, payoff, n_steps, R, 77, first, count, mode, dev_bb=bb))
+    if bb is not None:
+        bb.free()
+    obb = O.OracleBB(n_steps, model.T) if mode == fdmc.NOISE_BB else None
+    ref = O.qmc_accumulate(model, payoff, n_steps, R, 77, first, count, mode, bb=obb)
+    g, r = got[:, 0] / count, ref[:, 0] / count
+    tol = 2e-5 * np.maximum(1.0, np.abs(r)) + 1e-6
+    if isinstance(payoff, fdmc.Barrier):
+        tol = tol + 4.0 * 40.0 / count
+    assert (np.abs(g - r) <= tol).all(), (name, g, r)
+    g2, r2 = got[:, 1] / count, ref[:, 1] / count
+    tol2 = 1e-4 * np.maximum(1.0, np.abs(r2)) + (4.0 * 1600.0 / count if isinstance(payoff, fdmc.Barrier) else 1e-6)
+    assert (np.abs(g2 - r2) <= tol2).all(), (name, g2, r2)
