@@ -70,3 +70,66 @@ def test_random_pass_parity(engine, name, model, payoff, n_steps, mode, R, first
     g2, r2 = got[:, 1] / count, ref[:, 1] / count
     tol2 = 1e-4 * np.maximum(1.0, np.abs(r2)) + (4.0 * 1600.0 / count if isinstance(payoff, fdmc.Barrier) else 1e-6)
     assert (np.abs(g2 - r2) <= tol2).all(), (name, g2, r2)
+
+
+def _level_cases(n=16, seed=1204):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        T = float(rng.choice([0.5, 1.0, 2.0]))
+        mu = float(rng.uniform(0.0, 0.08))
+        kind = int(rng.integers(4))
+        if kind == 0:
+            model = fdmc.GBMParams(100.0, mu, float(rng.uniform(0.1, 0.5)), T)
+        elif kind == 1:
+            model = fdmc.DupireLocalParams(100.0, mu, float(rng.uniform(0.1, 0.3)), float(rng.uniform(0.0, 1.0)),
+                                           float(rng.uniform(0.5, 1.0)), T)
+        elif kind == 2:
+            model = fdmc.LocalVolQuadParams(100.0, mu, float(rng.uniform(0.1, 0.3)), float(rng.uniform(0.0, 0.2)),
+                                            float(rng.uniform(0.0, 2.0)), 100.0, T)
+        else:
+            model = fdmc.HestonParams(100.0, mu, float(rng.uniform(0.5, 3.0)), float(rng.uniform(0.02, 0.09)),
+                                      float(rng.uniform(0.1, 0.7)), float(rng.uniform(-0.9, 0.2)),
+                                      float(rng.uniform(0.02, 0.09)), T).compute_cholesky()
+        K = float(rng.uniform(80.0, 120.0))
+        pk = int(rng.integers(4))   # Lipschitz payoffs: level sums to the fixed-case tolerances
+        payoff = [fdmc.European(K, mu, T), fdmc.Asian(K), fdmc.GeomAsian(K), fdmc.Lookback(0.2)][pk]
+        M = int(rng.choice([2, 4]))
+        lmax = 5 if M == 2 else 3
+        qmc = bool(rng.random() < 0.6)
+        mode = fdmc.NOISE_BB if (qmc and kind != 3 and rng.random() < 0.5) else fdmc.NOISE_RAW
+        levels = sorted(set(int(x) for x in rng.integers(0, lmax + 1, size=3)))
+        R = int(rng.integers(1, 4)) if qmc else 1
+        first = int(rng.integers(0, 3000))
+        count = int(rng.integers(100, 900))
+        out.append(("l%02d-%s-%s-M%d-%s" % (i, ["gbm", "dup", "lvq", "hes"][kind], ["eur", "asi", "geo", "lkb"][pk], M,
+                                             ("bb" if mode == fdmc.NOISE_BB else "sobol") if qmc else "prng"),
+                    model, payoff, M, qmc, mode, levels, R, first, count))
+    return out
+
+
+LEVEL_CASES = _level_cases()
+
+
+@pytest.mark.parametrize("name,model,payoff,M,qmc,mode,levels,R,first,count", LEVEL_CASES,
+                         ids=[c[0] for c in LEVEL_CASES])
+def test_random_level_sums_parity(engine, name, model, payoff, M, qmc, mode, levels, R, first, count):
+    """Coupled fine/coarse level sums of the persistent multilevel kernel
+    (kernel_mlmc, methods_cuda.cu:445-518) against the oracle on the same
+    Philox streams or scrambled Sobol points."""
+    bbl = [engine.bb_upload(fdmc.bb_precompute(M ** l, model.T)) for l in range(max(levels) + 1)] \
+        if mode == fdmc.NOISE_BB else []
+    reqs = [(l, r, first, count) for l in levels for r in range(R)]
+    got = engine.levels_accumulate(model, payoff, M, reqs, qmc=qmc, mode=mode, R=R, salt_or_seed=91, bb_list=bbl)
+    for b in bbl:
+        b.free()
+    for (l, r, f, n), g in zip(reqs, got):
+        if qmc:
+            obb = O.OracleBB(M ** l, model.T) if mode == fdmc.NOISE_BB else None
+            ref = O.mlqmc_level_sums(model, payoff, M, l, R, 91, f, n, mode, bb=obb)[r]
+        else:
+            ref = O.mlmc_level_sums(model, payoff, M, l, 91, f, n)
+        g, ref = np.asarray(g) / n, np.asarray(ref) / n
+        tol = 2e-5 * max(1.0, abs(ref[2])) + 1e-6
+        assert abs(g[0] - ref[0]) <= tol and abs(g[2] - ref[2]) <= tol, (name, l, r, g, ref)
+        assert abs(g[3] - ref[3]) <= 1e-4 * abs(ref[3]) + 1e-6, (name, l, r, g, ref)
