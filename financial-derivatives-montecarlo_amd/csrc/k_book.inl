@@ -254,6 +254,10 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
             gc.m -= g.T * pc;
         }
         if (nc == 0) gc = Tan3{0.0f, 0.0f, 0.0f};
+        // a path that left the Euler scheme's domain (S <= 0 under the local vol)
+        // pays 0 through fmaxf and contributes 0 to the Greeks too
+        if (!(fabsf(gf.s) + fabsf(gf.v) + fabsf(gf.m) + fabsf(gc.s) + fabsf(gc.v) + fabsf(gc.m) < INFINITY))
+            gf = gc = Tan3{0.0f, 0.0f, 0.0f};
     }
     if (valid) {
         acc[0] += (double)(pf - pc);

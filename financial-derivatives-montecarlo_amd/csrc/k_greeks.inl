@@ -245,9 +245,13 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                 }
             }
             if (kmin == 0) adj += c_min;     // running minimum at S0
-            const float delta = adj + s0b;
+            float delta = adj + s0b;
             float rho = mb;
             if constexpr (PK == PK_EUROPEAN) rho -= a.T * ym;   // d/dr of the discount e^{-rT}
+            // a path whose state left the Euler scheme's domain (S <= 0 under a
+            // local vol: its log is NaN) pays 0 through fmaxf in the price and
+            // contributes 0 to the Greeks as well
+            if (!(fabsf(delta) + fabsf(vb) + fabsf(rho) < INFINITY)) delta = vb = rho = 0.0f;
 
             if (valid) {
                 const double dm = (double)ym;

@@ -827,7 +827,10 @@ void orc_book_path(const fdmc_model* m, const fdmc_book_contract* k, const float
         book_leg(&P, k->kind, dW, nf / 2, 1, hc, book_bgk(k, hc), (float)k->T, &yc, gc);
     }
     out[0] = yf - yc;
-    for (int i = 0; i < 3; i++) out[1 + i] = gf[i] - gc[i];
+    /* a path that left the Euler scheme's domain contributes 0 to the Greeks */
+    float mag = 0.0f;
+    for (int i = 0; i < 3; i++) mag += fabsf(gf[i]) + fabsf(gc[i]);
+    for (int i = 0; i < 3; i++) out[1 + i] = (mag < INFINITY) ? gf[i] - gc[i] : 0.0f;
 }
 
 static int book_levels(const fdmc_book_contract* k) {

@@ -133,3 +133,66 @@ def test_random_level_sums_parity(engine, name, model, payoff, M, qmc, mode, lev
         tol = 2e-5 * max(1.0, abs(ref[2])) + 1e-6
         assert abs(g[0] - ref[0]) <= tol and abs(g[2] - ref[2]) <= tol, (name, l, r, g, ref)
         assert abs(g[3] - ref[3]) <= 1e-4 * abs(ref[3]) + 1e-6, (name, l, r, g, ref)
+
+
+def _greek_cases(n=8, seed=4242):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        kind = int(rng.integers(3))
+        T = float(rng.choice([0.5, 1.0, 2.0]))
+        mu0 = float(rng.uniform(0.01, 0.08))
+        v0 = float(rng.uniform(0.15, 0.35))
+        extra = (float(rng.uniform(0.0, 1.0)), float(rng.uniform(0.5, 1.0))) if kind == 1 else \
+            (float(rng.uniform(0.0, 0.2)), float(rng.uniform(0.0, 2.0)))
+
+        def make(S0=100.0, mu=mu0, vol=v0, kind=kind, T=T, extra=extra):
+            if kind == 0:
+                return fdmc.GBMParams(S0, mu, vol, T)
+            if kind == 1:
+                return fdmc.DupireLocalParams(S0, mu, vol, extra[0], extra[1], T)
+            return fdmc.LocalVolQuadParams(S0, mu, vol, extra[0], extra[1], 100.0, T)
+
+        K = float(rng.uniform(85.0, 115.0))
+        pk = int(rng.integers(4))
+        bb = bool(rng.random() < 0.5)
+        n_steps = int(rng.choice([8, 16, 32, 64])) if bb else int(rng.choice([5, 12, 31, 64]))
+        out.append(("g%02d-%s-%s-%s-%d" % (i, ["gbm", "dup", "lvq"][kind], ["eur", "asi", "geo", "lkb"][pk],
+                                            "bb" if bb else "raw", n_steps),
+                    make, pk, K, mu0, v0, T, n_steps, fdmc.NOISE_BB if bb else fdmc.NOISE_RAW))
+    return out
+
+
+GREEK_CASES = _greek_cases()
+
+
+def _payoff(pk, K, r, T):
+    return [fdmc.European(K, r, T), fdmc.Asian(K), fdmc.GeomAsian(K), fdmc.Lookback(0.2)][pk]
+
+
+@pytest.mark.parametrize("name,make,pk,K,mu0,v0,T,n_steps,mode", GREEK_CASES, ids=[c[0] for c in GREEK_CASES])
+def test_random_adjoint_vs_crn_differences(engine, name, make, pk, K, mu0, v0, T, n_steps, mode):
+    """Pathwise-adjoint delta, vega and rho against common-random-number central
+    differences of the price on the same Sobol points (rho moves mu and r)."""
+    R, n = 2, 1 << 13
+    bb = engine.bb_upload(fdmc.bb_precompute(n_steps, T)) if mode == fdmc.NOISE_BB else None
+
+    def price(model, r):
+        return np.array(engine.qmc_accumulate(model, _payoff(pk, K, r, T), n_steps, R, 42, 0, n, mode,
+                                              dev_bb=bb))[:, 0] / n
+
+    rows = np.array(engine.greeks_accumulate(make(), _payoff(pk, K, mu0, T), n_steps, R, 42, 0, n, mode, dev_bb=bb))
+    bumps = {"delta": 0.05, "vega": 0.002, "rho": 0.002}
+    for k, which in enumerate(("delta", "vega", "rho")):
+        h = bumps[which]
+        if which == "delta":
+            up, dn = price(make(S0=100.0 + h), mu0), price(make(S0=100.0 - h), mu0)
+        elif which == "vega":
+            up, dn = price(make(vol=v0 + h), mu0), price(make(vol=v0 - h), mu0)
+        else:
+            up, dn = price(make(mu=mu0 + h), mu0 + h), price(make(mu=mu0 - h), mu0 - h)
+        fd = (up - dn) / (2 * h)
+        adj = rows[:, 2 + 2 * k] / n
+        assert np.all(np.abs(adj - fd) <= 0.01 * np.abs(fd) + 2e-3), (name, which, adj, fd)
+    if bb:
+        bb.free()
