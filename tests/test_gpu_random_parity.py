@@ -196,3 +196,52 @@ def test_random_adjoint_vs_crn_differences(engine, name, make, pk, K, mu0, v0, T
         assert np.all(np.abs(adj - fd) <= 0.01 * np.abs(fd) + 2e-3), (name, which, adj, fd)
     if bb:
         bb.free()
+
+
+def _basket_cases(n=12, seed=777):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        na = int(rng.choice([2, 3, 5, 8, 13, 16, 32, 40, 64]))
+        steps = int(rng.choice([1, 4, 12, 32]))
+        gbm = bool(rng.random() < 0.5)
+        corr = bool(rng.random() < 0.6)
+        L = None
+        if corr:
+            Bf = rng.uniform(0.2, 0.5, (na, 3))
+            Cm = Bf @ Bf.T
+            np.fill_diagonal(Cm, 1.0)
+            L = np.linalg.cholesky(Cm).ravel()
+        S0 = list(rng.uniform(80.0, 120.0, na))
+        if gbm:
+            model = fdmc.MultiGBMParams(na, S0, list(rng.uniform(0.1, 0.4, na)), 0.05, 1.0, L=L, uncorrelated=not corr)
+        else:
+            model = fdmc.MultiDupireParams(na, 0.05, 0.2, 0.5, 0.7, 1.0, uncorrelated=not corr, S0=S0, L=L)
+        w = list(rng.dirichlet(np.ones(na)))
+        payoff = fdmc.Basket(float(rng.uniform(90.0, 110.0)), 0.05, 1.0, na, weights=w)
+        prng = bool(rng.random() < 0.3) or na % 4 == 0 and rng.random() < 0.2
+        if prng and na <= 32 and na % 4:
+            prng = False   # pseudo-random baskets up to 32 assets need a multiple of 4
+        out.append(("b%02d-%s-n%d-s%d-%s-%s" % (i, "gbm" if gbm else "dup", na, steps, "corr" if corr else "ind",
+                                                 "prng" if prng else "sobol"), model, payoff, steps, prng))
+    return out
+
+
+BASKET_CASES = _basket_cases()
+
+
+@pytest.mark.parametrize("name,model,payoff,steps,prng", BASKET_CASES, ids=[c[0] for c in BASKET_CASES])
+def test_random_basket_parity(engine, name, model, payoff, steps, prng):
+    """Baskets of 2 to 64 assets (in registers up to 32, the wide kernel above),
+    weights, per-asset S0 and sigma, Cholesky correlation or none, against the
+    oracle's kernel_multi_dupire restatement on the same points or Philox paths."""
+    if prng:
+        gm, gv = engine.run_mc_fixed(model, payoff, steps, 1500, 5)
+        rm, rv = O.run_mc_fixed(model, payoff, steps, 1500, 5)
+        assert abs(gm - rm) <= 2e-5 * max(1.0, abs(rm)) + 1e-6, (name, gm, rm)
+        assert abs(gv - rv) <= 1e-3 * rv + 1e-12, (name, gv, rv)
+    else:
+        R, cnt = 2, 700
+        got = np.array(engine.qmc_accumulate(model, payoff, steps, R, 42, 33, cnt, fdmc.NOISE_RAW))
+        ref = O.qmc_accumulate(model, payoff, steps, R, 42, 33, cnt, fdmc.NOISE_RAW)
+        assert np.allclose(got[:, 0] / cnt, ref[:, 0] / cnt, rtol=2e-5, atol=1e-6), (name, got[:, 0], ref[:, 0])
