@@ -6,6 +6,8 @@ generator, each pass against the oracle's restatement of kernel_mc
 (methods_cuda.cu:414-437) on the same scrambled Sobol points, to the
 tolerances of the fixed-case tests (replica means within 2e-5 relative for
 Lipschitz payoffs, a few max-payoff/N more for a barrier's indicator flips).
+The same generator draws multilevel requests, baskets, Greek cases and a
+contract book, each checked against its oracle or CRN differences.
 """
 import numpy as np
 import pytest
@@ -245,3 +247,45 @@ def test_random_basket_parity(engine, name, model, payoff, steps, prng):
         got = np.array(engine.qmc_accumulate(model, payoff, steps, R, 42, 33, cnt, fdmc.NOISE_RAW))
         ref = O.qmc_accumulate(model, payoff, steps, R, 42, 33, cnt, fdmc.NOISE_RAW)
         assert np.allclose(got[:, 0] / cnt, ref[:, 0] / cnt, rtol=2e-5, atol=1e-6), (name, got[:, 0], ref[:, 0])
+
+
+def _random_book(seed=99, n=16):
+    rng = np.random.default_rng(seed)
+    model = fdmc.LocalVolQuadParams(S0=100.0, mu=float(rng.uniform(0.0, 0.08)), a=float(rng.uniform(0.15, 0.3)),
+                                    b=float(rng.uniform(0.0, 0.2)), c=float(rng.uniform(0.0, 2.0)), s_ref=100.0,
+                                    T=1.0)
+    book = []
+    for _ in range(n):
+        kind = [fdmc.PAYOFF_EUROPEAN, fdmc.PAYOFF_ASIAN, fdmc.PAYOFF_LOOKBACK, fdmc.PAYOFF_BARRIER][rng.integers(4)]
+        K = float(rng.uniform(70.0, 130.0))
+        book.append(fdmc.BookContract(kind, int(2 ** rng.integers(0, 10)), K, float(rng.uniform(1.15, 1.4)) * K,
+                                      float(rng.choice([0.25, 0.5, 1.0, 2.0])), model.mu, 0.2))
+    return model, book
+
+
+@pytest.mark.parametrize("gm", [1, 2])
+def test_random_book_level_sums_match_oracle(engine, gm):
+    """A seeded random book (local-vol parameters, payoff kinds, strikes,
+    barriers, maturities, 1-512 dates): the adjoint (gm=1) and tangent (gm=2)
+    kernels' per-(contract, level, replica) price and Greek sums against the
+    book oracle on the same points, to the fixed-book tolerances (a barrier's
+    knock-out and a payoff kink may flip for a point on the edge to FP32
+    rounding)."""
+    model, book = _random_book()
+    R, n, off = 4, 512, 4097
+    b = engine.book_create(model, book, fdmc.BookConfig(R=R, greeks=gm))
+    try:
+        for c, k in enumerate(book):
+            L = int(np.log2(k.n_steps))
+            for l in sorted({0, L // 2, L}):
+                got = np.array(b.level_sums(c, l, off, n))
+                ref = O.book_level_sums(model, k, c, l, R, 42, off, n)
+                tol = 2e-4 * n + (2 * k.K if k.kind == fdmc.PAYOFF_BARRIER else 0.0)
+                np.testing.assert_allclose(got[:, 0], ref[:, 0], atol=tol, rtol=1e-6,
+                                           err_msg="contract %d kind %d n %d level %d" % (c, k.kind, k.n_steps, l))
+                if k.kind != fdmc.PAYOFF_BARRIER:
+                    d = np.abs(got[:, 1:] - ref[:, 1:])
+                    bad = d > 2e-4 * n + 1e-4 * np.abs(ref[:, 1:])
+                    assert bad.sum(axis=0).max() <= 2 and d.max() <= 600.0, (c, l, got, ref)
+    finally:
+        b.free()
