@@ -51,7 +51,7 @@ CModel to_c(const ModelVariant& mv) {
             r.m.kind = FDMC_MODEL_GBM; r.m.S0 = m.S0; r.m.sigma = m.sigma;
         } else if constexpr (std::is_same_v<T, HestonParams>) {
             r.m.kind = FDMC_MODEL_HESTON; r.m.S0 = m.S0; r.m.kappa = m.kappa; r.m.theta = m.theta;
-            r.m.xi = m.xi; r.m.rho = m.rho; r.m.v0 = m.v0;
+            r.m.xi = m.xi; r.m.rho = m.rho; r.m.v0 = m.v0; r.m.heston_scheme = m.scheme;
             r.m.L[0] = m.L[0][0]; r.m.L[1] = m.L[0][1]; r.m.L[2] = m.L[1][0]; r.m.L[3] = m.L[1][1];
         } else if constexpr (std::is_same_v<T, DupireLocalParams>) {
             r.m.kind = FDMC_MODEL_DUPIRE; r.m.S0 = m.S0; r.m.sigma0 = m.sigma0; r.m.alpha = m.alpha;

@@ -21,6 +21,9 @@ struct GBMParams {          // dS = mu S dt + sigma S dW
 struct HestonParams {       // dS = mu S dt + sqrt(v) S dW1; dv = kappa (theta - v) dt + xi sqrt(v) dW2
     double S0 = 100.0, mu = 0.05, kappa = 2.0, theta = 0.04, xi = 0.5, rho = -0.9, v0 = 0.04, T = 1.0;
     double L[2][2] = {};    // Cholesky factor of [[1, rho], [rho, 1]]
+    // variance scheme: 0 = the reference's full-truncation exponential Euler,
+    // 1 = with the Milstein term xi^2/4 (dW2^2 - h) (no counterpart in the reference)
+    int scheme = 0;
     void compute_cholesky() {
         L[0][0] = 1.0;
         L[0][1] = 0.0;

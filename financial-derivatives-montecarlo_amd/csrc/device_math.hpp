@@ -153,12 +153,16 @@ __device__ __forceinline__ float step_1f(const LegParams& P, float S, float dw, 
 }
 
 // Full-truncation exponential-Euler Heston step (methods_cuda.cu:278-284) with
-// e^{-kappa h} precomputed.
+// e^{-kappa h} precomputed; MIL adds the Milstein term mil_q (dW2^2 - h),
+// mil_q = xi^2/4, of the optional scheme.
+template <bool MIL>
 __device__ __forceinline__ void step_heston(const LegParams& P, float& S, float& V, float dw1,
                                             float dw2, float h, float em) {
     const float sq = fast_sqrt(fmaxf(V, 0.0f));
     S = S + P.mu * S * h + sq * S * dw1;
-    V = P.theta + em * (V - P.theta) + P.xi * sq * dw2;
+    const float v = P.theta + em * (V - P.theta) + P.xi * sq * dw2;
+    if constexpr (MIL) V = fmaf(P.mil_q, dw2 * dw2 - h, v);
+    else V = v;
 }
 
 // ---- payoff accumulators (methods_cuda.cu:298-352) --------------------------

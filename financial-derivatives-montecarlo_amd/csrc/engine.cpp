@@ -99,6 +99,9 @@ void validate_model(const fdmc_model* m) {
     if (m->kind < 0 || m->kind > FDMC_MODEL_MULTI_GBM) fail(FDMC_E_INVALID, "unknown model kind");
     if (!(m->T > 0.0)) fail(FDMC_E_INVALID, "model T must be positive");
     if (!is_basket(*m) && !(m->S0 > 0.0)) fail(FDMC_E_INVALID, "model S0 must be positive");
+    if (m->kind == FDMC_MODEL_HESTON && m->heston_scheme != FDMC_HESTON_EXP_EULER &&
+        m->heston_scheme != FDMC_HESTON_MILSTEIN)
+        fail(FDMC_E_INVALID, "unknown Heston variance scheme");
     if (is_basket(*m)) {
         if (m->n_assets <= 0) fail(FDMC_E_INVALID, "basket n_assets must be positive");
         if (!m->S0_assets) fail(FDMC_E_INVALID, "basket S0_assets is null");
@@ -134,7 +137,8 @@ LegParams make_leg(const fdmc_model& m, const fdmc_payoff& p) {
     case FDMC_MODEL_GBM: L.sigma = (float)m.sigma; break;
     case FDMC_MODEL_HESTON:
         L.kappa = (float)m.kappa; L.theta = (float)m.theta; L.xi = (float)m.xi; L.v0 = (float)m.v0;
-        L.L10 = (float)m.L[2]; L.L11 = (float)m.L[3]; break;
+        L.L10 = (float)m.L[2]; L.L11 = (float)m.L[3];
+        L.mil_q = m.heston_scheme == FDMC_HESTON_MILSTEIN ? 0.25f * L.xi * L.xi : 0.0f; break;
     case FDMC_MODEL_DUPIRE:
         L.sigma0 = (float)m.sigma0; L.alpha = (float)m.alpha;
         L.beta_m1 = (float)m.beta_d - 1.0f; L.inv_S0 = 1.0f / (float)m.S0;
@@ -155,7 +159,7 @@ float bgk_factor(const fdmc_payoff& p, float h) {
 int model_kind_dev(const fdmc_model& m) {
     switch (m.kind) {
     case FDMC_MODEL_GBM: return MK_GBM;
-    case FDMC_MODEL_HESTON: return MK_HESTON;
+    case FDMC_MODEL_HESTON: return m.heston_scheme == FDMC_HESTON_MILSTEIN ? MK_HESTON_MIL : MK_HESTON;
     case FDMC_MODEL_DUPIRE: return MK_DUPIRE;
     case FDMC_MODEL_LOCALVOL_QUAD: return MK_LVQ;
     case FDMC_MODEL_MULTI_DUPIRE: return MK_MULTI_DUPIRE;
@@ -174,7 +178,8 @@ bool same_model(const fdmc_model& a, const fdmc_model& b) {
     return a.kind == b.kind && a.S0 == b.S0 && a.mu == b.mu && a.T == b.T && a.sigma == b.sigma &&
            a.sigma0 == b.sigma0 && a.alpha == b.alpha && a.beta_d == b.beta_d && a.lv_a == b.lv_a &&
            a.lv_b == b.lv_b && a.lv_c == b.lv_c && a.lv_sref == b.lv_sref && a.kappa == b.kappa &&
-           a.theta == b.theta && a.xi == b.xi && a.v0 == b.v0;
+           a.theta == b.theta && a.xi == b.xi && a.v0 == b.v0 &&
+           (a.kind != FDMC_MODEL_HESTON || a.heston_scheme == b.heston_scheme);
 }
 
 // ---------------------------------------------------------------- device helpers

@@ -47,10 +47,10 @@ struct Coupled {
         const float dw2 = (P.L10 * z1 + P.L11 * z2) * E.sqrt_hf;
         a1 += dw1;
         a2 += dw2;
-        step_heston(P, Sf, Vf, dw1, dw2, E.h_f, E.em_f);
+        step_heston<MK == MK_HESTON_MIL>(P, Sf, Vf, dw1, dw2, E.h_f, E.em_f);
         if constexpr (is_path_dep<PK>()) run_update<PK>(rf, Sf);
         if (++cnt == M) {
-            step_heston(P, Sc, Vc, a1, a2, E.h_c, E.em_c);
+            step_heston<MK == MK_HESTON_MIL>(P, Sc, Vc, a1, a2, E.h_c, E.em_c);
             if constexpr (is_path_dep<PK>()) run_update<PK>(rc, Sc);
             a1 = a2 = 0.0f;
             cnt = 0;
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             for (int k = 0; k < nfine; k++) cp.fine1(P, E, src[k], k, M);
         } else if constexpr (NZ == NZ_PRNG) {
             float z[4];
-            if constexpr (MK == MK_HESTON) {
+            if constexpr (is_heston(MK)) {
                 for (int b = 0; 2 * b < nfine; b++) {
                     prng_normals4(E.seed, E.tag, (uint64_t)idx, (uint32_t)b, z);
                     cp.fineH(P, E, z[0], z[1], M);
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
             }
         } else if constexpr (NZ == NZ_SOBOL_RAW) {
-            if constexpr (MK == MK_HESTON) {
+            if constexpr (is_heston(MK)) {
                 for (int k = 0; k < nfine; k++)
                     cp.fineH(P, E, icdf_u32_tab(sobol_coord(sl, 2 * k), itab), icdf_u32_tab(sobol_coord(sl, 2 * k + 1), itab), M);
             } else {

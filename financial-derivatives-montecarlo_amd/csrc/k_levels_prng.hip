@@ -8,7 +8,7 @@ const void* levels_kernel_prng(int mk, int pk) {
     if (mk == MKv && pk == PKv) return (const void*)&k_levels<MKv, PKv, NZ_PRNG, 0>;
 #define XP(MKv) X(MKv, PK_EUROPEAN) X(MKv, PK_ASIAN) X(MKv, PK_GEOM_ASIAN) X(MKv, PK_LOOKBACK) X(MKv, PK_BARRIER)
     XP(MK_GBM) XP(MK_DUPIRE) XP(MK_LVQ)
-    XP(MK_HESTON)
+    XP(MK_HESTON) XP(MK_HESTON_MIL)
 #undef XP
 #undef X
     return nullptr;

@@ -114,7 +114,7 @@ struct Leg {
         if constexpr (is_path_dep<PK>()) run_update<PK>(run, S);
     }
     __device__ __forceinline__ void stepH(const LegParams& P, float dw1, float dw2, float h, float em) {
-        step_heston(P, S, V, dw1, dw2, h, em);
+        step_heston<MK == MK_HESTON_MIL>(P, S, V, dw1, dw2, h, em);
         if constexpr (is_path_dep<PK>()) run_update<PK>(run, S);
     }
     __device__ __forceinline__ float pay(const LegParams& P, int n, float bgk) const {
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             // Philox normals, 4 per call; normal n = k * nf + c.
             const int nn = a.n_steps * a.nf;
             float z[4];
-            if constexpr (MK == MK_HESTON) {
+            if constexpr (is_heston(MK)) {
                 for (int b = 0; 4 * b < nn; b++) {
                     prng_normals4(a.seed + (uint32_t)r, a.tag, (uint64_t)idx, (uint32_t)b, z);
 #pragma unroll
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
             }
         } else if constexpr (NZ == NZ_SOBOL_RAW) {
-            if constexpr (MK == MK_HESTON) {
+            if constexpr (is_heston(MK)) {
                 for (int k = 0; k < a.n_steps; k++) {
                     const float d1 = icdf_u32_tab(sobol_coord(sl, 2 * k), itab) * a.sqrt_h;
                     const float d2 = icdf_u32_tab(sobol_coord(sl, 2 * k + 1), itab) * a.sqrt_h;

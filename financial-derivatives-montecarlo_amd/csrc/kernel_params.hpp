@@ -13,8 +13,11 @@
 
 namespace fdmc {
 
+// MK_HESTON_MIL is Heston with the Milstein variance term, a kernel variant of its own
+// so that the reference's scheme (MK_HESTON) carries no extra instructions.
 enum ModelKind : int { MK_GBM = 0, MK_HESTON = 1, MK_DUPIRE = 2, MK_MULTI_DUPIRE = 3, MK_LVQ = 4,
-                       MK_MULTI_GBM = 5 };
+                       MK_MULTI_GBM = 5, MK_HESTON_MIL = 6 };
+constexpr bool is_heston(int mk) { return mk == MK_HESTON || mk == MK_HESTON_MIL; }
 enum PayoffKind : int { PK_EUROPEAN = 0, PK_ASIAN = 1, PK_GEOM_ASIAN = 2, PK_LOOKBACK = 3,
                         PK_BARRIER = 4, PK_BASKET = 5 };
 // Noise sources of the fused path kernels.
@@ -28,6 +31,7 @@ struct LegParams {
     float theta, xi, L10, L11, v0, kappa;
     float sigma0, alpha, beta_m1, inv_S0;
     float lv_a, lv_b, lv_c, inv_sref;
+    float mil_q;   // Heston Milstein coefficient xi^2/4, 0 for the reference's scheme
 };
 
 // Scrambled Sobol direction record of one (replica, dimension), 64 words:

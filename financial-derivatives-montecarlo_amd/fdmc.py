@@ -22,6 +22,7 @@ MODEL_GBM, MODEL_HESTON, MODEL_DUPIRE, MODEL_MULTI_DUPIRE, MODEL_LOCALVOL_QUAD, 
 PAYOFF_EUROPEAN, PAYOFF_ASIAN, PAYOFF_GEOM_ASIAN, PAYOFF_LOOKBACK, PAYOFF_BARRIER, PAYOFF_BASKET = range(6)
 NOISE_RAW, NOISE_BB, NOISE_PCA, NOISE_PRNG = 0, 1, 2, 3
 FLAG_PRNG_FALLBACK, FLAG_NOT_CONVERGED = 1, 2
+HESTON_EXP_EULER, HESTON_MILSTEIN = 0, 1  # Heston variance schemes
 PCA_BF16 = 0x100  # OR into the mode of qmc_accumulate
 BGK_BETA = 0.5826  # payoffs.hpp:7
 D_MAX_SOBOL = 20000  # models.hpp:9
@@ -39,7 +40,7 @@ class CModel(C.Structure):
                 ("v0", C.c_double), ("L", C.c_double * 4),
                 ("sigma0", C.c_double), ("alpha", C.c_double), ("beta_d", C.c_double),
                 ("lv_a", C.c_double), ("lv_b", C.c_double), ("lv_c", C.c_double), ("lv_sref", C.c_double),
-                ("uncorrelated", C.c_int32), ("reserved0", C.c_int32),
+                ("uncorrelated", C.c_int32), ("heston_scheme", C.c_int32),
                 ("S0_assets", C.POINTER(C.c_double)), ("sigma_assets", C.POINTER(C.c_double)),
                 ("L_assets", C.POINTER(C.c_double))]
 
@@ -149,9 +150,13 @@ class GBMParams(_Model):
 class HestonParams(_Model):
     kind = MODEL_HESTON
 
-    def __init__(self, S0=100.0, mu=0.05, kappa=2.0, theta=0.04, xi=0.5, rho=-0.9, v0=0.04, T=1.0):
+    def __init__(self, S0=100.0, mu=0.05, kappa=2.0, theta=0.04, xi=0.5, rho=-0.9, v0=0.04, T=1.0,
+                 scheme=HESTON_EXP_EULER):
+        # scheme: HESTON_EXP_EULER is the reference's full-truncation step
+        # (methods_cuda.cu:276-284); HESTON_MILSTEIN adds xi^2/4 (dW2^2 - h)
         self.S0, self.mu, self.kappa, self.theta, self.xi = S0, mu, kappa, theta, xi
         self.rho, self.v0, self.T = rho, v0, T
+        self.scheme = scheme
         self.L = [0.0, 0.0, 0.0, 0.0]
 
     def compute_cholesky(self):  # models.hpp:39-42
@@ -160,6 +165,7 @@ class HestonParams(_Model):
 
     def _fill(self, m):
         m.kappa, m.theta, m.xi, m.rho, m.v0 = self.kappa, self.theta, self.xi, self.rho, self.v0
+        m.heston_scheme = self.scheme
         for i in range(4):
             m.L[i] = self.L[i]
 

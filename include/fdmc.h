@@ -59,11 +59,17 @@ typedef struct fdmc_model {
     double sigma0, alpha, beta_d;/* DUPIRE, MULTI_DUPIRE */
     double lv_a, lv_b, lv_c, lv_sref; /* LOCALVOL_QUAD */
     int32_t uncorrelated;        /* MULTI_*: nonzero -> identity correlation */
-    int32_t reserved0;
+    int32_t heston_scheme;       /* HESTON: FDMC_HESTON_EXP_EULER (default) or FDMC_HESTON_MILSTEIN */
     const double* S0_assets;     /* MULTI_*: n_assets values */
     const double* sigma_assets;  /* MULTI_GBM: n_assets values */
     const double* L_assets;      /* MULTI_*: n x n lower Cholesky, row-major; NULL -> identity */
 } fdmc_model;
+/* Heston variance schemes.  EXP_EULER is the reference's full-truncation step
+ * V' = theta + e^{-kappa h}(V - theta) + xi sqrt(V+) dW2 (methods_cuda.cu:276-284);
+ * MILSTEIN adds the Milstein term xi^2/4 (dW2^2 - h) to it (no counterpart in
+ * the reference, whose comment names the scheme Milstein but omits the term). */
+#define FDMC_HESTON_EXP_EULER 0
+#define FDMC_HESTON_MILSTEIN 1
 
 /* ---- payoffs (reference payoffs.hpp:15-22) ---- */
 typedef enum {

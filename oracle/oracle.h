@@ -100,6 +100,7 @@ typedef struct orc_params {
     int model_kind, payoff_kind, n_assets;
     float S0, mu, T, h, discount, sigma;
     float kappa, theta, xi, rho, v0, L10, L11;
+    float mil_q;          /* Heston Milstein coefficient xi^2/4 (0: the reference's scheme) */
     float sigma0, alpha, beta_d;
     float lv_a, lv_b, lv_c, lv_sref;
     float K, B;
@@ -112,6 +113,9 @@ typedef struct orc_params {
 int orc_make_params(const fdmc_model* m, const fdmc_payoff* p, int n_steps, orc_params* out);
 /* kernel_mc per path: dW holds n_steps*nf scaled increments, index k*nf + c. */
 float orc_path_single(const orc_params* P, const float* dW, int n_steps);
+/* Terminal S and V of Heston paths from explicit (dW1, dW2) pairs (test hook). */
+void orc_heston_terminal(const orc_params* P, const float* dW, int n_steps, int n_paths,
+                         float* S_T, float* V_T);
 /* kernel_mlmc per path (with running updates for every model, fixing the Heston
  * omission at methods_cuda.cu:464-470).  in: scaled dW (1 factor) or unscaled Z
  * (Heston), n_fine*nf values. */

@@ -47,7 +47,8 @@ int orc_make_params(const fdmc_model* m, const fdmc_payoff* p, int n_steps, orc_
     case FDMC_MODEL_HESTON:
         P->S0 = (float)m->S0; P->kappa = (float)m->kappa; P->theta = (float)m->theta;
         P->xi = (float)m->xi; P->rho = (float)m->rho; P->v0 = (float)m->v0;
-        P->L10 = (float)m->L[2]; P->L11 = (float)m->L[3]; break;
+        P->L10 = (float)m->L[2]; P->L11 = (float)m->L[3];
+        P->mil_q = m->heston_scheme == FDMC_HESTON_MILSTEIN ? 0.25f * P->xi * P->xi : 0.0f; break;
     case FDMC_MODEL_DUPIRE:
         P->S0 = (float)m->S0; P->sigma0 = (float)m->sigma0; P->alpha = (float)m->alpha;
         P->beta_d = (float)m->beta_d; P->sigma = (float)m->sigma0; break;
@@ -87,7 +88,8 @@ static inline void euler_heston(const orc_params* P, float* S, float* V, float d
     float sq = sqrtf(Vp);
     *S = *S + P->mu * *S * h + sq * *S * dw1;
     float em = expf(-P->kappa * h);
-    *V = P->theta + em * (*V - P->theta) + P->xi * sq * dw2;
+    float v = P->theta + em * (*V - P->theta) + P->xi * sq * dw2;
+    *V = fmaf(P->mil_q, dw2 * dw2 - h, v);  /* optional Milstein term; exact no-op at mil_q = 0 */
 }
 
 /* ---- payoff machinery (methods_cuda.cu:298-352) ---- */
@@ -151,6 +153,21 @@ float orc_path_single(const orc_params* P, const float* dW, int n_steps) {
     return payoff_eval(P, S, run, n_steps);
 }
 
+/* Terminal (S, V) of n_paths Heston paths through the same step as
+ * orc_path_single; dW holds each path's n_steps (dW1, dW2) pairs of scaled,
+ * already correlated increments, path-major.  Test hook for the variance
+ * scheme's strong order. */
+void orc_heston_terminal(const orc_params* P, const float* dW, int n_steps, int n_paths,
+                         float* S_T, float* V_T) {
+    for (int p = 0; p < n_paths; p++) {
+        const float* d = dW + (size_t)p * n_steps * 2;
+        float S = P->S0, V = P->v0;
+        for (int k = 0; k < n_steps; k++) euler_heston(P, &S, &V, d[2 * k], d[2 * k + 1], P->h);
+        S_T[p] = S;
+        V_T[p] = V;
+    }
+}
+
 /* kernel_mlmc (methods_cuda.cu:445-518), with the running update applied for
  * Heston too (the reference skips it there, :464-470 and :484-490). */
 void orc_path_coupled(const orc_params* P, const float* in, int n_fine, int M, float* pf,
@@ -180,7 +197,8 @@ void orc_path_coupled(const orc_params* P, const float* in, int n_fine, int M, f
                 float Vcp = fmaxf(Vc, 0.0f), sq = sqrtf(Vcp);
                 Sc = Sc + P->mu * Sc * h_c + sq * Sc * acc1;
                 float emc = expf(-P->kappa * h_c);
-                Vc = P->theta + emc * (Vc - P->theta) + P->xi * sq * acc2;
+                float vc = P->theta + emc * (Vc - P->theta) + P->xi * sq * acc2;
+                Vc = fmaf(P->mil_q, acc2 * acc2 - h_c, vc);
                 acc1 = acc2 = 0.0f;
             } else {
                 one_factor_step(P, &Sc, acc1, ck, h_c);

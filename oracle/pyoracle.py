@@ -31,7 +31,7 @@ class COrcParams(C.Structure):
                 ("S0", C.c_float), ("mu", C.c_float), ("T", C.c_float), ("h", C.c_float),
                 ("discount", C.c_float), ("sigma", C.c_float), ("kappa", C.c_float), ("theta", C.c_float),
                 ("xi", C.c_float), ("rho", C.c_float), ("v0", C.c_float), ("L10", C.c_float), ("L11", C.c_float),
-                ("sigma0", C.c_float), ("alpha", C.c_float), ("beta_d", C.c_float),
+                ("mil_q", C.c_float), ("sigma0", C.c_float), ("alpha", C.c_float), ("beta_d", C.c_float),
                 ("lv_a", C.c_float), ("lv_b", C.c_float), ("lv_c", C.c_float), ("lv_sref", C.c_float),
                 ("K", C.c_float), ("B", C.c_float), ("bgk_on", C.c_int), ("sigma_bgk", C.c_float),
                 ("S0_assets", C.POINTER(C.c_double)), ("sigma_assets", C.POINTER(C.c_double)),
@@ -68,6 +68,7 @@ _SIGS = {
     "orc_make_params": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff), C.c_int,
                                   C.POINTER(COrcParams)]),
     "orc_path_single": (C.c_float, [C.POINTER(COrcParams), _f32p, C.c_int]),
+    "orc_heston_terminal": (None, [C.POINTER(COrcParams), _f32p, C.c_int, C.c_int, _f32p, _f32p]),
     "orc_path_coupled": (None, [C.POINTER(COrcParams), _f32p, C.c_int, C.c_int, _f32p, _f32p]),
     "orc_path_basket": (C.c_float, [C.POINTER(COrcParams), _f32p, C.c_int]),
     "orc_qmc_accumulate": (C.c_int, [C.POINTER(fdmc.CModel), C.POINTER(fdmc.CPayoff), C.POINTER(fdmc.CModel),
@@ -263,6 +264,15 @@ def make_params(model, payoff, n_steps):
 def path_single(P, dW, n_steps):
     dW = np.ascontiguousarray(dW, np.float32)
     return lib().orc_path_single(C.byref(P), _ptr(dW, C.c_float), n_steps)
+
+
+def heston_terminal(P, dW, n_steps):
+    """dW: [n_paths, n_steps, 2] scaled (dW1, dW2) increments -> (S_T, V_T) arrays."""
+    dW = np.ascontiguousarray(dW, np.float32)
+    n = dW.shape[0]
+    S, V = np.empty(n, np.float32), np.empty(n, np.float32)
+    lib().orc_heston_terminal(C.byref(P), _ptr(dW, C.c_float), n_steps, n, _ptr(S, C.c_float), _ptr(V, C.c_float))
+    return S, V
 
 
 def path_coupled(P, inp, n_fine, M):

@@ -35,6 +35,8 @@ def test_estimator_arguments(engine):
     _raises(INVALID, engine.run_mlmc_cuda, GBM, EURO, 1e-2, fdmc.MLMCConfig(M=1))
     _raises(INVALID, engine.run_mlmc_cuda, GBM, EURO, 1e-2, fdmc.MLMCConfig(max_L=40))
     _raises(UNSUPPORTED, engine.run_is_cuda, HES, EURO, 1.0, 1e-2)
+    bad = fdmc.HestonParams(scheme=2).compute_cholesky()
+    assert "variance scheme" in _raises(INVALID, engine.run_mlmc_cuda, bad, EURO, 1e-2, fdmc.MLMCConfig(M=4))
     _raises(UNSUPPORTED, engine.run_is_cuda, GBM, ASIAN, 1.0, 1e-2)
 
 

@@ -19,6 +19,8 @@ pytestmark = pytest.mark.gpu
 
 GBM = fdmc.GBMParams(100.0, 0.05, 0.2, 1.0)
 HES = fdmc.HestonParams(100.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04, 1.0).compute_cholesky()
+HES_MIL = fdmc.HestonParams(100.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04, 1.0,
+                            scheme=fdmc.HESTON_MILSTEIN).compute_cholesky()
 DUP = fdmc.DupireLocalParams(100.0, 0.05, 0.2, 0.5, 0.7, 1.0)
 LVQ = fdmc.LocalVolQuadParams(100.0, 0.05, 0.2, 0.1, 1.0, 100.0, 1.0)
 EURO = fdmc.European(100.0, 0.05, 1.0)
@@ -38,10 +40,12 @@ def _check_sums(got, ref, n, barrier):
 @pytest.mark.parametrize("model,payoff,M,levels", [
     (HES, EURO, 4, (0, 1, 2, 3, 4)),
     (HES, BAR, 4, (0, 1, 3)),
+    (HES_MIL, EURO, 4, (0, 1, 2, 4)),
     (GBM, ASIAN, 2, (0, 1, 2, 5, 7)),
     (DUP, LOOK, 2, (1, 4)),
     (LVQ, EURO, 4, (2, 3)),
-], ids=["heston-euro-M4", "heston-barrier-M4", "gbm-asian-M2", "dupire-look-M2", "lvq-euro-M4"])
+], ids=["heston-euro-M4", "heston-barrier-M4", "heston-milstein-euro-M4", "gbm-asian-M2", "dupire-look-M2",
+        "lvq-euro-M4"])
 def test_mlmc_level_sums_parity(engine, model, payoff, M, levels):
     reqs = [(l, 0, 1000 * l, 700 + 37 * l) for l in levels]   # unaligned ranges
     got = engine.levels_accumulate(model, payoff, M, reqs, qmc=False, salt_or_seed=42)
@@ -54,8 +58,9 @@ def test_mlmc_level_sums_parity(engine, model, payoff, M, levels):
     (GBM, ASIAN, 2, fdmc.NOISE_BB),
     (DUP, EURO, 2, fdmc.NOISE_RAW),
     (HES, EURO, 4, fdmc.NOISE_RAW),
+    (HES_MIL, ASIAN, 4, fdmc.NOISE_RAW),
     (LVQ, LOOK, 2, fdmc.NOISE_BB),
-], ids=["gbm-asian-bb", "dupire-euro-raw", "heston-euro-raw", "lvq-look-bb"])
+], ids=["gbm-asian-bb", "dupire-euro-raw", "heston-euro-raw", "heston-milstein-asian-raw", "lvq-look-bb"])
 def test_mlqmc_level_sums_parity(engine, model, payoff, M, mode):
     R, L = 3, 5
     bbl = [engine.bb_upload(fdmc.bb_precompute(M ** l, 1.0)) for l in range(L + 1)] if mode == fdmc.NOISE_BB else []
@@ -170,6 +175,24 @@ def test_c4_heston_mlmc_european_vs_closed_form(engine):
     ref = heston_call(100, 100, 1, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04)
     assert r.std_error < 5e-4 / math.sqrt(2) * 1.01
     assert abs(r.price - ref) < 3 * r.std_error + 5e-4 / math.sqrt(2), (r.price, ref, r.std_error, r.levels)
+
+
+def test_heston_milstein_mlmc_vs_closed_form(engine):
+    """The optional Milstein variance step: run_mlmc reaches the semi-closed-form
+    Heston price (Milstein keeps Euler's weak order 1 in S) at eps = 1e-3, and
+    its level variances of the coupled V-dependent payoff are no larger than the
+    reference scheme's."""
+    cfg = fdmc.MLMCConfig(M=4, max_L=8, pilot_n=10000, seed=42)
+    r = engine.run_mlmc_cuda(HES_MIL, EURO, 1e-3, cfg)
+    ref = heston_call(100, 100, 1, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04)
+    assert r.flags == 0 and r.std_error < 1e-3 / math.sqrt(2) * 1.01
+    assert abs(r.price - ref) < 3 * r.std_error + 1e-3 / math.sqrt(2), (r.price, ref, r.std_error, r.levels)
+    n = 200_000
+    for l in (3, 4):
+        vm = engine.levels_accumulate(HES_MIL, EURO, 4, [(l, 0, 0, n)], qmc=False, salt_or_seed=7)[0]
+        ve = engine.levels_accumulate(HES, EURO, 4, [(l, 0, 0, n)], qmc=False, salt_or_seed=7)[0]
+        var = lambda s: s[1] / n - (s[0] / n) ** 2
+        assert var(vm) <= 1.1 * var(ve), (l, vm, ve)
 
 
 def test_c4_heston_mlmc_barrier_vs_single_level(engine):

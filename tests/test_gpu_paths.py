@@ -22,6 +22,8 @@ pytestmark = pytest.mark.gpu
 GBM = fdmc.GBMParams(100.0, 0.05, 0.2, 1.0)
 GBM3 = fdmc.GBMParams(100.0, 0.05, 0.3, 1.0)
 HES = fdmc.HestonParams(100.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04, 1.0).compute_cholesky()
+HES_MIL = fdmc.HestonParams(100.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04, 1.0,
+                            scheme=fdmc.HESTON_MILSTEIN).compute_cholesky()
 DUP = fdmc.DupireLocalParams(100.0, 0.05, 0.2, 0.5, 0.7, 1.0)
 LVQ = fdmc.LocalVolQuadParams(100.0, 0.05, 0.2, 0.1, 1.0, 100.0, 1.0)
 
@@ -49,6 +51,7 @@ QMC_CASES = [
     ("gbm-bar-raw", GBM, BAR, 64, fdmc.NOISE_RAW),
     ("heston-euro-raw", HES, EURO, 32, fdmc.NOISE_RAW),
     ("heston-bar-raw", HES, BAR, 32, fdmc.NOISE_RAW),
+    ("heston-milstein-asian-raw", HES_MIL, ASIAN, 64, fdmc.NOISE_RAW),
     ("dupire-euro-bb", DUP, EURO, 64, fdmc.NOISE_BB),
     ("dupire-look-raw", DUP, LOOK, 16, fdmc.NOISE_RAW),
     ("lvq-asian-bb", LVQ, ASIAN, 128, fdmc.NOISE_BB),
@@ -106,8 +109,8 @@ def test_prng_fallback_above_20000_dims(engine):
     _close(got[:, 0] / n, ref[:, 0] / n, n, EURO, "fallback")
 
 
-@pytest.mark.parametrize("model,payoff,n_steps", [(GBM, EURO, 64), (HES, BAR, 32), (DUP, ASIAN, 16),
-                                                  (LVQ, LOOK, 24)])
+@pytest.mark.parametrize("model,payoff,n_steps", [(GBM, EURO, 64), (HES, BAR, 32), (HES_MIL, EURO, 32),
+                                                  (DUP, ASIAN, 16), (LVQ, LOOK, 24)])
 def test_mc_fixed_parity(engine, model, payoff, n_steps):
     n = 5000
     gm, gv = engine.run_mc_fixed(model, payoff, n_steps, n, 99)
