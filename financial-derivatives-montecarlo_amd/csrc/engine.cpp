@@ -290,15 +290,16 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
         if (nf != 1) fail(FDMC_E_UNSUPPORTED, "Brownian bridge needs a one-factor model");
         if (!s.bb) fail(FDMC_E_INVALID, "BB mode needs a DeviceBBData");
         if (s.bb->N != s.n_steps) fail(FDMC_E_INVALID, "BB data has N != n_steps");
-        if (s.n_steps > SOBOL_LDS_MAX_D)
-            fail(FDMC_E_UNSUPPORTED, "Brownian bridge supports at most 1024 steps");
-        nz = NZ_SOBOL_BB;
+        if (s.greeks && s.n_steps > SOBOL_LDS_MAX_D)
+            fail(FDMC_E_UNSUPPORTED, "adjoint Greeks with a Brownian bridge support at most 1024 steps");
+        nz = s.n_steps > SOBOL_LDS_MAX_D ? NZ_SOBOL_BB_GLOBAL : NZ_SOBOL_BB;
         lb = s.n_steps >= 16 ? 4 : 0;
         a.G = s.n_steps >> lb;
         a.bbw = s.bb->d_w;
         a.bb_uniform = s.bb->uniform;
         for (int k = 0; k < BB_SD_DEPTHS; k++) a.bb_sd[k] = s.bb->sd_depth[k];
-        fn = s.greeks ? greeks_kernel_bb(mk, pk, cv, cpk, lb) : path_kernel_bb(mk, pk, cv, cpk, lb);
+        fn = s.greeks ? greeks_kernel_bb(mk, pk, cv, cpk, lb)
+                      : path_kernel_bb(mk, pk, cv, cpk, lb, nz == NZ_SOBOL_BB_GLOBAL);
     } else if (s.mode == FDMC_NOISE_PCA) {
         if (nf != 1) fail(FDMC_E_UNSUPPORTED, "PCA noise needs a one-factor model (methods_cuda.cu:1702-1704)");
         if (s.greeks) fail(FDMC_E_UNSUPPORTED, "adjoint Greeks are available with Raw and BB noise");

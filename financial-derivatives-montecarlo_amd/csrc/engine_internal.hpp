@@ -63,7 +63,7 @@ hipError_t launch_book_final(const BookLaunch& a, hipStream_t s);
 
 // k_path_{raw,bb,prng}.hip: kernel for a (model, payoff, control) combination, or null.
 const void* path_kernel_raw(int mk, int pk, int cv, int cpk, int lb);
-const void* path_kernel_bb(int mk, int pk, int cv, int cpk, int lb);
+const void* path_kernel_bb(int mk, int pk, int cv, int cpk, int lb, bool global_records);
 const void* path_kernel_prng(int mk, int pk, int cv, int cpk, int lb);
 const void* path_kernel_rows(int mk, int pk, int cv, int cpk, int lb);
 

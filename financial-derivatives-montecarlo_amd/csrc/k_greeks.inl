@@ -125,13 +125,13 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
             constexpr bool UNI = decltype(uniform_tag)::value;
             if constexpr (NZ == NZ_SOBOL_BB) {
                 sk[0] = 0.0f;
-                sk[G * 256] = bb_terminal<UNI>(a, sl, itab);
+                sk[G * 256] = bb_terminal<UNI, true>(a, sl, itab);
                 for (int j = 1; j < G; j++) {
                     const int lev = 31 - __builtin_clz(j);
                     const int q = j - (1 << lev);
                     const int st = G >> lev;
                     const int L = q * st, R = L + st, M = L + (st >> 1);
-                    sk[M * 256] = bb_mid<UNI>(a, sl, itab, sk[L * 256], sk[R * 256], j, lev + 1);
+                    sk[M * 256] = bb_mid<UNI, true>(a, sl, itab, sk[L * 256], sk[R * 256], j, lev + 1);
                 }
             }
             // increments of block i (B steps; steps beyond n are zero)
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                             for (int q = 0; q < (1 << l); q++) {
                                 const int s = B >> l;
                                 const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                                W[Mp] = bb_mid<UNI>(a, sl, itab, W[Lp], W[Rp], j0 + q, lgG + l + 1);
+                                W[Mp] = bb_mid<UNI, true>(a, sl, itab, W[Lp], W[Rp], j0 + q, lgG + l + 1);
                             }
                         }
     #pragma unroll

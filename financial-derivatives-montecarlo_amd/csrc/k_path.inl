@@ -40,15 +40,21 @@ __device__ __forceinline__ uint32_t sobol_coord_lds(const SobolLane& s, int d) {
     return r[s.ga] ^ r[8 + s.gb] ^ r[16 + s.w];
 }
 
+// Global mode: the lane's Gray bits from the record, the higher bits of the
+// tile's point index (hi_gray) one direction number each.
+__device__ __forceinline__ uint32_t sobol_coord_glb(const SobolLane& s, int d) {
+    const uint32_t* r = s.grec + (int64_t)d * SDIR_WORDS;
+    uint32_t x = r[s.ga] ^ r[8 + s.gb];
+    for (uint32_t g = s.hi_gray; g; g &= g - 1u) x ^= r[32 + __builtin_ctz(g)];
+    return x;
+}
+
 __device__ __forceinline__ uint32_t sobol_coord(const SobolLane& s, int d) {
     if (s.lds_mode) {
         const uint32_t* r = s.rec + d * 20;
         return r[s.ga] ^ r[8 + s.gb] ^ r[16 + s.w];
     }
-    const uint32_t* r = s.grec + (int64_t)d * SDIR_WORDS;
-    uint32_t x = r[s.ga] ^ r[8 + s.gb];
-    for (uint32_t g = s.hi_gray; g; g &= g - 1u) x ^= r[32 + __builtin_ctz(g)];
-    return x;
+    return sobol_coord_glb(s, d);
 }
 
 // Words 16..19 of the LDS direction records for the 256-point tile n (points
@@ -77,28 +83,35 @@ __device__ __forceinline__ void sobol_tile_words(uint32_t* rec, const uint32_t* 
 }
 
 // A Brownian-bridge midpoint W_M from the ends of its interval and the normal of
-// coordinate j (LDS direction records: a bridge has at most SOBOL_LDS_MAX_D
-// dates): with bb_precompute's FP32 weights (wl, wr, sd), or on a uniform date
-// grid (UNI) with weights 1/2 and the sd of the depth, read from no table per
-// coordinate.  The path and Greeks kernels both build their bridges from these,
-// so the same points give the same paths.
-template <bool UNI>
+// coordinate j (direction records in LDS for bridges of at most SOBOL_LDS_MAX_D
+// dates, LDS = true, else read from global memory): with bb_precompute's FP32
+// weights (wl, wr, sd), or on a uniform date grid (UNI) with weights 1/2 and the
+// sd of the depth, read from no table per coordinate.  The path and Greeks
+// kernels both build their bridges from these, so the same points give the
+// same paths.
+template <bool LDS>
+__device__ __forceinline__ uint32_t bb_coord(const SobolLane& sl, int j) {
+    if constexpr (LDS) return sobol_coord_lds(sl, j);
+    else return sobol_coord_glb(sl, j);
+}
+
+template <bool UNI, bool LDS>
 __device__ __forceinline__ float bb_mid(const PathLaunch& a, const SobolLane& sl, const float4* itab, float WL,
                                         float WR, int j, int depth) {
     if constexpr (UNI) {
         const float sd = a.bb_sd[depth];
-        return 0.5f * (WL + WR) - icdf_u32_tab_scaled_neg(sobol_coord_lds(sl, j), itab, sd, -2.0f * sd);
+        return 0.5f * (WL + WR) - icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j), itab, sd, -2.0f * sd);
     } else {
         const float4 c = ((const float4*)a.bbw)[j];
-        return c.x * WL + c.y * WR + c.z * icdf_u32_tab(sobol_coord_lds(sl, j), itab);
+        return c.x * WL + c.y * WR + c.z * icdf_u32_tab(bb_coord<LDS>(sl, j), itab);
     }
 }
 
 // W at the last date: sd sqrt(T) times coordinate 0's normal.
-template <bool UNI>
+template <bool UNI, bool LDS>
 __device__ __forceinline__ float bb_terminal(const PathLaunch& a, const SobolLane& sl, const float4* itab) {
-    if constexpr (UNI) return -icdf_u32_tab_scaled_neg(sobol_coord_lds(sl, 0), itab, a.bb_sd[0], -2.0f * a.bb_sd[0]);
-    else return ((const float4*)a.bbw)[0].z * icdf_u32_tab(sobol_coord_lds(sl, 0), itab);
+    if constexpr (UNI) return -icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, 0), itab, a.bb_sd[0], -2.0f * a.bb_sd[0]);
+    else return ((const float4*)a.bbw)[0].z * icdf_u32_tab(bb_coord<LDS>(sl, 0), itab);
 }
 
 template <int MK, int PK>
@@ -161,7 +174,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = blockIdx.y;
     const int D = a.D;
-    constexpr bool SOBOL = (NZ == NZ_SOBOL_RAW || NZ == NZ_SOBOL_BB);
+    constexpr bool BRIDGE = (NZ == NZ_SOBOL_BB || NZ == NZ_SOBOL_BB_GLOBAL);
+    constexpr bool SOBOL = (NZ == NZ_SOBOL_RAW || BRIDGE);
     // inverse-CDF table (icdf_u32_tab), filled before the first barrier-separated use
     constexpr bool ITAB = SOBOL;
     __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
@@ -169,12 +183,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         icdf_tab_load(itab, threadIdx.x, 256);
         __syncthreads();
     }
-    const bool lds_mode = SOBOL && D <= SOBOL_LDS_MAX_D;
+    const bool lds_mode = SOBOL && NZ != NZ_SOBOL_BB_GLOBAL && D <= SOBOL_LDS_MAX_D;
     uint32_t* rec = (uint32_t*)smem;
     const int rec_words = lds_mode ? D * 20 : 0;
     float* skel = (float*)(smem + (size_t)rec_words * 4);
     // the bridge skeleton (dates < 16) or the walk stack of the block boundaries
-    const int skel_words = (NZ == NZ_SOBOL_BB) ? (LB == 0 ? a.G + 1 : BB_WALK_DEPTH) * 256 : 0;
+    const int skel_words = BRIDGE ? (LB == 0 ? a.G + 1 : 32 - __builtin_clz(a.G)) * 256 : 0;
     double* red = (double*)(smem + (((size_t)(rec_words + skel_words) * 4 + 15) & ~(size_t)15));
 
     const uint32_t* grec = SOBOL ? a.sdir + (int64_t)r * D * SDIR_WORDS : nullptr;
@@ -260,22 +274,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 for (int k = 0; k < a.n_steps; k++)
                     ps.step(a, icdf_u32_tab(sobol_coord(sl, k), itab) * a.sqrt_h, k);
             }
-        } else {  // NZ_SOBOL_BB (one-factor models)
+        } else {  // NZ_SOBOL_BB, NZ_SOBOL_BB_GLOBAL (one-factor models)
             // W_M from the ends of its interval and coordinate j's normal: with
             // bb_precompute's weights, or on a uniform date grid (weights 1/2, the
-            // sd of the depth; no weight loads per coordinate).  A bridge has at
-            // most SOBOL_LDS_MAX_D dates, so its direction records are in LDS.
+            // sd of the depth; no weight loads per coordinate).  Direction
+            // records in LDS up to SOBOL_LDS_MAX_D dates, global beyond.
             const int G = a.G, lgG = 31 - __builtin_clz(G);
-            auto bridge = [&](auto uniform_tag) {
+            auto bridge = [&](auto uniform_tag, auto lds_tag) {
                 constexpr bool UNI = decltype(uniform_tag)::value;
+                constexpr bool LDS = decltype(lds_tag)::value;
                 auto mid = [&](float WL, float WR, int j, int depth) {
-                    return bb_mid<UNI>(a, sl, itab, WL, WR, j, depth);
+                    return bb_mid<UNI, LDS>(a, sl, itab, WL, WR, j, depth);
                 };
                 if constexpr (LB == 0) {
                     // fewer than 16 dates: the whole skeleton in LDS, one step per entry
                     float* sk = skel + tid;
                     sk[0] = 0.0f;
-                    sk[G * 256] = bb_terminal<UNI>(a, sl, itab);
+                    sk[G * 256] = bb_terminal<UNI, LDS>(a, sl, itab);
                     for (int j = 1; j < G; j++) {
                         const int lev = 31 - __builtin_clz(j);
                         const int q = j - (1 << lev);
@@ -307,7 +322,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         return mid(WL, WR, (1 << lev) + (L >> (k + 1)), lev + 1);
                     };
                     float Wp = 0.0f;
-                    rt[m * 256] = bb_terminal<UNI>(a, sl, itab);
+                    rt[m * 256] = bb_terminal<UNI, LDS>(a, sl, itab);
                     for (int k = m - 1; k >= 0; k--) rt[k * 256] = node(k, 0, Wp, rt[(k + 1) * 256]);
                     for (int i = 0; i < G; i++) {
                         if (i > 0) {
@@ -334,8 +349,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     }
                 }
             };
-            if (a.bb_uniform) bridge(std::true_type{});
-            else bridge(std::false_type{});
+            using LdsRec = std::integral_constant<bool, NZ == NZ_SOBOL_BB>;
+            if (a.bb_uniform) bridge(std::true_type{}, LdsRec{});
+            else bridge(std::false_type{}, LdsRec{});
         }
 
         float ym, yc;

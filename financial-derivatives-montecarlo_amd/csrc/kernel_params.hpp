@@ -21,7 +21,10 @@ constexpr bool is_heston(int mk) { return mk == MK_HESTON || mk == MK_HESTON_MIL
 enum PayoffKind : int { PK_EUROPEAN = 0, PK_ASIAN = 1, PK_GEOM_ASIAN = 2, PK_LOOKBACK = 3,
                         PK_BARRIER = 4, PK_BASKET = 5 };
 // Noise sources of the fused path kernels.
-enum NoiseKind : int { NZ_SOBOL_RAW = 0, NZ_SOBOL_BB = 1, NZ_PRNG = 2, NZ_ROWS = 3 };
+// NZ_SOBOL_BB_GLOBAL: a bridge of more than SOBOL_LDS_MAX_D dates, whose
+// direction records are read from global memory (its own kernel variant, so
+// the LDS-record bridge keeps its register allocation).
+enum NoiseKind : int { NZ_SOBOL_RAW = 0, NZ_SOBOL_BB = 1, NZ_PRNG = 2, NZ_ROWS = 3, NZ_SOBOL_BB_GLOBAL = 4 };
 // Control-variate leg: none, a second payoff on the same path, or a GBM leg.
 enum CtrlKind : int { CV_NONE = 0, CV_SHARED = 1, CV_GBM = 2 };
 
@@ -44,8 +47,14 @@ struct LegParams {
 //  g = gray(lane).
 constexpr int SDIR_WORDS = 64;
 
-constexpr int BB_SD_DEPTHS = 12;   // a bridge of at most 1024 steps has 11 depths
-constexpr int BB_WALK_DEPTH = 7;   // skeleton walk of at most 64 16-date blocks: W at 2^0..2^6 blocks
+constexpr int BB_SD_DEPTHS = 16;   // a uniform bridge of at most 2^15 steps has at most 16 depths
+// Entries of the skeleton walk's per-lane stack over G = 2^m blocks: W at the
+// right ends of aligned 2^0 .. 2^m-block intervals.
+inline int bb_walk_levels(int G) {
+    int m = 0;
+    while ((1 << m) < G) m++;
+    return m + 1;
+}
 
 struct PathLaunch {
     LegParams leg[2];       // main, control
@@ -170,8 +179,9 @@ constexpr int SOBOL_LDS_MAX_D = 1024;
 // Dynamic LDS bytes of one k_path launch (layout documented in k_path.inl).
 inline size_t path_lds_bytes(int nz, int D, int G) {
     size_t b = 0;
-    if (nz != NZ_PRNG && nz != NZ_ROWS && D <= SOBOL_LDS_MAX_D) b += (size_t)D * 20 * 4;
-    if (nz == NZ_SOBOL_BB) b += (size_t)(D < 16 ? G + 1 : BB_WALK_DEPTH) * 256 * 4;   // skeleton or walk stack
+    if (nz != NZ_PRNG && nz != NZ_ROWS && nz != NZ_SOBOL_BB_GLOBAL && D <= SOBOL_LDS_MAX_D) b += (size_t)D * 20 * 4;
+    if (nz == NZ_SOBOL_BB || nz == NZ_SOBOL_BB_GLOBAL)   // skeleton or walk stack
+        b += (size_t)(D < 16 ? G + 1 : bb_walk_levels(G)) * 256 * 4;
     b = (b + 15) & ~(size_t)15;
     return b + 4 * 7 * sizeof(double);
 }

@@ -57,6 +57,10 @@ def test_pass_arguments(engine):
     _raises(UNSUPPORTED, engine.qmc_accumulate, HES, EURO, 32, 1, 42, 0, 256, fdmc.NOISE_BB, dev_bb=bb)
     _raises(INVALID, engine.qmc_accumulate, GBM, EURO, 64, 1, 42, 0, 256, fdmc.NOISE_BB, dev_bb=bb)
     bb.free()
+    bb = engine.bb_upload(fdmc.bb_precompute(2048, 1.0))
+    msg = _raises(UNSUPPORTED, engine.greeks_accumulate, GBM, EURO, 2048, 1, 42, 0, 256, fdmc.NOISE_BB, dev_bb=bb)
+    assert "1024" in msg
+    bb.free()
     pca = engine.pca_upload(16, O.pca_compute(16, 1.0).T.ravel())
     _raises(INVALID, engine.qmc_accumulate, GBM, EURO, 32, 1, 42, 0, 256, fdmc.NOISE_PCA, dev_pca=pca)
     pca.free()

@@ -73,6 +73,33 @@ def test_qmc_pass_parity(engine, name, model, payoff, n_steps, mode):
     assert np.allclose(got[:, 1] / n, ref[:, 1] / n, rtol=1e-4, atol=1e-4 + (1e4 * 4 / n if isinstance(payoff, fdmc.Barrier) else 0))
 
 
+@pytest.mark.parametrize("model,payoff,n_steps", [(GBM3, ASIAN, 2048), (LVQ, LOOK, 2048), (GBM, BAR, 4096)],
+                         ids=["gbm-asian-2048", "lvq-look-2048", "gbm-bar-4096"])
+def test_qmc_bridge_above_1024_dates(engine, model, payoff, n_steps):
+    """Bridges of more than 1024 dates (the reference's examples ask for up to
+    2^11 with BB) read their direction records from global memory instead of
+    LDS: the same points and paths as the oracle."""
+    R, off, n = 2, 300, 700
+    bb = engine.bb_upload(fdmc.bb_precompute(n_steps, model.T))
+    got = np.array(engine.qmc_accumulate(model, payoff, n_steps, R, 42, off, n, fdmc.NOISE_BB, dev_bb=bb))
+    bb.free()
+    ref = O.qmc_accumulate(model, payoff, n_steps, R, 42, off, n, fdmc.NOISE_BB, bb=O.OracleBB(n_steps, model.T))
+    _close(got[:, 0] / n, ref[:, 0] / n, n, payoff, "bridge %d mean" % n_steps)
+    assert np.allclose(got[:, 1] / n, ref[:, 1] / n, rtol=1e-4,
+                       atol=1e-4 + (1e4 * 4 / n if isinstance(payoff, fdmc.Barrier) else 0))
+
+
+def test_run_qmc_bridge_2048_matches_oracle_loop(engine):
+    cfg = fdmc.QMCConfig(R=8, N0=1 << 10, max_doublings=2)
+    bb = engine.bb_upload(fdmc.bb_precompute(2048, 1.0))
+    g = engine.run_qmc_cuda(GBM, ASIAN, 1e-5, 2048, cfg, fdmc.NOISE_BB, dev_bb=bb)
+    bb.free()
+    r = O.run_qmc(GBM, ASIAN, 1e-5, 2048, cfg, fdmc.NOISE_BB, bb=O.OracleBB(2048, 1.0))
+    assert g.n_samples == r.n_samples == 8 * (1 << 11)
+    assert abs(g.price - r.price) < 1e-4
+    assert abs(g.std_error - r.std_error) < 0.05 * r.std_error
+
+
 def test_qmc_cv_pass_parity(engine):
     """C2 kernel: arithmetic Asian + geometric Asian control on the same bridge path."""
     n_steps, R, n = 256, 4, 2048
