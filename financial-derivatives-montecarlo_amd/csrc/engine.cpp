@@ -12,6 +12,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -665,14 +666,25 @@ void qmc_loop(fdmc_context* c, PassSpec s, double eps, const fdmc_qmc_config& cf
     bool conv = false;
     *out = fdmc_result{};
     const int D = s.n_steps * noise_dim(*s.model);
+    // A PCA pass holds its normals and GEMM rows (6 bytes per coordinate and
+    // replica) in HBM: the new points of a doubling go through in chunks of at
+    // most 8 GiB of those buffers, whatever N grows to.
+    int64_t chunk = INT64_MAX;
+    if (s.mode == FDMC_NOISE_PCA) {
+        const int64_t per_point = (int64_t)cfg.R * ((D + 127) / 128 * 128) * 6;
+        chunk = std::max<int64_t>(256, (((int64_t)8 << 30) / per_point) & ~(int64_t)255);
+    }
     for (int d = 0; d < cfg.max_doublings; d++) {
         if (N > (int64_t)4294967296LL && D <= D_MAX_SOBOL)
             fail(FDMC_E_RANGE, "N_per_replica exceeds the 32-bit Sobol offset");
-        s.first = done;
-        s.count = N - done;
-        const int nacc = run_pass(c, s, part);
-        if (tot.empty()) tot.assign(part.size(), 0.0);
-        for (size_t i = 0; i < part.size(); i++) tot[i] += part[i];
+        int nacc = 0;
+        for (int64_t f = done; f < N; f += s.count) {
+            s.first = f;
+            s.count = std::min(chunk, N - f);
+            nacc = run_pass(c, s, part);
+            if (tot.empty()) tot.assign(part.size(), 0.0);
+            for (size_t i = 0; i < part.size(); i++) tot[i] += part[i];
+        }
         out->path_steps += (int64_t)cfg.R * (N - done) * s.n_steps;
         done = N;
         doublings++;

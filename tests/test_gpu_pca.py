@@ -172,3 +172,33 @@ def test_c3_price_and_per_asset_deltas(engine, c3):
         fd = (pu - pd) / (2 * h) / (1 << 16)
         adj = g[2 + 2 * a] / (1 << 16)
         assert abs(adj - fd) < 0.01 * abs(fd) + 1e-4, (a, adj, fd)
+
+
+def test_run_qmc_pca_chunks_large_doublings(engine):
+    """run_qmc_cuda with PCA noise over 2048 dates (the largest step count the
+    reference's examples use) and 32 replicas: the doubling from 2^16 to 2^17
+    points per replica needs 26 GB of normals and GEMM rows, so it runs in
+    chunks of at most 8 GiB.  Its replica sums equal those of single passes over
+    the same point ranges (FP64 summation order apart), so chunks neither skip
+    nor repeat points."""
+    n, R = 2048, 32
+    t = np.arange(1, n + 1) / n
+    w, V = np.linalg.eigh(np.minimum.outer(t, t))
+    W = V[:, ::-1] * np.sqrt(np.maximum(w[::-1], 0.0))     # Brownian path W = W z, leading components first
+    M = np.diff(W, axis=0, prepend=0.0)                    # increments dW = M z, as pca_compute returns
+    pca = engine.pca_upload(n, M.T.ravel())
+    gbm, asian = fdmc.GBMParams(100.0, 0.05, 0.3, 1.0), fdmc.Asian(100.0)
+    try:
+        r = engine.run_qmc_cuda(gbm, asian, 1e-9, n, fdmc.QMCConfig(R=R, N0=1 << 14, max_doublings=4),
+                                fdmc.NOISE_PCA, dev_pca=pca)
+        assert r.n_samples == R * (1 << 17) and r.flags & fdmc.FLAG_NOT_CONVERGED
+        tot = np.zeros(R)
+        for first, count in [(0, 1 << 14), (1 << 14, 1 << 14), (1 << 15, 1 << 15), (1 << 16, 1 << 15),
+                             (3 << 15, 1 << 15)]:
+            tot += np.array(engine.qmc_accumulate(gbm, asian, n, R, 42, first, count, fdmc.NOISE_PCA,
+                                                  dev_pca=pca))[:, 0]
+    finally:
+        pca.free()
+    means = tot / (1 << 17)
+    assert abs(r.price - means.mean()) <= 1e-12 * abs(r.price)
+    assert abs(r.std_error - means.std(ddof=1) / np.sqrt(R)) <= 1e-9 * r.std_error
