@@ -23,7 +23,7 @@ SOURCES = ["engine.cpp", "k_sobol.hip", "k_path_raw.hip", "k_path_bb.hip", "k_pa
            "k_greeks_raw.hip", "k_greeks_bb.hip", "k_greeks_prng.hip",
            "k_pca.hip", "k_basket.hip", "k_is.hip", "k_path_rows.hip", "k_book.hip", "k_book_o3.hip", "book.cpp",
            "../cpp/utils.cpp", "../cpp/methods_hip.cpp"]
-EXAMPLES = ["asian_table", "book_table"]
+EXAMPLES = ["asian_table", "book_table", "heston_table"]
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -51,3 +51,16 @@ def test_facade_book_example_batched_equals_plain():
     print(p.stdout)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "batched == plain: yes" in p.stdout
+
+
+@pytest.mark.gpu
+def test_facade_heston_example_and_long_bridges():
+    """The reference's Heston example call sequence (Richardson c1, MC, MLMC, QMC
+    Raw, MLQMC Raw) plus MLMC with the Milstein variance term, and a 2048-date
+    GBM Asian with bridge and PCA noise, all within tolerance of fixed-step runs."""
+    exe = os.path.join(PKG, "lib", "heston_table")
+    p = subprocess.run([exe, "0.05"], capture_output=True, text=True, timeout=300)
+    print(p.stdout)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "MLMC Milstein" in p.stdout and "QMC PCA 2048" in p.stdout
+    assert "facade rows outside tolerance: 0" in p.stdout
