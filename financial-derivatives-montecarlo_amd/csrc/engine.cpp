@@ -785,7 +785,7 @@ void launch_levels(fdmc_context* c, const MultiSpec& ms, int nz, const std::vect
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ent[x].D > ent[y].D; });
     std::vector<LevelEntry> sorted;
     std::vector<int> sw;
-    int total = 0, max_D = 0, max_G = 0;
+    int total = 0, max_D = 0, skel_n = 0;
     for (int o : order) {
         LevelEntry e = ent[o];
         const int64_t t0 = e.first >> 8, t1 = (e.first + e.count - 1) >> 8;
@@ -800,7 +800,8 @@ void launch_levels(fdmc_context* c, const MultiSpec& ms, int nz, const std::vect
         e.ntiles = (int)items;
         total += e.ntiles;
         max_D = std::max(max_D, e.D);
-        max_G = std::max(max_G, e.G);
+        if (nz == NZ_SOBOL_BB)   // skeleton (under 16 dates) or walk-stack entries per lane
+            skel_n = std::max(skel_n, e.n_fine < 16 ? e.G + 1 : bb_walk_levels(e.G));
         sorted.push_back(e);
         sw.push_back(which[o]);
     }
@@ -830,11 +831,11 @@ void launch_levels(fdmc_context* c, const MultiSpec& ms, int nz, const std::vect
     a.entries = d_ent;
     a.queue = d_queue;
     a.partials = d_part;
-    const size_t lds = levels_lds_bytes(nz, max_D, max_G);
+    const size_t lds = levels_lds_bytes(nz, max_D, skel_n);
     if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "multilevel LDS footprint too large");
     if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int grid = (int)std::min<int64_t>(total, (int64_t)c->n_cu * 8);
-    void* args[] = {(void*)&a, (void*)&max_D, (void*)&max_G};
+    void* args[] = {(void*)&a, (void*)&max_D, (void*)&skel_n};
     if (c->timing) HIPC(hipEventRecord(c->ev0, c->stream));
     HIPC(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, c->stream));
     if (c->timing) HIPC(hipEventRecord(c->ev1, c->stream));
@@ -947,7 +948,6 @@ void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cac
                     if (b->N != nfine)
                         fail(FDMC_E_INVALID, "bb_list[" + std::to_string(q.level) + "] has N=" +
                                                  std::to_string(b->N) + ", expected " + std::to_string(nfine));
-                    if (nfine > SOBOL_LDS_MAX_D) fail(FDMC_E_UNSUPPORTED, "BB supports at most 1024 steps");
                     e.bbw = b->d_w;
                     e.G = nfine >= 16 ? nfine / 16 : nfine;
                 } else if (ms.mode == FDMC_NOISE_PCA) {  // methods_cuda.cu:1781-1789

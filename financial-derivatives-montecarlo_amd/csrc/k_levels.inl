@@ -66,7 +66,7 @@ template <int MK, int PK, int NZ, int LB>
 // Compiled for three waves per SIMD (168 VGPRs with the bridge, some spills):
 // 15 % faster than two waves at 202 VGPRs on MI355X; the PRNG variants use
 // far fewer registers and are unaffected.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_levels(LevelsLaunch a, int max_D, int max_G) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_levels(LevelsLaunch a, int max_D, int skel_n) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63;
     constexpr bool SOBOL = NZ == NZ_SOBOL_RAW || NZ == NZ_SOBOL_BB;
@@ -81,7 +81,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     uint32_t* rec = (uint32_t*)smem;
     const int rec_words = lds_ok ? max_D * 20 : 0;
     float* skel = (float*)(smem + (size_t)rec_words * 4);
-    const int skel_words = (NZ == NZ_SOBOL_BB) ? (max_G + 1) * 256 : 0;
+    // bridge: the skeleton of levels under 16 dates or the walk stack, per lane
+    const int skel_words = (NZ == NZ_SOBOL_BB) ? skel_n * 256 : 0;
     double* red = (double*)(smem + (((size_t)(rec_words + skel_words) * 4 + 15) & ~(size_t)15));
     int* s_item = (int*)(red + 4 * 4);
 
@@ -177,18 +178,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         } else {  // Brownian bridge over n_fine steps (one-factor models)
             const float4* bw = (const float4*)E.bbw;
             const int G = E.G;
-            float* sk = skel + tid;
-            sk[0] = 0.0f;
-            sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
-            for (int j = 1; j < G; j++) {
-                const int lev = 31 - __builtin_clz(j);
-                const int q = j - (1 << lev);
-                const int st = G >> lev;
-                const int L = q * st, R = L + st, Mi = L + (st >> 1);
+            // W_M of dyadic node j from the ends of its interval, bb_precompute's weights
+            auto mid = [&](float WL, float WR, int j) {
                 const float4 c = bw[j];
-                sk[Mi * 256] = c.x * sk[L * 256] + c.y * sk[R * 256] + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
-            }
+                return c.x * WL + c.y * WR + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
+            };
             if (LB == 0 || nfine < 16) {
+                // fewer than 16 dates: the whole skeleton in LDS, one step per entry
+                float* sk = skel + tid;
+                sk[0] = 0.0f;
+                sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
+                for (int j = 1; j < G; j++) {
+                    const int lev = 31 - __builtin_clz(j);
+                    const int q = j - (1 << lev);
+                    const int st = G >> lev;
+                    const int L = q * st, R = L + st, Mi = L + (st >> 1);
+                    sk[Mi * 256] = mid(sk[L * 256], sk[R * 256], j);
+                }
                 float left = 0.0f;
                 for (int i = 0; i < G; i++) {
                     const float right = sk[(i + 1) * 256];
@@ -196,12 +202,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     left = right;
                 }
             } else {
+                // G = 2^m blocks of B dates; the skeleton at the block boundaries is
+                // walked left to right on a per-lane LDS stack rt[0..m] (rt[k]: W at
+                // the right end of the aligned 2^k-block interval holding the current
+                // boundary), each node generated from the same operands as the
+                // top-down skeleton, so the LDS does not grow with the date count.
                 constexpr int B = 1 << (LB > 0 ? LB : 1);
-                float left = 0.0f;
+                const int m = 31 - __builtin_clz(G);
+                float* rt = skel + tid;
+                auto node = [&](int k, int L, float WL, float WR) {
+                    return mid(WL, WR, (1 << (m - 1 - k)) + (L >> (k + 1)));
+                };
+                float Wp = 0.0f;
+                rt[m * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
+                for (int k = m - 1; k >= 0; k--) rt[k * 256] = node(k, 0, Wp, rt[(k + 1) * 256]);
                 for (int i = 0; i < G; i++) {
+                    if (i > 0) {
+                        Wp = rt[0];
+                        const int c = __builtin_ctz(i);
+                        rt[c * 256] = rt[(c + 1) * 256];
+                        for (int k = c - 1; k >= 0; k--) rt[k * 256] = node(k, i, Wp, rt[(k + 1) * 256]);
+                    }
                     float W[B + 1];
-                    W[0] = left;
-                    W[B] = sk[(i + 1) * 256];
+                    W[0] = Wp;
+                    W[B] = rt[0];
 #pragma unroll
                     for (int l = 0; l < LB; l++) {
                         const int j0 = (G << l) + (i << l);
@@ -209,13 +233,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         for (int q = 0; q < (1 << l); q++) {
                             const int s = B >> l;
                             const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                            const float4 c = bw[j0 + q];
-                            W[Mp] = c.x * W[Lp] + c.y * W[Rp] + c.z * icdf_u32_tab(sobol_coord(sl, j0 + q), itab);
+                            W[Mp] = mid(W[Lp], W[Rp], j0 + q);
                         }
                     }
 #pragma unroll
                     for (int k = 0; k < B; k++) cp.fine1(P, E, W[k + 1] - W[k], i * B + k, M);
-                    left = W[B];
                 }
             }
         }

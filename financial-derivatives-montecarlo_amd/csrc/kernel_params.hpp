@@ -199,10 +199,10 @@ inline size_t greeks_lds_bytes(int nz, int D, int G, int n_steps) {
 }
 
 // Dynamic LDS bytes of one k_levels launch (max over its entries).
-inline size_t levels_lds_bytes(int nz, int max_D, int max_G) {
+inline size_t levels_lds_bytes(int nz, int max_D, int skel_n) {
     size_t b = 0;
     if (nz != NZ_PRNG && nz != NZ_ROWS && max_D <= SOBOL_LDS_MAX_D) b += (size_t)max_D * 20 * 4;
-    if (nz == NZ_SOBOL_BB) b += (size_t)(max_G + 1) * 256 * 4;
+    if (nz == NZ_SOBOL_BB) b += (size_t)skel_n * 256 * 4;
     b = (b + 15) & ~(size_t)15;
     return b + 4 * 4 * sizeof(double) + 16;
 }

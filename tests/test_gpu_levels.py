@@ -54,6 +54,29 @@ def test_mlmc_level_sums_parity(engine, model, payoff, M, levels):
         _check_sums(g, ref, n, isinstance(payoff, fdmc.Barrier))
 
 
+@pytest.mark.parametrize("model,payoff,M,levels", [
+    (GBM, ASIAN, 2, (3, 11)),
+    (LVQ, LOOK, 4, (6,)),
+], ids=["gbm-asian-M2-8+2048", "lvq-look-M4-4096"])
+def test_mlqmc_bridge_levels_above_1024_dates(engine, model, payoff, M, levels):
+    """Bridge levels of 2048 and 4096 fine dates (the skeleton is walked on a
+    per-lane stack, direction records come from global memory above 1024
+    dimensions), alone and in one launch with a short level: level sums equal
+    the oracle's at the same points."""
+    R, n = 2, 513
+    bbl = {l: engine.bb_upload(fdmc.bb_precompute(M ** l, 1.0)) for l in levels}
+    lst = [bbl.get(l) for l in range(max(levels) + 1)]
+    reqs = [(l, r, 300 * r, n) for l in levels for r in range(R)]
+    got = engine.levels_accumulate(model, payoff, M, reqs, qmc=True, mode=fdmc.NOISE_BB, R=R, salt_or_seed=42,
+                                   bb_list=lst)
+    for b in bbl.values():
+        b.free()
+    for (l, r, first, cnt), g in zip(reqs, got):
+        ref = O.mlqmc_level_sums(model, payoff, M, l, R, 42, first, cnt, fdmc.NOISE_BB, bb=O.OracleBB(M ** l, 1.0))
+        print("level %d replica %d" % (l, r), flush=True)
+        _check_sums(g, ref[r], cnt, False)
+
+
 @pytest.mark.parametrize("model,payoff,M,mode", [
     (GBM, ASIAN, 2, fdmc.NOISE_BB),
     (DUP, EURO, 2, fdmc.NOISE_RAW),
