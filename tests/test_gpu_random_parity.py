@@ -56,6 +56,43 @@ def _cases(n=24, seed=20261204):
 CASES = _cases()
 
 
+def _long_cases(n=6, seed=20261205):
+    """Long paths: bridges of 2048-8192 dates (direction records from global
+    memory) on one-factor models, raw Heston up to 12000 dimensions."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        T = float(rng.choice([0.5, 1.0, 2.0]))
+        mu = float(rng.uniform(0.0, 0.08))
+        kind = i % 4
+        if kind == 0:
+            model = fdmc.GBMParams(100.0, mu, float(rng.uniform(0.1, 0.5)), T)
+        elif kind == 1:
+            model = fdmc.DupireLocalParams(100.0, mu, float(rng.uniform(0.1, 0.3)), float(rng.uniform(0.0, 1.0)),
+                                           float(rng.uniform(0.5, 1.0)), T)
+        elif kind == 2:
+            model = fdmc.LocalVolQuadParams(100.0, mu, float(rng.uniform(0.1, 0.3)), float(rng.uniform(0.0, 0.2)),
+                                            float(rng.uniform(0.0, 2.0)), 100.0, T)
+        else:
+            model = fdmc.HestonParams(100.0, mu, float(rng.uniform(0.5, 3.0)), float(rng.uniform(0.02, 0.09)),
+                                      float(rng.uniform(0.1, 0.7)), float(rng.uniform(-0.9, 0.2)),
+                                      float(rng.uniform(0.02, 0.09)), T, scheme=int(rng.integers(2))).compute_cholesky()
+        K = float(rng.uniform(80.0, 120.0))
+        pk = int(rng.integers(5))
+        payoff = [fdmc.European(K, mu, T), fdmc.Asian(K), fdmc.GeomAsian(K), fdmc.Lookback(0.2),
+                  fdmc.Barrier(K, K * float(rng.uniform(1.2, 1.5)), 0.2, mu, T)][pk]
+        bb = kind != 3
+        n_steps = int(rng.choice([2048, 4096, 8192])) if bb else int(rng.choice([1500, 3000, 6000]))
+        out.append(("l%02d-%s-%s-%s-%d" % (i, ["gbm", "dup", "lvq", "hes"][kind],
+                                           ["eur", "asi", "geo", "lkb", "bar"][pk], "bb" if bb else "raw", n_steps),
+                    model, payoff, n_steps, fdmc.NOISE_BB if bb else fdmc.NOISE_RAW, 2,
+                    int(rng.integers(0, 5000)), int(rng.integers(200, 600))))
+    return out
+
+
+LONG_CASES = _long_cases()
+
+
 @pytest.mark.parametrize("name,model,payoff,n_steps,mode,R,first,count", CASES, ids=[c[0] for c in CASES])
 def test_random_pass_parity(engine, name, model, payoff, n_steps, mode, R, first, count):
     bb = engine.bb_upload(fdmc.bb_precompute(n_steps, model.T)) if mode == fdmc.NOISE_BB else None
@@ -72,6 +109,11 @@ def test_random_pass_parity(engine, name, model, payoff, n_steps, mode, R, first
     g2, r2 = got[:, 1] / count, ref[:, 1] / count
     tol2 = 1e-4 * np.maximum(1.0, np.abs(r2)) + (4.0 * 1600.0 / count if isinstance(payoff, fdmc.Barrier) else 1e-6)
     assert (np.abs(g2 - r2) <= tol2).all(), (name, g2, r2)
+
+
+@pytest.mark.parametrize("name,model,payoff,n_steps,mode,R,first,count", LONG_CASES, ids=[c[0] for c in LONG_CASES])
+def test_random_long_path_parity(engine, name, model, payoff, n_steps, mode, R, first, count):
+    test_random_pass_parity(engine, name, model, payoff, n_steps, mode, R, first, count)
 
 
 def _level_cases(n=16, seed=1204):
