@@ -974,7 +974,22 @@ void run_levels(fdmc_context* c, const MultiSpec& ms, std::map<int, DevBuf>& cac
             continue;
         }
         const int nz = pass == 1 ? NZ_PRNG : (ms.mode == FDMC_NOISE_BB ? NZ_SOBOL_BB : NZ_SOBOL_RAW);
-        launch_levels(c, ms, nz, ent, which, out);
+        if (pass == 1) {
+            launch_levels(c, ms, nz, ent, which, out);
+            continue;
+        }
+        // levels whose direction records fit in LDS run in one launch, longer
+        // ones (records read from global memory) in another, so a fine level
+        // does not move the short levels' records out of LDS
+        std::vector<LevelEntry> part_ent[2];
+        std::vector<int> part_which[2];
+        for (size_t k = 0; k < ent.size(); k++) {
+            const int g = ent[k].D > SOBOL_LDS_MAX_D ? 1 : 0;
+            part_ent[g].push_back(ent[k]);
+            part_which[g].push_back(which[k]);
+        }
+        for (int g = 0; g < 2; g++)
+            if (!part_ent[g].empty()) launch_levels(c, ms, nz, part_ent[g], part_which[g], out);
     }
 }
 
