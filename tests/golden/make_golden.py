@@ -3,7 +3,8 @@
 sobol_scrambled.json  -- (salt, dim, index) -> scrambled uint32, from the
     oracle's row-by-row restatement of hh_scramble over the Gray-code Sobol
     point (methods_cuda.cu:781-825).  Regression vectors for the oracle and the
-    GPU generator.
+    GPU generator; tests/test_hh_seeding.py checks them against a numpy
+    restatement of the seeding that shares no code with the oracle.
 report_prices.json -- the reference prices printed in the reference's own
     reports (informes_finales/informe_completo.txt, lines cited per entry):
     fixed-seed 500k-path MC runs of the examples' parameters, statistical

@@ -85,3 +85,20 @@ def test_icdf_extremes_and_sweep(engine, torch_cuda):
     for i, j in ((0, 11), (1, 10), (4, 9)):
         assert got[i] == -got[j], (i, j)
     assert abs(got[6]) < 1e-9 and abs(got[7]) < 1e-9
+
+
+def test_scrambled_u32_against_numpy_and_scipy(engine, torch_cuda):
+    """The device points equal the numpy restatement of the seeding and scramble
+    (methods_cuda.cu:753-795, 819) applied to scipy's independent unscrambled
+    Sobol points: no oracle code on this path."""
+    from scipy.stats import qmc
+    from test_hh_seeding import SHIFT_KEY, _np_scramble, sm
+
+    D, m = 24, 12
+    X = (qmc.Sobol(D, scramble=False, bits=32).random_base2(m) * 2.0 ** 32).astype(np.uint64).astype(np.uint32)
+    for salt in (42, 42 + 7 * 16 + 5):
+        got = _gen_u32(engine, torch_cuda, salt, D, 1 << m, 0)
+        for d in range(D):
+            base = sm(salt) ^ d
+            want = _np_scramble(X[:, d], [sm(base ^ k) for k in range(32)], sm(base ^ SHIFT_KEY))
+            assert np.array_equal(got[d], want), (salt, d)
