@@ -24,6 +24,12 @@ SOURCES = ["engine.cpp", "k_sobol.hip", "k_path_raw.hip", "k_path_bb.hip", "k_pa
            "k_pca.hip", "k_basket.hip", "k_is.hip", "k_path_rows.hip", "k_book.hip", "k_book_o3.hip", "book.cpp",
            "../cpp/utils.cpp", "../cpp/methods_hip.cpp"]
 EXAMPLES = ["asian_table", "book_table", "heston_table"]
+# The reference's own example programs (examples/ejemplo01..11*.cpp) include
+# "../methods_cuda.cuh"; compiled unchanged from their place in the reference
+# checkout, fed on stdin from a folder whose parent holds a one-line
+# methods_cuda.cuh that includes the facade, so the include resolves there.
+REF_EXAMPLES = "/root/reference/examples"
+REF_EXAMPLES_OUT = os.path.join(HERE, "lib", "reference_examples")
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
@@ -70,7 +76,38 @@ def build(jobs=None, verbose=False):
             sys.stderr.write(p.stdout + p.stderr)
             raise RuntimeError("example build failed: " + ex)
         print("  built", exe, flush=True)
+    ref = build_reference_examples()
+    if ref:
+        print("  built %d reference example programs in %s" % (len(ref), REF_EXAMPLES_OUT), flush=True)
     return LIB
+
+
+def build_reference_examples(out_dir=REF_EXAMPLES_OUT, src_dir=REF_EXAMPLES, cxx="g++"):
+    """Compile every example program of the reference, unchanged, against the
+    C++ facade and libfdmc.so.  Returns the built executables ([] when the
+    reference checkout is absent)."""
+    if not os.path.isdir(src_dir):
+        return []
+    cwd = os.path.join(out_dir, "src")
+    os.makedirs(cwd, exist_ok=True)
+    with open(os.path.join(out_dir, "methods_cuda.cuh"), "w") as f:
+        f.write('#include "methods_hip.hpp"\n')
+    built = []
+    libdir = os.path.dirname(LIB)
+    for name in sorted(os.listdir(src_dir)):
+        if not name.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, name[:-4])
+        rpath = os.path.relpath(libdir, out_dir)
+        cmd = [cxx, "-std=c++17", "-O2", "-I" + os.path.join(HERE, "cpp"), "-x", "c++", "-", "-o", exe,
+               "-L" + libdir, "-lfdmc", "-Wl,-rpath,$ORIGIN/" + rpath]
+        with open(os.path.join(src_dir, name)) as src:
+            p = subprocess.run(cmd, stdin=src, cwd=cwd, capture_output=True, text=True)
+        if p.returncode != 0:
+            sys.stderr.write(" ".join(cmd) + "\n" + p.stdout + p.stderr)
+            raise RuntimeError("reference example failed to compile against the facade: " + name)
+        built.append(exe)
+    return built
 
 
 if __name__ == "__main__":

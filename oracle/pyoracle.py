@@ -16,7 +16,8 @@ ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "financial-derivatives-montecarlo_amd"))
 import fdmc  # noqa: E402  (ctypes struct definitions only)
 
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# FDMC_ORACLE_LIB selects another build of the same sources (the sanitizer build)
+LIB_PATH = os.environ.get("FDMC_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 _P = C.c_void_p
 
 
