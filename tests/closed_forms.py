@@ -33,3 +33,22 @@ def heston_call(S0, K, T, r, kappa, theta, xi, rho, v0):
     P1 = 0.5 + integrate.quad(p1, 1e-10, 200, limit=400)[0] / math.pi
     P2 = 0.5 + integrate.quad(p2, 1e-10, 200, limit=400)[0] / math.pi
     return S0 * P1 - K * math.exp(-r * T) * P2
+
+
+def pca_uniform(m, T):
+    """Increment matrix M (dW = M z, row-major [date][component]) of the PCA of a
+    Brownian motion on m uniform dates over [0, T], from the closed-form
+    eigenpairs of the covariance h min(i, j): lambda_k = h / (4 sin^2((2k-1) pi /
+    (2 (2m+1)))), v_k(i) = 2 / sqrt(2m+1) sin((2k-1) i pi / (2m+1)), components in
+    descending eigenvalue order; rows of the cumulative matrix V sqrt(Lambda)
+    differenced into increments."""
+    import numpy as np
+
+    h = T / m
+    k = np.arange(1, m + 1)
+    lam = h / (4.0 * np.sin((2 * k - 1) * np.pi / (2 * (2 * m + 1))) ** 2)
+    V = 2.0 / np.sqrt(2 * m + 1) * np.sin(np.outer(k, 2 * k - 1) * np.pi / (2 * m + 1))
+    Mc = V * np.sqrt(lam)
+    M = Mc.copy()
+    M[1:] -= Mc[:-1]
+    return M

@@ -139,6 +139,42 @@ def test_c2_price_and_greeks_with_control_variate(engine):
     assert g.delta_se < g2.delta_se and g.vega_se < g2.vega_se      # the control reduces Greek variance too
 
 
+def test_c2_adjoint_with_control_matches_crn_differences(engine):
+    """Config 2's own setting: sigma = 0.3, D = 256 bridge, R = 32 replicas, the
+    geometric-Asian control with beta = 1.  The adjoint Greeks of the controlled
+    estimator Y_a - beta (Y_g - E(theta)) (kernel sums of d(Y_a - beta Y_g) plus
+    beta dE/dtheta of the closed form) against central differences of the same
+    controlled estimator repriced at theta +- h on the same points, with E(theta)
+    re-evaluated at each bump."""
+    n_steps, R, n, beta = 256, 32, 1 << 12, 1.0
+
+    def E_at(S0=100.0, mu=0.05, sig=0.3):
+        return fdmc.geom_asian_analytic(S0, 100.0, 1.0, mu, sig, n_steps)
+
+    bb = engine.bb_upload(fdmc.bb_precompute(n_steps, 1.0))
+    m = gbm(sig=0.3)
+    rows = np.array(engine.greeks_accumulate(m, ASIAN, n_steps, R, 42, 0, n, fdmc.NOISE_BB, dev_bb=bb,
+                                             ctrl_model=m, ctrl_payoff=GEO, E_ctrl=E_at(), beta=beta))
+    dE = fdmc.geom_asian_greeks(100.0, 100.0, 1.0, 0.05, 0.3, n_steps)
+    npr = rows.shape[1] - 6
+    bumps = {"delta": ("S0", 0.05), "vega": ("sig", 0.002), "rho": ("mu", 0.002)}
+    for k, which in enumerate(("delta", "vega", "rho")):
+        key, h = bumps[which]
+        cv = []
+        for sgn in (+1, -1):
+            kw = {"S0": 100.0, "mu": 0.05, "sig": 0.3}
+            kw[key] += sgn * h
+            mb = gbm(**kw)
+            got = np.array(engine.qmc_accumulate(mb, ASIAN, n_steps, R, 42, 0, n, fdmc.NOISE_BB, dev_bb=bb,
+                                                 ctrl_model=mb, ctrl_payoff=GEO, E_ctrl=E_at(**kw), beta=beta))
+            cv.append(got[:, 2] / n)
+        fd = (cv[0] - cv[1]) / (2 * h)
+        adj = rows[:, npr + 2 * k] / n + beta * dE[k]
+        print(which, "adjoint", adj[:3], "crn", fd[:3], flush=True)
+        assert np.all(np.abs(adj - fd) <= 0.01 * np.abs(fd) + 2e-3), (which, adj, fd)
+    bb.free()
+
+
 def test_autograd_binding_returns_adjoint_greeks(engine, torch_cuda):
     torch = torch_cuda
     import autograd as ag

@@ -14,6 +14,7 @@ import pytest
 
 import fdmc
 import pyoracle as O
+from book_parity import check_greek_sums
 
 pytestmark = pytest.mark.gpu
 
@@ -326,8 +327,8 @@ def test_random_book_level_sums_match_oracle(engine, gm):
                 np.testing.assert_allclose(got[:, 0], ref[:, 0], atol=tol, rtol=1e-6,
                                            err_msg="contract %d kind %d n %d level %d" % (c, k.kind, k.n_steps, l))
                 if k.kind != fdmc.PAYOFF_BARRIER:
-                    d = np.abs(got[:, 1:] - ref[:, 1:])
-                    bad = d > 2e-4 * n + 1e-4 * np.abs(ref[:, 1:])
-                    assert bad.sum(axis=0).max() <= 2 and d.max() <= 600.0, (c, l, got, ref)
+                    print("contract %d kind %d n %d level %d" % (c, k.kind, k.n_steps, l), flush=True)
+                    check_greek_sums(got[:, 1:], ref[:, 1:], n,
+                                     lambda i: O.book_level_sums(model, k, c, l, R, 42, off + i, 1))
     finally:
         b.free()

@@ -501,3 +501,15 @@ def test_heston_milstein_coupled_coarse_leg_uses_the_coarse_step():
             acc[0, j] = (a1, a2)
         s_c, _ = O.heston_terminal(Pc, acc, nf // M)
         assert abs(pc - math.exp(-0.05) * float(s_c[0])) <= 2e-6 * abs(pc), (pc, s_c)
+
+
+def test_closed_form_pca_matches_oracle_jacobi():
+    """The test helper's closed-form increment matrix equals the oracle's Jacobi
+    PCA up to eigenvector signs, and M M^T = h I."""
+    from closed_forms import pca_uniform
+
+    for m in (1, 2, 8, 64):
+        M = pca_uniform(m, 1.0)
+        Mo = O.pca_compute(m, 1.0)
+        assert np.allclose(M @ M.T, np.eye(m) / m, atol=1e-13)
+        assert np.allclose(np.abs(M), np.abs(Mo), atol=1e-10), m
