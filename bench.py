@@ -189,10 +189,18 @@ def c3_gemm(eng, reps=5):
         # of at most 8 GiB of dW): flops of all timed passes over all GEMM time
         flops = 2.0 * 16 * (1 << 18) * 1024 * 1024 * reps
         tf = flops / (ms * 1e-3) / 1e12
+        # the same pass with the per-asset pathwise deltas (replica means and their se)
+        t0 = time.perf_counter()
+        g = np.array(eng.greeks_accumulate_pca(model, pay, 64, 16, 42, 0, 1 << 18, pca, bf16))
+        greeks_ms = 1e3 * (time.perf_counter() - t0)
+        dl = g[:, 2::2] / (1 << 18)
         out["bf16" if bf16 else "fp16"] = {"gemm_ms": ms / n, "gemm_launches_per_pass": n // reps, "tflops": tf,
                                            "frac_mfma_peak": tf / MFMA_PEAK_TFLOPS, "pass_ms": 1e3 * wall,
                                            "price": sum(r[0] for r in sums) / (16 << 18),
-                                           "gemm_rel_err_vs_fp64": gemm_error(eng, pca, M, bf16)}
+                                           "gemm_rel_err_vs_fp64": gemm_error(eng, pca, M, bf16),
+                                           "pass_with_deltas_ms": greeks_ms,
+                                           "deltas": [float(x) for x in dl.mean(axis=0)],
+                                           "delta_se": [float(x) for x in dl.std(axis=0, ddof=1) / 4.0]}
     pca.free()
     return out
 

@@ -814,9 +814,9 @@ void launch_levels(fdmc_context* c, const MultiSpec& ms, int nz, const std::vect
     const size_t ent_bytes = sorted.size() * sizeof(LevelEntry);
     const size_t part_bytes = (size_t)total * 4 * sizeof(double);
     const size_t sums_bytes = sorted.size() * 4 * sizeof(double);
-    char* scratch = (char*)c->partials.ensure(part_bytes + ent_bytes + sums_bytes + 256);
-    double* d_part = (double*)scratch;
-    LevelEntry* d_ent = (LevelEntry*)(scratch + ((part_bytes + 63) & ~(size_t)63));
+    char* pbuf = (char*)c->partials.ensure(part_bytes + ent_bytes + sums_bytes + 256);
+    double* d_part = (double*)pbuf;
+    LevelEntry* d_ent = (LevelEntry*)(pbuf + ((part_bytes + 63) & ~(size_t)63));
     double* d_out = (double*)((char*)d_ent + ((ent_bytes + 63) & ~(size_t)63));
     unsigned int* d_queue = (unsigned int*)c->sums.ensure(64);
     HIPC(hipMemcpyAsync(d_ent, sorted.data(), ent_bytes, hipMemcpyHostToDevice, c->stream));
@@ -874,7 +874,30 @@ void run_levels_pca(fdmc_context* c, const MultiSpec& ms, const std::vector<Leve
         const fdmc_pca* pca = ms.pca_list[q0.level];
         // Z (2 B) and dW (4 B) per padded element: groups of at most 8 GiB
         const double per = (double)q0.count * pca->mp * 6.0;
-        if (per > 16.0 * (1 << 30)) fail(FDMC_E_RANGE, "PCA level pass above 16 GiB of buffers: lower the sample count");
+        if (per > 8.0 * (1 << 30)) {
+            // one request above 8 GiB (the reference batches such a level by free
+            // memory, methods_cuda.cu:1791-1803): its point range runs in chunks of
+            // a multiple of 256 points with at most 8 GiB of buffers each, whose
+            // sums add up to the request's
+            const int64_t chunk = std::max<int64_t>(
+                256, (int64_t)(8.0 * (1 << 30) / ((double)pca->mp * 6.0)) / 256 * 256);
+            const int w = which[idx[i]];
+            std::array<double, 4> acc{0.0, 0.0, 0.0, 0.0};
+            for (int64_t f = q0.first; f < q0.first + q0.count; f += chunk) {
+                LevelEntry e = e0;
+                e.first = f;
+                e.count = std::min(chunk, q0.first + q0.count - f);
+                e.ld = pca_rows(c, pca, e0.sdir, e0.D, 1, e.first, e.count, ms.pca_bf16);
+                e.rows = (const float*)c->pca_dw.p;
+                std::vector<LevelEntry> ge{e};
+                std::vector<int> gw{w};
+                launch_levels(c, ms, NZ_ROWS, ge, gw, out);
+                for (int k = 0; k < 4; k++) acc[k] += out[w][k];
+            }
+            out[w] = acc;
+            i++;
+            continue;
+        }
         const size_t gmax = (size_t)std::max(1.0, 8.0 * (1 << 30) / per);
         size_t j = i + 1;
         while (j < idx.size() && j - i < gmax) {
@@ -1405,6 +1428,31 @@ fdmc_status fdmc_greeks_accumulate(fdmc_context* c, const fdmc_model* model, con
         s.first = offset; s.count = n_points; s.bb = bb; s.greeks = true;
         if (mode == FDMC_NOISE_PRNG) { s.prng = true; s.tag = 0; s.mode = FDMC_NOISE_RAW; }
         else s.mode = mode;
+        std::vector<double> v;
+        const int nacc = run_pass(c, s, v);
+        std::memcpy(sums_out, v.data(), v.size() * sizeof(double));
+        if (n_acc_out) *n_acc_out = nacc;
+    });
+}
+
+fdmc_status fdmc_greeks_accumulate_pca(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff,
+                                       int32_t n_steps, int32_t R, uint32_t salt_base, int64_t offset,
+                                       int64_t n_points, int32_t mode, const fdmc_pca* pca, double* sums_out,
+                                       int32_t* n_acc_out) {
+    const bool bf16 = (mode & FDMC_PCA_BF16) != 0;
+    mode &= ~FDMC_PCA_BF16;
+    return guarded([&] {
+        if (!c || !sums_out || !pca) fail(FDMC_E_INVALID, "null argument");
+        validate_model(model);
+        validate_payoff(payoff);
+        if (mode != FDMC_NOISE_PCA) fail(FDMC_E_INVALID, "mode must be FDMC_NOISE_PCA (optionally | FDMC_PCA_BF16)");
+        if (model->kind != FDMC_MODEL_MULTI_GBM || model->n_assets > BASKET_MAX)
+            fail(FDMC_E_UNSUPPORTED, "PCA Greeks are available for baskets of at most 32 GBM assets");
+        if (offset < 0) fail(FDMC_E_INVALID, "offset must be non-negative");
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.n_steps = n_steps; s.R = R; s.salt_base = salt_base;
+        s.first = offset; s.count = n_points; s.mode = FDMC_NOISE_PCA; s.pca = pca; s.pca_bf16 = bf16 ? 1 : 0;
+        s.greeks = true;
         std::vector<double> v;
         const int nacc = run_pass(c, s, v);
         std::memcpy(sums_out, v.data(), v.size() * sizeof(double));

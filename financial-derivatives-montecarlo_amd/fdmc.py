@@ -423,6 +423,9 @@ _SIGS = {
                                          C.POINTER(CPayoff), C.c_double, C.c_double, C.c_int32, C.c_int32,
                                          C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P,
                                          C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "fdmc_greeks_accumulate_pca": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_int32, C.c_int32,
+                                             C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "fdmc_run_qmc_greeks": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.POINTER(CModel),
                                       C.POINTER(CPayoff), C.c_double, C.POINTER(C.c_double), C.c_double,
                                       C.c_double, C.c_int32, C.POINTER(CQMCConfig), C.c_int32, _P,
@@ -665,6 +668,19 @@ class Engine:
         _check(self.lib, self.lib.fdmc_greeks_accumulate(self.ctx, C.byref(m), C.byref(p), cm, cp, E_ctrl, beta,
                                                          n_steps, R, salt_or_seed, offset, n_points, mode,
                                                          dev_bb.handle if dev_bb else None, sums, C.byref(nacc)))
+        na = nacc.value
+        return [[sums[r * na + i] for i in range(na)] for r in range(R)]
+
+    def greeks_accumulate_pca(self, model, payoff, n_steps, R, salt_base, offset, n_points, dev_pca, bf16=False):
+        """One PCA pass of a GBM basket with per-asset pathwise deltas; R rows of
+        [sum Y, sum Y^2, then {sum, sum^2} of each asset's delta]."""
+        m, p = model.to_c(), payoff.to_c()
+        sums = (C.c_double * (R * 66))()
+        nacc = C.c_int32()
+        mode = NOISE_PCA | (PCA_BF16 if bf16 else 0)
+        _check(self.lib, self.lib.fdmc_greeks_accumulate_pca(self.ctx, C.byref(m), C.byref(p), n_steps, R,
+                                                             salt_base, offset, n_points, mode, dev_pca.handle,
+                                                             sums, C.byref(nacc)))
         na = nacc.value
         return [[sums[r * na + i] for i in range(na)] for r in range(R)]
 

@@ -316,7 +316,8 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* ctx, const fdmc_model* model,
  * 0x100 + level); qmc = 1: Sobol points of salt salt_or_seed + level*R + replica
  * (mode RAW, BB with bb_list[level], or PCA with pca_list[level], m = M^level,
  * optionally | FDMC_PCA_BF16; PCA runs one GEMM and one launch per level and
- * group of consecutive replicas with equal ranges).  sums_out: n_req x
+ * group of consecutive replicas with equal ranges, at most 8 GiB of normals and
+ * rows at a time: a larger request runs in chunks of its range).  sums_out: n_req x
  * {sum dY, sum dY^2, sum Yf, sum Yf^2} (methods_cuda.cu:514-517). */
 fdmc_status fdmc_levels_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                    const fdmc_payoff* payoff, int32_t M, int32_t qmc,
@@ -343,6 +344,19 @@ fdmc_status fdmc_greeks_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                    int32_t n_steps, int32_t R, uint32_t salt_or_seed,
                                    int64_t offset, int64_t n_points, int32_t mode,
                                    const fdmc_bb* bb, double* sums_out, int32_t* n_acc_out);
+/* greeks_accumulate for baskets of GBM assets (n <= 32) under PCA noise
+ * (replaces the basket branch of run_qmc_cuda's PCA pass, methods_cuda.cu:1151-1166
+ * and 1593, which prices only): per replica the price sums and {sum, sum^2} of each
+ * asset's pathwise delta disc 1{B > K} w_a S_T^a / S0^a, from the same MFMA GEMM
+ * (product epilogue when the asset count is a power of two and m a multiple of
+ * 256, else dW rows) as fdmc_qmc_accumulate.  mode: FDMC_NOISE_PCA, optionally
+ * | FDMC_PCA_BF16; pca: the uploaded m x m increment matrix (m = n_steps * n).
+ * sums_out: R x (2 + 2 n) doubles. */
+fdmc_status fdmc_greeks_accumulate_pca(fdmc_context* ctx, const fdmc_model* model,
+                                       const fdmc_payoff* payoff, int32_t n_steps, int32_t R,
+                                       uint32_t salt_base, int64_t offset, int64_t n_points,
+                                       int32_t mode, const fdmc_pca* pca, double* sums_out,
+                                       int32_t* n_acc_out);
 /* run_qmc_cuda / run_qmc_cv_cuda with Greeks: same doubling loop and stopping
  * rule on the price; the control's expectation E_ctrl enters the Greeks through
  * dE_ctrl[3] = its delta, vega, rho (NULL: no control). */

@@ -284,3 +284,26 @@ def test_c4_heston_mlmc_barrier_at_config(engine):
     print("256-step price %.5f + oracle E5..E8 %.5f +- %.5f" % (m256, corr, math.sqrt(cvar)), flush=True)
     assert 0.05 < corr < 0.3                     # the bias the loop could not remove below L = 8
     assert abs(r.price - (m256 + corr)) < 3 * math.sqrt(r.std_error ** 2 + v256 + cvar), (r.price, m256, corr)
+
+
+def test_pca_level_request_above_8_gib_runs_in_chunks(engine):
+    """One (level, replica) PCA request of 1.5M points on a 2048-date level needs
+    18 GB of normals and GEMM rows: it runs in chunks of a multiple of 256 points
+    with at most 8 GiB of buffers each (the reference batches such a level by free
+    memory, methods_cuda.cu:1791-1803), and its sums equal those of three requests
+    of 500,000 points over the same range (each below 8 GiB: one pass each) up to
+    FP64 summation order."""
+    pca = engine.pca_upload(2048, pca_uniform(2048, 1.0).T.ravel())
+    lst = [None] * 11 + [pca]
+    try:
+        print("one request of 1.5M points", flush=True)
+        big = engine.levels_accumulate(GBM, ASIAN, 2, [(11, 0, 1000, 1_500_000)], qmc=True, mode=fdmc.NOISE_PCA,
+                                       R=1, salt_or_seed=42, pca_list=lst)[0]
+        print("three requests of 500k points", flush=True)
+        parts = engine.levels_accumulate(GBM, ASIAN, 2, [(11, 0, 1000 + k * 500_000, 500_000) for k in range(3)],
+                                         qmc=True, mode=fdmc.NOISE_PCA, R=1, salt_or_seed=42, pca_list=lst)
+    finally:
+        pca.free()
+    tot = np.sum(np.array(parts), axis=0)
+    assert np.allclose(big, tot, rtol=1e-10, atol=1e-9), (big, tot)
+    assert abs(big[2] / 1.5e6 - 6.06) < 0.05          # the 2048-date Asian's mean payoff

@@ -174,6 +174,45 @@ def test_c3_price_and_per_asset_deltas(engine, c3):
         assert abs(adj - fd) < 0.01 * abs(fd) + 1e-4, (a, adj, fd)
 
 
+@pytest.mark.parametrize("bf16", [False, True], ids=["fp16", "bf16"])
+def test_c3_pca_per_asset_deltas_vs_crn(engine, c3, bf16):
+    """Config 3 through the PCA pipeline itself (2^18 points x R = 16, Kronecker
+    PCA of min(t_i, t_j) x Corr on matrix cores, the Euler factors multiplied in
+    the GEMM's epilogue): the pass with Greeks returns the same price sums as the
+    price-only pass, and every asset's pathwise delta disc 1{B > K} w_a S_T^a /
+    S0^a agrees per replica with central differences of the price over the same
+    points with S0^a bumped by +-0.05 (the price is linear in S0^a on each path,
+    so the difference moves only with paths crossing the strike)."""
+    sig, C, M, pca = c3
+    m = _basket(sig, C)
+    R, n, h = 16, 1 << 18, 0.05
+    mode = fdmc.NOISE_PCA | (fdmc.PCA_BF16 if bf16 else 0)
+    g = np.array(engine.greeks_accumulate_pca(m, PAY, 64, R, 42, 0, n, pca, bf16))
+    assert g.shape == (R, 2 + 2 * 16)
+    p = np.array(engine.qmc_accumulate(m, PAY, 64, R, 42, 0, n, mode, dev_pca=pca))
+    assert np.allclose(g[:, :2], p[:, :2], rtol=1e-12, atol=0.0)
+    deltas = g[:, 2::2] / n                                  # [R][asset]
+    for a in range(16):
+        px = []
+        for sgn in (+1, -1):
+            s0 = [100.0] * 16
+            s0[a] += sgn * h
+            mb = fdmc.MultiGBMParams(16, s0, list(sig), 0.05, 1.0, L=cholesky_lower(C).ravel())
+            px.append(np.array(engine.qmc_accumulate(mb, PAY, 64, R, 42, 0, n, mode, dev_pca=pca))[:, 0] / n)
+        fd = (px[0] - px[1]) / (2 * h)
+        assert np.all(np.abs(deltas[:, a] - fd) <= 2e-3 * np.abs(fd) + 2e-5), (a, deltas[:, a], fd)
+    # the replicas' mean deltas: positive, weights 1/16, summing to about the basket's delta
+    mean = deltas.mean(axis=0)
+    assert np.all(mean > 0.0) and 0.3 < mean.sum() < 1.0, mean
+
+
+def test_pca_greeks_entry_rejects_other_models(engine, c3):
+    sig, C, M, pca = c3
+    with pytest.raises(fdmc.FdmcError):
+        engine.greeks_accumulate_pca(fdmc.GBMParams(100.0, 0.05, 0.2, 1.0), fdmc.European(100.0, 0.05, 1.0),
+                                     1024, 1, 42, 0, 256, pca)
+
+
 def test_run_qmc_pca_chunks_large_doublings(engine):
     """run_qmc_cuda with PCA noise over 2048 dates (the largest step count the
     reference's examples use) and 32 replicas: the doubling from 2^16 to 2^17
