@@ -36,6 +36,12 @@ constexpr uint32_t TAG_QMC_FALLBACK = 0x51u, TAG_MLMC = 0x100u, TAG_CV_PILOT = 0
                    TAG_CV_MAIN = 0x201u, TAG_IS = 0x300u;
 }  // namespace
 
+// Device memory of fdmc_qmc_accumulate_noalloc: FDMC_NOALLOC_SCRATCH_BYTES per
+// device, held from the load of the code object until the process ends.  Host
+// code never uses the array's name as an address: fdmc_context_create looks up
+// this device's address with hipGetSymbolAddress and keeps it in the context.
+static __device__ __align__(256) unsigned char scratch[FDMC_NOALLOC_SCRATCH_BYTES];
+
 struct fdmc_bb {
     fdmc_context* ctx;
     int N;
@@ -209,6 +215,15 @@ const uint32_t* get_sdir(fdmc_context* c, const std::vector<uint32_t>& salts, in
 }
 
 // ---------------------------------------------------------------- one fused pass
+// Outputs of a pass that may allocate nothing (fdmc_qmc_accumulate_noalloc):
+// direction records and partials in the static scratch array, sums reduced
+// into the caller's device buffer.
+struct NoAlloc {
+    unsigned char* scratch = nullptr;   // device address of the scratch array
+    size_t cap = 0;
+    double* d_sums = nullptr;           // caller's device buffer, R x n_acc doubles
+};
+
 struct PassSpec {
     const fdmc_model* model = nullptr;
     const fdmc_payoff* payoff = nullptr;
@@ -232,13 +247,16 @@ int run_basket_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out
 int pca_rows(fdmc_context* c, const fdmc_pca* pca, const uint32_t* sdir, int D, int R, int64_t first,
              int64_t count, bool bf16);
 
-// Runs one pass; writes R x nacc sums into out, returns nacc.
-int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
+// Runs one pass; writes R x nacc sums into out (into na->d_sums when na is
+// given, out left untouched), returns nacc.
+int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out, const NoAlloc* na = nullptr) {
     const fdmc_model& m = *s.model;
     const fdmc_payoff& p = *s.payoff;
     if (s.n_steps <= 0) fail(FDMC_E_INVALID, "n_steps must be positive");
     if (s.R <= 0 || s.R > 65535) fail(FDMC_E_INVALID, "R out of range");
     if (s.count <= 0) fail(FDMC_E_INVALID, "number of points must be positive");
+    if (na && (is_basket(m) || s.prng || s.greeks || (s.mode != FDMC_NOISE_RAW && s.mode != FDMC_NOISE_BB)))
+        fail(FDMC_E_UNSUPPORTED, "the no-allocation pass takes one-factor models with Raw or BB Sobol noise");
     if (is_basket(m)) return run_basket_pass(c, s, out);
     const int nf = noise_dim(m);
     const int64_t D64 = (int64_t)s.n_steps * nf;
@@ -281,6 +299,7 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     int nz, lb = 0;
     const void* fn = nullptr;
     if (use_prng) {
+        if (na) fail(FDMC_E_UNSUPPORTED, "the no-allocation pass needs Sobol points (dimension <= 20000)");
         if (s.mode != FDMC_NOISE_RAW && !s.prng)
             fail(FDMC_E_UNSUPPORTED, "BB/PCA need Sobol points (dimension <= 20000)");
         nz = NZ_PRNG;
@@ -313,7 +332,16 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     }
     if (!fn) fail(FDMC_E_UNSUPPORTED, "model/payoff/control combination not supported");
 
-    if (nz != NZ_PRNG) {
+    // scratch layout of a no-allocation pass: direction records, then partials
+    const size_t na_sdir_bytes = ((size_t)s.R * D * SDIR_WORDS * sizeof(uint32_t) + 255) / 256 * 256;
+    if (na) {
+        if (na_sdir_bytes > na->cap)
+            fail(FDMC_E_RANGE, "R x D too large for the no-allocation scratch (R * D * 256 B must fit in " +
+                                   std::to_string(na->cap >> 20) + " MiB)");
+        uint32_t* rec = (uint32_t*)na->scratch;
+        HIPC(launch_build_sdir(c->d_dirv, nullptr, s.R, D, rec, c->stream, s.salt_base));
+        a.sdir = rec;
+    } else if (nz != NZ_PRNG) {
         std::vector<uint32_t> salts((size_t)s.R);
         for (int r = 0; r < s.R; r++) salts[r] = s.salt_base + (uint32_t)r;
         a.sdir = get_sdir(c, salts, D);
@@ -339,8 +367,18 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     a.tiles_per_wg = (int)tpw;
     a.nwg_x = (int)nwg;
     const int nacc = ((cv == CV_NONE) ? 2 : 7) + (s.greeks ? 6 : 0);
-    a.partials = (double*)c->partials.ensure((size_t)s.R * nwg * nacc * sizeof(double));
-    double* d_sums = (double*)c->sums.ensure((size_t)s.R * nacc * sizeof(double));
+    double* d_sums;
+    if (na) {
+        const size_t pbytes = (size_t)s.R * nwg * nacc * sizeof(double);
+        if (na_sdir_bytes + pbytes > na->cap)
+            fail(FDMC_E_RANGE, "pass too large for the no-allocation scratch (" + std::to_string(na->cap >> 20) +
+                                   " MiB)");
+        a.partials = (double*)(na->scratch + na_sdir_bytes);
+        d_sums = na->d_sums;
+    } else {
+        a.partials = (double*)c->partials.ensure((size_t)s.R * nwg * nacc * sizeof(double));
+        d_sums = (double*)c->sums.ensure((size_t)s.R * nacc * sizeof(double));
+    }
 
     const size_t lds = s.greeks ? greeks_lds_bytes(nz, D, a.G, s.n_steps) : path_lds_bytes(nz, D, a.G);
     if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "LDS footprint of this pass exceeds 160 KiB");
@@ -351,9 +389,11 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
     HIPC(hipLaunchKernel(fn, dim3((unsigned)nwg, (unsigned)s.R), dim3(256), args, lds, c->stream));
     if (time_path) HIPC(hipEventRecord(c->ev1, c->stream));
     HIPC(launch_reduce_partials(a.partials, s.R, (int)nwg, nacc, d_sums, c->stream));
-    ensure_host_sums(c, (size_t)s.R * nacc);
-    HIPC(hipMemcpyAsync(c->h_sums, d_sums, (size_t)s.R * nacc * sizeof(double), hipMemcpyDeviceToHost,
-                        c->stream));
+    if (!na) {
+        ensure_host_sums(c, (size_t)s.R * nacc);
+        HIPC(hipMemcpyAsync(c->h_sums, d_sums, (size_t)s.R * nacc * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    }
     HIPC(hipStreamSynchronize(c->stream));
     if (c->timing) {
         float ms = 0.0f;
@@ -361,7 +401,7 @@ int run_pass(fdmc_context* c, const PassSpec& s, std::vector<double>& out) {
         c->timed_ms += ms;
         c->timed_launches++;
     }
-    out.assign(c->h_sums, c->h_sums + (size_t)s.R * nacc);
+    if (!na) out.assign(c->h_sums, c->h_sums + (size_t)s.R * nacc);
     return nacc;
 }
 
@@ -1222,6 +1262,8 @@ fdmc_status fdmc_context_create(int32_t device, fdmc_context** out) {
             c->n_cu = prop.multiProcessorCount;
         try {
             HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            // the no-allocation entry point's scratch array on this (current) device
+            HIPC(hipGetSymbolAddress((void**)&c->noalloc_scratch, HIP_SYMBOL(scratch)));
             std::vector<uint32_t> dv;
             expand_dirvectors(dv);
             HIPC(hipMalloc((void**)&c->d_dirv, dv.size() * sizeof(uint32_t)));
@@ -1405,6 +1447,34 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* c, const fdmc_model* model, const 
         std::vector<double> v;
         const int nacc = run_pass(c, s, v);
         std::memcpy(sums_out, v.data(), v.size() * sizeof(double));
+        if (n_acc_out) *n_acc_out = nacc;
+    });
+}
+
+fdmc_status fdmc_qmc_accumulate_noalloc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff,
+                                        const fdmc_model* cmodel, const fdmc_payoff* cpayoff, double E_ctrl,
+                                        double beta, int32_t n_steps, int32_t R, uint32_t salt_base,
+                                        int64_t offset, int64_t n_points, int32_t mode, const fdmc_bb* bb,
+                                        double* d_sums_out, int32_t* n_acc_out) {
+    return guarded([&] {
+        if (!c || !d_sums_out) fail(FDMC_E_INVALID, "null argument");
+        if (!c->noalloc_scratch) fail(FDMC_E_DEVICE, "no-allocation scratch not available on this device");
+        validate_model(model);
+        validate_payoff(payoff);
+        if ((cmodel == nullptr) != (cpayoff == nullptr))
+            fail(FDMC_E_INVALID, "control model and payoff must both be given");
+        if (cmodel) { validate_model(cmodel); validate_payoff(cpayoff); }
+        if (offset < 0) fail(FDMC_E_INVALID, "offset must be non-negative");
+        PassSpec s;
+        s.model = model; s.payoff = payoff; s.cmodel = cmodel; s.cpayoff = cpayoff;
+        s.E_ctrl = E_ctrl; s.beta = beta; s.n_steps = n_steps; s.R = R; s.salt_base = salt_base;
+        s.first = offset; s.count = n_points; s.mode = mode; s.bb = bb;
+        NoAlloc na;
+        na.scratch = c->noalloc_scratch;
+        na.cap = FDMC_NOALLOC_SCRATCH_BYTES;
+        na.d_sums = d_sums_out;
+        std::vector<double> unused;   // stays empty: sums go to d_sums_out
+        const int nacc = run_pass(c, s, unused, &na);
         if (n_acc_out) *n_acc_out = nacc;
     });
 }

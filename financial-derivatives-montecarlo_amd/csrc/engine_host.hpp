@@ -92,6 +92,7 @@ struct fdmc_context {
     int sdir_D = -1;
     double* h_sums = nullptr;     // pinned
     size_t h_sums_cap = 0;
+    unsigned char* noalloc_scratch = nullptr;   // this device's address of the static scratch array
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int64_t timed_launches = 0;

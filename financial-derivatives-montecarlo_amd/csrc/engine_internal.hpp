@@ -10,8 +10,9 @@
 namespace fdmc {
 
 // k_sobol.hip
+// d_salts null: replica salts salt_base + slot (no salt array in device memory)
 hipError_t launch_build_sdir(const uint32_t* dirv, const uint32_t* d_salts, int n_slots, int D,
-                             uint32_t* out, hipStream_t st);
+                             uint32_t* out, hipStream_t st, uint32_t salt_base = 0);
 hipError_t launch_sobol_gen(const uint32_t* sdir, int D, int64_t n, uint32_t offset, bool normal,
                             void* out, hipStream_t st);
 hipError_t launch_icdf(const uint32_t* in, int64_t n, float* out, hipStream_t st);

@@ -13,14 +13,16 @@
 
 namespace fdmc {
 
-// One thread per (slot, dimension).  salts[slot] is the replica salt.
+// One thread per (slot, dimension).  salts[slot] is the replica salt, or
+// salt_base + slot when salts is null.
 __global__ __launch_bounds__(256) void k_build_sdir(const uint32_t* __restrict__ dirv,
-                                                    const uint32_t* __restrict__ salts, int n_slots,
-                                                    int D, uint32_t* __restrict__ out) {
+                                                    const uint32_t* __restrict__ salts, uint32_t salt_base,
+                                                    int n_slots, int D, uint32_t* __restrict__ out) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= (int64_t)n_slots * D) return;
     const int slot = (int)(gid / D), d = (int)(gid - (int64_t)slot * D);
-    const uint32_t base = splitmix32(salts[slot]) ^ (uint32_t)d;
+    const uint32_t salt = salts ? salts[slot] : salt_base + (uint32_t)slot;
+    const uint32_t base = splitmix32(salt) ^ (uint32_t)d;
     uint32_t row[32];
 #pragma unroll
     for (int k = 0; k < 32; k++) {
@@ -148,10 +150,10 @@ __global__ void k_zero_u32(unsigned int* p, int n) {
 
 // ---- host launchers ----------------------------------------------------------
 hipError_t launch_build_sdir(const uint32_t* dirv, const uint32_t* d_salts, int n_slots, int D,
-                             uint32_t* out, hipStream_t st) {
+                             uint32_t* out, hipStream_t st, uint32_t salt_base) {
     const int64_t n = (int64_t)n_slots * D;
     const int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(k_build_sdir, dim3(blocks), dim3(256), 0, st, dirv, d_salts, n_slots, D, out);
+    hipLaunchKernelGGL(k_build_sdir, dim3(blocks), dim3(256), 0, st, dirv, d_salts, salt_base, n_slots, D, out);
     return hipGetLastError();
 }
 

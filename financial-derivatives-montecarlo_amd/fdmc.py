@@ -414,6 +414,10 @@ _SIGS = {
                                       C.POINTER(CPayoff), C.c_double, C.c_double, C.c_int32, C.c_int32,
                                       C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P, _P,
                                       C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "fdmc_qmc_accumulate_noalloc": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.POINTER(CModel),
+                                              C.POINTER(CPayoff), C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                              C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P, _P,
+                                              C.POINTER(C.c_int32)]),
     "fdmc_levels_accumulate": (C.c_int, [_P, C.POINTER(CModel), C.POINTER(CPayoff), C.c_int32, C.c_int32,
                                          C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64),

@@ -310,6 +310,26 @@ fdmc_status fdmc_qmc_accumulate(fdmc_context* ctx, const fdmc_model* model,
                                 const fdmc_bb* bb, const fdmc_pca* pca, double* sums_out,
                                 int32_t* n_acc_out);
 
+/* fdmc_qmc_accumulate that allocates nothing (no counterpart in the reference):
+ * for callers that cannot allocate device or pinned memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  One-factor
+ * models, Raw or BB noise on Sobol points (D = n_steps <= 20000), an optional
+ * control leg; the same kernels and summation order as fdmc_qmc_accumulate, so
+ * the sums are identical.  The scrambled direction records (R x D x 256 B) and the
+ * per-workgroup FP64 partials live in a static device array of
+ * FDMC_NOALLOC_SCRATCH_BYTES per device (held until the process ends); a pass
+ * that needs more is rejected with FDMC_E_RANGE.  d_sums_out: device memory of
+ * R x n_acc doubles (n_acc = 2, or 7 with a control), written on the context's
+ * stream; the call returns after the device work is done.  Calls that overlap on
+ * one device (from several threads or contexts) share the array and race. */
+#define FDMC_NOALLOC_SCRATCH_BYTES (32u << 20)
+fdmc_status fdmc_qmc_accumulate_noalloc(fdmc_context* ctx, const fdmc_model* model,
+                                        const fdmc_payoff* payoff, const fdmc_model* ctrl_model,
+                                        const fdmc_payoff* ctrl_payoff, double E_ctrl, double beta,
+                                        int32_t n_steps, int32_t R, uint32_t salt_base,
+                                        int64_t offset, int64_t n_points, int32_t mode,
+                                        const fdmc_bb* bb, double* d_sums_out, int32_t* n_acc_out);
+
 /* One coupled fine/coarse pass (the body of launch_level, methods_cuda.cu:1347-1395
  * and 1800-1924) for n_req (level, replica, sample range) requests, run as one
  * persistent launch.  qmc = 0: Philox paths (seed = salt_or_seed, stream
