@@ -1,8 +1,8 @@
 // device_math.hpp -- per-lane math shared by the gfx950 path kernels.
 //
 //  * splitmix32 / Hong-Hickernell row products (methods_cuda.cu:753-795)
-//  * inverse normal CDF: own least-squares fit (gen/fit_icdf.py) of the
-//    Giles-form ratio as a table of piecewise cubics read from LDS
+//  * inverse normal CDF: own least-squares fit (gen/fit_icdf.py) of -Phi^-1 on
+//    the folded uniform as a table of piecewise cubics read from LDS
 //  * Philox4x32-10 (Salmon et al., SC'11) + Box-Muller for pseudo-random paths
 //  * Euler steppers and payoff accumulators (methods_cuda.cu:271-352)
 //  * wave64 FP64 reductions
@@ -29,16 +29,15 @@ __device__ __forceinline__ float fast_ln(float x) { return __builtin_amdgcn_logf
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Inverse standard normal CDF of u = (s + 1/2) 2^-32, never 0 or 1, in the
-// form of M. Giles ("Approximating the erfinv function", GPU Computing Gems
-// Jade, 2011): with q = min(u, 1-u) folded to the lower half from the integer
-// (both tails keep full relative precision) and w = -ln(4 q (1-q)),
-// Phi^-1(u) = +-(1 - 2q) f(w) with f smooth; the coefficients are fitted in
-// gen/fit_icdf.py.  f is a table of cubics, 16 segments per binade of t = q (1-q):
-// the row is bits 19..27 of t's FP32 encoding (low 5 exponent bits, top 4
-// mantissa bits; t in [2^-33, 1/4] spans 32 binades), the variable the
-// remaining 19 mantissa bits as an integer.  No logarithm and no branch: a
-// 16-byte LDS read and three FMAs.  Kernels copy the table into LDS with
+// Inverse standard normal CDF of u = (s + 1/2) 2^-32, never 0 or 1: with
+// q = min(u, 1-u) folded to the lower half from the integer (both tails keep
+// full relative precision), Phi^-1(u) = -+g(q), g = -Phi^-1 >= 0 on (0, 1/2].
+// g is a table of cubics (own fit, gen/fit_icdf.py), 16 segments per binade of
+// q read directly from q's FP32 encoding: the row is bits 19..31 of the
+// encoding (biased exponent and top 4 mantissa bits) minus FDMC_ICDF_ROW0 (q in
+// [2^-33, 1/2] spans biased exponents 94..126), the variable the remaining 19
+// mantissa bits as an integer.  No logarithm and no branch: a 16-byte LDS read,
+// three FMAs and the caller's scale.  Kernels copy the table into LDS with
 // icdf_tab_load before use.
 static __constant__ __align__(16) float g_icdf_tab[FDMC_ICDF_TAB_ROWS * 4] = FDMC_ICDF_TAB;
 
@@ -46,25 +45,24 @@ __device__ __forceinline__ void icdf_tab_load(float4* lds, int tid, int nthreads
     for (int i = tid; i < FDMC_ICDF_TAB_ROWS; i += nthreads) lds[i] = ((const float4*)g_icdf_tab)[i];
 }
 
-// -sd Phi^-1((s + 1/2) 2^-32) from the LDS table, for a scale sd > 0 and
-// m2sd = -2 sd (kept by the caller).  The fold to the lower half comes from
-// reading s as a signed integer i: f + 1/2 = |i + 1/2| for f = s (i >= 0) and
-// f = ~s (i < 0), and the absolute value is a free source modifier.  (Near
-// u = 1/2, i = 2^31 - 1 rounds to 2^31 in FP32 and the result's sign can flip
-// on a value of size 2^-32 sd.)  The result takes its sign bit from s with one
-// OR, and callers fold the negation into the sum they add it to.
-__device__ __forceinline__ float icdf_u32_tab_scaled_neg(uint32_t s, const float4* tab, float sd, float m2sd) {
-    const float q = fabsf(fmaf((float)(int32_t)s, 0x1p-32f, 0x1p-33f));
-    const uint32_t b = __float_as_uint(fmaf(-q, q, q));    // t = q (1 - q)
-    const float4 c = *(const float4*)((const char*)tab + ((b >> 15) & 0x1FF0u));
+// -sd Phi^-1((s + 1/2) 2^-32) from the LDS table, for a scale sd > 0.  The
+// fold to the lower half comes from reading s as a signed integer i:
+// v = i 2^-32 + 2^-33 has |v| = q for both halves, and the row index and the
+// variable are read from v's bits below the sign bit, so |v| is never formed.
+// (Within 64 of u = 1/2, FP32 rounds q to exactly 1/2, whose row gives g = 0.)
+// The result takes its sign bit from s with one OR, and callers fold the
+// negation into the sum they add it to.
+__device__ __forceinline__ float icdf_u32_tab_scaled_neg(uint32_t s, const float4* tab, float sd) {
+    const uint32_t b = __float_as_uint(fmaf((float)(int32_t)s, 0x1p-32f, 0x1p-33f));
+    const float4 c = tab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
     const float x = (float)(b & 0x7FFFFu);
     const float p = fmaf(fmaf(fmaf(c.w, x, c.z), x, c.y), x, c.x);
-    const float y = fmaf(q, m2sd, sd) * p;                 // (1 - 2q) sd f >= 0
+    const float y = sd * p;                                // sd g(q) >= 0
     return __uint_as_float(__float_as_uint(y) | (s & 0x80000000u));
 }
 
 __device__ __forceinline__ float icdf_u32_tab(uint32_t s, const float4* tab) {
-    return -icdf_u32_tab_scaled_neg(s, tab, 1.0f, -2.0f);
+    return -icdf_u32_tab_scaled_neg(s, tab, 1.0f);
 }
 
 // ---- Philox4x32-10 -------------------------------------------------------

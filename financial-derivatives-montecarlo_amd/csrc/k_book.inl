@@ -5,6 +5,18 @@
 
 namespace fdmc {
 
+// Workgroup-uniform tables (the local-vol time factors, the bridge sds, the
+// entries and the contracts) are read through the constant address space:
+// nothing writes them while the kernel runs, and the compiler then issues
+// scalar loads into SGPRs instead of vector loads into VGPRs (it cannot prove
+// generic global pointers unclobbered in a kernel that also stores).
+template <class T>
+using cst = const __attribute__((address_space(4))) T;
+template <class T>
+__device__ __forceinline__ cst<T>* as_const(const T* p) {
+    return (cst<T>*)p;
+}
+
 struct BookGeom {
     int l, nf, nc, G;
     float hf, hc, muh_f, muh_c, bgk_f, bgk_c, T;
@@ -163,9 +175,9 @@ struct BookAdj {
 //   theta_bar += sum_k d_k dS_k/dtheta + adj_end dS_end/dtheta.
 // For a one-dimensional state this is the exact vector-Jacobian product of the
 // block, with no per-step storage.
-// PAIR: the step increments are dw[2k] + dw[2k+1] (coarse leg), whose step
-// times are the even fine times (vb holds the block's fine-grid time factors,
-// in registers).  The direct terms d_k by payoff: the Asian's `each` at every
+// PAIR: the coarse leg, whose increments dw[k] are those of the bridge's
+// two-date intervals and whose step times are the even fine times (vb holds the
+// block's fine-grid time factors, in registers).  The direct terms d_k by payoff: the Asian's `each` at every
 // date, the lookback's `cmin` at its running minimum; the final date's `last`
 // is the initial adjoint of the sweep (A.adj at S_n), so no step carries it.
 template <int MK, int PK, int N, bool PAIR>
@@ -179,7 +191,7 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
     for (int k = 0; k < N; k++) {
         if (k < nsteps) {
             const int gi = k0 + k;
-            const float w = PAIR ? dw[2 * k] + dw[2 * k + 1] : dw[k];
+            const float w = dw[k];
             const float v = vb[PAIR ? 2 * k : k];
             float fS, fv, fm, f0;
             book_partials<MK>(P, S, w, v, h, muh, fS, fv, fm, f0);
@@ -275,28 +287,35 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
 // dm[d]: XOR of the current tile's high Sobol words relative to the words in
 // the LDS records (the work item's first tile), per dimension.
 template <int MK, int PK, int GM, int NF>
-__device__ __forceinline__ void book_path_short(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                                const float* __restrict__ vt, const SobolLane& sl,
+__device__ __forceinline__ void book_path_short(const LegParams& P, const BookGeom& g, cst<float>* sdt,
+                                                cst<float>* vt, const SobolLane& sl,
                                                 const float4* itab, const uint32_t (&dm)[NF], bool valid,
                                                 double (&acc)[BOOK_NACC]) {
     constexpr bool TAN = LIP_TAN(PK, GM);
     constexpr int NC = NF / 2;
-    float W[NF + 1];
-    W[0] = 0.0f;
-    // icdf_u32_tab_scaled_neg gives -sd z: the bridge subtracts it
-    W[NF] = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0) ^ dm[0], itab, sdt[0], -2.0f * sdt[0]);
+    // the bridge in increments: dw[0] starts as W_T; level lev splits each
+    // interval's increment d at its midpoint into d/2 - y and d/2 + y (y = -sd z
+    // of node 2^lev + q), which is W_M = (W_L + W_R)/2 + sd z written for the
+    // increments; the two-date intervals' increments are the coarse steps
+    float dw[NF > 1 ? NF : 1], dc[NC > 0 ? NC : 1];
+    // icdf_u32_tab_scaled_neg gives -sd z
+    dw[0] = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0) ^ dm[0], itab, sdt[0]);
 #pragma unroll
-    for (int j = 1; j < NF; j++) {
-        const int lev = 31 - __builtin_clz(j);
-        const int q = j - (1 << lev);
-        const int st = NF >> lev;
-        const int L = q * st, R = L + st, M = L + (st >> 1);
+    for (int lev = 0; (1 << lev) < NF; lev++) {
+        if ((2 << lev) == NF) {
+#pragma unroll
+            for (int q = 0; q < NC; q++) dc[q] = dw[q];
+        }
         const float sd = sdt[lev + 1];
-        W[M] = 0.5f * (W[L] + W[R]) - icdf_u32_tab_scaled_neg(sobol_coord(sl, j) ^ dm[j], itab, sd, -2.0f * sd);
-    }
-    float dw[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) dw[k] = W[k + 1] - W[k];
+        for (int q = (1 << lev) - 1; q >= 0; q--) {
+            const int j = (1 << lev) + q;
+            const float y = icdf_u32_tab_scaled_neg(sobol_coord(sl, j) ^ dm[j], itab, sd);
+            const float d = dw[q];
+            dw[2 * q] = fmaf(d, 0.5f, -y);
+            dw[2 * q + 1] = fmaf(d, 0.5f, y);
+        }
+    }
     BookLeg<MK, PK, TAN> F, Cc;
     F.init(P);
     Cc.init(P);
@@ -306,12 +325,12 @@ __device__ __forceinline__ void book_path_short(const LegParams& P, const BookGe
 #pragma unroll
     for (int k = 0; k < NF; k++) {
         F.step(P, dw[k], vb[k], g.hf, g.muh_f, k + 1);
-        if (k & 1) Cc.step(P, dw[k - 1] + dw[k], vb[k - 1], g.hc, g.muh_c, (k >> 1) + 1);
+        if (k & 1) Cc.step(P, dc[k >> 1], vb[k - 1], g.hc, g.muh_c, (k >> 1) + 1);
     }
     book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,
                           [&](float ef, float mf, float ec, float mc, BookAdj& Af, BookAdj& Ac) {
                               if constexpr (NC > 0)
-                                  book_block_adj<MK, PK, NC, true>(P, P.S0, dw, NC, 0, vb, g.hc, g.muh_c, ec, mc,
+                                  book_block_adj<MK, PK, NC, true>(P, P.S0, dc, NC, 0, vb, g.hc, g.muh_c, ec, mc,
                                                                    Cc.kmin, Ac);
                               book_block_adj<MK, PK, NF, false>(P, P.S0, dw, NF, 0, vb, g.hf, g.muh_f, ef, mf,
                                                                 F.kmin, Af);
@@ -330,8 +349,8 @@ __device__ __forceinline__ void book_path_short(const LegParams& P, const BookGe
 // inner points are generated in registers; the adjoint regenerates them block
 // by block from the Sobol index, from checkpoints of S.
 template <int MK, int PK, int GM>
-__device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                               const float* __restrict__ vt, const SobolLane& sl, const float4* itab,
+__device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeom& g, cst<float>* sdt,
+                                               cst<float>* vt, const SobolLane& sl, const float4* itab,
                                                float* ckf, float* ckc, bool valid, double (&acc)[BOOK_NACC]) {
     constexpr bool ADJ = PK != PK_BARRIER && GM == 1;
     constexpr bool TAN = LIP_TAN(PK, GM);
@@ -344,33 +363,33 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
         const int lev = m - 1 - k;
         const float sd = sdt[lev + 1];
         return 0.5f * (WL + WR) -
-               icdf_u32_tab_scaled_neg(sobol_coord(sl, (1 << lev) + (L >> (k + 1))), itab, sd, -2.0f * sd);
+               icdf_u32_tab_scaled_neg(sobol_coord(sl, (1 << lev) + (L >> (k + 1))), itab, sd);
     };
     // icdf_u32_tab_scaled_neg gives -sd z: the bridge subtracts it
-    const float WG = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0), itab, sdt[0], -2.0f * sdt[0]);
-    float sdb[4], m2sdb[4];
+    const float WG = -icdf_u32_tab_scaled_neg(sobol_coord(sl, 0), itab, sdt[0]);
+    float sdb[4];
 #pragma unroll
-    for (int lv = 0; lv < 4; lv++) {
-        sdb[lv] = sdt[m + lv + 1];
-        m2sdb[lv] = -2.0f * sdb[lv];
-    }
-    auto gen_block = [&](int i, float WL, float WR, float (&dw)[16]) {
-        float W[17];
-        W[0] = WL;
-        W[16] = WR;
+    for (int lv = 0; lv < 4; lv++) sdb[lv] = sdt[m + lv + 1];
+    // the block's increments from W at its ends, level by level in increments
+    // (as the short path): each interval's increment d splits into d/2 - y and
+    // d/2 + y; dc receives the two-date intervals' increments (coarse steps)
+    auto gen_block = [&](int i, float WL, float WR, float (&dw)[16], float (&dc)[8]) {
+        dw[0] = WR - WL;
 #pragma unroll
         for (int lv = 0; lv < 4; lv++) {
+            if (lv == 3) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) dc[q] = dw[q];
+            }
             const int j0 = (G << lv) + (i << lv);
 #pragma unroll
-            for (int q = 0; q < (1 << lv); q++) {
-                const int s = 16 >> lv;
-                const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                W[Mp] = 0.5f * (W[Lp] + W[Rp]) -
-                        icdf_u32_tab_scaled_neg(sobol_coord(sl, j0 + q), itab, sdb[lv], m2sdb[lv]);
+            for (int q = (1 << lv) - 1; q >= 0; q--) {
+                const float y = icdf_u32_tab_scaled_neg(sobol_coord(sl, j0 + q), itab, sdb[lv]);
+                const float d = dw[q];
+                dw[2 * q] = fmaf(d, 0.5f, -y);
+                dw[2 * q + 1] = fmaf(d, 0.5f, y);
             }
         }
-#pragma unroll
-        for (int k = 0; k < 16; k++) dw[k] = W[k + 1] - W[k];
     };
 
     BookLeg<MK, PK, TAN> F, Cc;
@@ -400,9 +419,9 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
                 else if (k < c) rt[k] = node(k, i, Wp, rt[k + 1]);
             }
         }
-        float dw[16], vb[16];
+        float dw[16], dc[8], vb[16];
         load_vb(i, vb);
-        gen_block(i, Wp, rt[0], dw);
+        gen_block(i, Wp, rt[0], dw, dc);
         if constexpr (ADJ) {
             ckf[i * 256] = F.S;
             ckc[i * 256] = Cc.S;
@@ -411,7 +430,7 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
         for (int k = 0; k < 16; k++) {
             const int gi = i * 16 + k;
             F.step(P, dw[k], vb[k], g.hf, g.muh_f, gi + 1);
-            if (k & 1) Cc.step(P, dw[k - 1] + dw[k], vb[k - 1], g.hc, g.muh_c, (gi >> 1) + 1);
+            if (k & 1) Cc.step(P, dc[k >> 1], vb[k - 1], g.hc, g.muh_c, (gi >> 1) + 1);
         }
     }
     book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,
@@ -434,10 +453,10 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
                                           else if (k < c) lt[k] = node(k, p - (2 << k), lt[k + 1], WR);
                                       }
                                   }
-                                  float dw[16], vb[16];
+                                  float dw[16], dc[8], vb[16];
                                   load_vb(i, vb);
-                                  gen_block(i, lt[0], WR, dw);
-                                  book_block_adj<MK, PK, 8, true>(P, ckc[i * 256], dw, 8, i * 8, vb, g.hc,
+                                  gen_block(i, lt[0], WR, dw, dc);
+                                  book_block_adj<MK, PK, 8, true>(P, ckc[i * 256], dc, 8, i * 8, vb, g.hc,
                                                                   g.muh_c, ec, mc, Cc.kmin, Ac);
                                   book_block_adj<MK, PK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vb, g.hf,
                                                                     g.muh_f, ef, mf, F.kmin, Af);
@@ -460,8 +479,8 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
 // direction numbers of bits 7..31 in LDS (hl[d][25]): tile n changes
 // gray(256 n) in bits 7 and 8 + ctz(n) only.  No barrier between tiles.
 template <int MK, int GM, int NF>
-__device__ __forceinline__ void book_tiles_short(const LegParams& P, const BookGeom& g, const float* __restrict__ sdt,
-                                                 const float* __restrict__ vt, const SobolLane& sl,
+__device__ __forceinline__ void book_tiles_short(const LegParams& P, const BookGeom& g, cst<float>* sdt,
+                                                 cst<float>* vt, const SobolLane& sl,
                                                  const float4* itab, const uint32_t* hl, int kind, int64_t tfirst,
                                                  int t0, int t1, int64_t first, int64_t count, int tid,
                                                  double (&acc)[BOOK_NACC]) {
@@ -537,25 +556,25 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         int lo = 0, hi = a.n_entries - 1; // last entry with tile_begin <= item
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (a.entries[mid].tile_begin <= item) lo = mid;
+            if (as_const(a.entries)[mid].tile_begin <= item) lo = mid;
             else hi = mid - 1;
         }
         // the entry and contract are uniform over the workgroup, but the
         // compiler cannot prove their tables read-only and loads them with
         // vector loads: readfirstlane moves every field into scalar registers
         // (fewer VGPRs live across the tile loops)
-        const BookEntry& Ev = a.entries[lo];
+        cst<BookEntry>& Ev = as_const(a.entries)[lo];
         BookEntry E;
-        E.c = sreg(Ev.c);
-        E.l = sreg(Ev.l);
-        E.r = sreg(Ev.r);
-        E.tpi = sreg(Ev.tpi);
-        E.tile_begin = sreg(Ev.tile_begin);
+        E.c = Ev.c;
+        E.l = Ev.l;
+        E.r = Ev.r;
+        E.tpi = Ev.tpi;
+        E.tile_begin = Ev.tile_begin;
         E.ntiles = 0;
-        E.first = sreg(Ev.first);
-        E.count = sreg(Ev.count);
-        const BookContract& C = a.contracts[E.c];
-        const double CT = sreg(C.T);
+        E.first = Ev.first;
+        E.count = Ev.count;
+        cst<BookContract>& C = as_const(a.contracts)[E.c];
+        const double CT = C.T;
         BookGeom g;
         g.l = E.l;
         g.nf = 1 << E.l;
@@ -565,19 +584,19 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         g.hc = g.nc > 0 ? (float)(CT / (double)g.nc) : 0.0f;
         g.muh_f = a.leg.mu * g.hf;
         g.muh_c = a.leg.mu * g.hc;
-        g.bgk_f = sreg(C.bgk[E.l]);
-        g.bgk_c = E.l > 0 ? sreg(C.bgk[E.l - 1]) : 1.0f;
-        g.T = sreg(C.Tf);
-        const int kind = sreg(C.kind);
+        g.bgk_f = C.bgk[E.l];
+        g.bgk_c = E.l > 0 ? C.bgk[E.l - 1] : 1.0f;
+        g.T = C.Tf;
+        const int kind = C.kind;
         LegParams P = a.leg;
-        P.K = sreg(C.K);
-        P.B = sreg(C.B);
-        P.discount = sreg(C.discount);
+        P.K = C.K;
+        P.B = C.B;
+        P.discount = C.discount;
         const int D = g.nf;
-        const int slot = sreg(C.slot[E.l]), t_idx = sreg(C.t_idx);
+        const int slot = C.slot[E.l], t_idx = C.t_idx;
         const uint32_t* grec = a.sdir[E.l] + ((size_t)slot * a.R + E.r) * (size_t)D * SDIR_WORDS;
-        const float* sdt = a.bbtab + ((size_t)t_idx * BOOK_NL + E.l) * BOOK_NL;
-        const float* vt = a.voltab + (size_t)t_idx * BOOK_VT_STRIDE + (size_t)(D - 1);
+        cst<float>* sdt = as_const(a.bbtab) + ((size_t)t_idx * BOOK_NL + E.l) * BOOK_NL;
+        cst<float>* vt = as_const(a.voltab) + (size_t)t_idx * BOOK_VT_STRIDE + (size_t)(D - 1);
         const int64_t tfirst = E.first >> 8;
         const int nt_all = (int)(((E.first + E.count - 1) >> 8) - tfirst + 1);
         const int t0 = (item - E.tile_begin) * E.tpi;

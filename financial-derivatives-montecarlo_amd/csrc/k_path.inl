@@ -100,7 +100,7 @@ __device__ __forceinline__ float bb_mid(const PathLaunch& a, const SobolLane& sl
                                         float WR, int j, int depth) {
     if constexpr (UNI) {
         const float sd = a.bb_sd[depth];
-        return 0.5f * (WL + WR) - icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j), itab, sd, -2.0f * sd);
+        return 0.5f * (WL + WR) - icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j), itab, sd);
     } else {
         const float4 c = ((const float4*)a.bbw)[j];
         return c.x * WL + c.y * WR + c.z * icdf_u32_tab(bb_coord<LDS>(sl, j), itab);
@@ -110,7 +110,7 @@ __device__ __forceinline__ float bb_mid(const PathLaunch& a, const SobolLane& sl
 // W at the last date: sd sqrt(T) times coordinate 0's normal.
 template <bool UNI, bool LDS>
 __device__ __forceinline__ float bb_terminal(const PathLaunch& a, const SobolLane& sl, const float4* itab) {
-    if constexpr (UNI) return -icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, 0), itab, a.bb_sd[0], -2.0f * a.bb_sd[0]);
+    if constexpr (UNI) return -icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, 0), itab, a.bb_sd[0]);
     else return ((const float4*)a.bbw)[0].z * icdf_u32_tab(bb_coord<LDS>(sl, 0), itab);
 }
 

@@ -1,10 +1,10 @@
 """The FP32 inverse normal CDF's generated table (csrc/icdf_coeffs.h), evaluated
 on the CPU the way icdf_u32_tab_scaled_neg (csrc/device_math.hpp) evaluates it:
-q = |(float)(int32)s 2^-32 + 2^-33| (the fold to the lower half), t = q (1 - q),
-row = bits(t) >> 19 & 511, x = bits(t) & 0x7FFFF, FP32 Horner steps, the sign
-taken from s.  numpy rounds each multiply and add where the device fuses them,
-so the bound checked here is looser than the device's error and tighter than
-the GPU test's 4e-7."""
+q = |(float)(int32)s 2^-32 + 2^-33| (the fold to the lower half),
+row = (bits(q) >> 19) - FDMC_ICDF_ROW0, x = bits(q) & 0x7FFFF, FP32 Horner
+steps, the sign taken from s.  numpy rounds each multiply and add where the
+device fuses them, so the bound checked here is looser than the device's error
+and tighter than the GPU test's 4e-7."""
 import os
 import re
 
@@ -24,18 +24,23 @@ def _table():
     return tab
 
 
-def _icdf(s, tab):
+ROW0 = int(re.search(r"#define FDMC_ICDF_ROW0 (\d+)u", open(HDR).read()).group(1))
+
+
+def _fold(s):
     i = s.view(np.int32)
-    q = np.abs((i.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32))
-    t = (-q * q + q).astype(np.float32)
-    b = t.view(np.uint32)
-    c = tab[(b >> 19) & 0x1FF]
+    return np.abs((i.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32))
+
+
+def _icdf(s, tab):
+    q = _fold(s)
+    b = q.view(np.uint32)
+    c = tab[(b >> 19) - ROW0]
     x = (b & 0x7FFFF).astype(np.float32)
     p = c[:, 3]
     for k in (2, 1, 0):
         p = (p * x + c[:, k]).astype(np.float32)
-    y = ((q * F32(-2.0) + F32(1.0)) * p).astype(np.float32)
-    neg = (y.view(np.uint32) | (s & np.uint32(0x80000000))).view(np.float32)   # -Phi^-1
+    neg = (p.view(np.uint32) | (s & np.uint32(0x80000000))).view(np.float32)   # -Phi^-1
     return -neg.astype(np.float64)
 
 
@@ -56,18 +61,17 @@ def test_fp32_evaluation_error():
 
 
 def test_rows_reached_are_fitted():
-    # t in [2^-33, 1/4]: biased exponents 94..125; only t = 1/4 reads an
-    # exponent-125 row, and that row is the constant sqrt(pi/2)
+    # q in [2^-33, 1/2]: biased exponents 94..126; only q = 1/2 (s within 64 of
+    # 2^31, where FP32 rounds) reads an exponent-126 row, and those rows are 0
     tab = _table()
     s = _samples()
-    i = s.view(np.int32)
-    q = np.abs((i.astype(np.float32) * F32(2.0 ** -32) + F32(2.0 ** -33)).astype(np.float32))
-    b = (-q * q + q).astype(np.float32).view(np.uint32)
+    b = _fold(s).view(np.uint32)
     e = (b >> 23) & 0xFF
-    assert e.min() >= 94 and e.max() <= 125
-    assert np.all(b[e == 125] == np.float32(0.25).view(np.uint32))
-    row125 = tab[((np.uint32(125) << 4) & 0x1FF)]
-    assert abs(row125[0] - np.sqrt(np.pi / 2)) < 1e-7 and np.all(row125[1:] == 0)
+    assert e.min() == 94 and e.max() == 126
+    assert np.all(b[e == 126] == np.float32(0.5).view(np.uint32))
+    assert ((b >> 19) - ROW0).max() < len(tab)
+    assert np.all(tab[(126 - 94) * 16:] == 0.0)
+    assert np.all(tab[:(126 - 94) * 16, 0] > 0.0)
 
 
 def test_antisymmetry_to_rounding():
