@@ -36,8 +36,9 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 // q read directly from q's FP32 encoding: the row is bits 19..31 of the
 // encoding (biased exponent and top 4 mantissa bits) minus FDMC_ICDF_ROW0 (q in
 // [2^-33, 1/2] spans biased exponents 94..126), the variable the remaining 19
-// mantissa bits as an integer.  No logarithm and no branch: a 16-byte LDS read,
-// three FMAs and the caller's scale.  Kernels copy the table into LDS with
+// mantissa bits under the exponent of 1.0 (the float 1 + x 2^-23, one AND-OR).
+// No logarithm and no branch: a 16-byte LDS read, three FMAs and the caller's
+// scale.  Kernels copy the table into LDS with
 // icdf_tab_load before use.
 static __constant__ __align__(16) float g_icdf_tab[FDMC_ICDF_TAB_ROWS * 4] = FDMC_ICDF_TAB;
 
@@ -55,8 +56,8 @@ __device__ __forceinline__ void icdf_tab_load(float4* lds, int tid, int nthreads
 __device__ __forceinline__ float icdf_u32_tab_scaled_neg(uint32_t s, const float4* tab, float sd) {
     const uint32_t b = __float_as_uint(fmaf((float)(int32_t)s, 0x1p-32f, 0x1p-33f));
     const float4 c = tab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
-    const float x = (float)(b & 0x7FFFFu);
-    const float p = fmaf(fmaf(fmaf(c.w, x, c.z), x, c.y), x, c.x);
+    const float v = __uint_as_float((b & 0x7FFFFu) | 0x3F800000u);
+    const float p = fmaf(fmaf(fmaf(c.w, v, c.z), v, c.y), v, c.x);
     const float y = sd * p;                                // sd g(q) >= 0
     return __uint_as_float(__float_as_uint(y) | (s & 0x80000000u));
 }

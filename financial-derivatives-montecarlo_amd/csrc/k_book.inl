@@ -20,7 +20,17 @@ __device__ __forceinline__ cst<T>* as_const(const T* p) {
 struct BookGeom {
     int l, nf, nc, G;
     float hf, hc, muh_f, muh_c, bgk_f, bgk_c, T;
+    float unit;   // price units per path unit: s_ref for the local vol (see book_body), else 1
 };
+
+// The quadratic local vol's paths are stepped in units of s_ref (S0, K and B
+// divided by s_ref per work item), so log2(S / s_ref) is one log with no
+// multiply on every fine and coarse step.  Every payoff is linear in the path
+// (and the Euler step homogeneous in S), so the price and its derivatives with
+// respect to the vol level and the drift are s_ref times their values in path
+// units, and delta (a ratio of two prices) is unchanged.
+template <int MK>
+constexpr bool book_scaled = MK == MK_LVQ;
 
 // Sensitivities carried along a path: dS/dS0, dS/d(vol parameter), dS/dmu.
 struct Tan3 {
@@ -32,8 +42,8 @@ struct Tan3 {
 // VALU operations; the kernel is VALU-issue bound).  The local vol's time
 // factor comes from a table (uniform across the workgroup: a work item is one
 // (contract, level), so the step time is the same for every lane):
-//   quadratic local vol: v = b e^{-c t} ln(2)^2 and
-//       sigma_loc = a + v log2(S/s_ref)^2  (= a + b e^{-c t} ln(S/s_ref)^2);
+//   quadratic local vol: v = b e^{-c t} ln(2)^2 and, S in units of s_ref,
+//       sigma_loc = a + v log2(S)^2  (= a + b e^{-c t} ln(S/s_ref)^2);
 //   Dupire: v = e^{-alpha t}, sigma_loc = sigma0 v (S/S0)^(beta-1);  GBM: unused.
 template <int MK>
 __device__ __forceinline__ float book_sigma(const LegParams& P, float S, float v, float& aux) {
@@ -44,7 +54,7 @@ __device__ __forceinline__ float book_sigma(const LegParams& P, float S, float v
         aux = fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));   // (S/S0)^(beta-1)
         return P.sigma0 * v * aux;
     } else {
-        aux = fast_log2(S * P.inv_sref);                          // log2(S/s_ref)
+        aux = fast_log2(S);                                       // log2(S/s_ref): S in units of s_ref
         return fmaf(v, aux * aux, P.lv_a);
     }
 }
@@ -105,6 +115,18 @@ __device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t,
 // path.
 constexpr int BOOK_SHORT_L = 4;
 constexpr int BOOK_SHORT_D = 1 << BOOK_SHORT_L;
+
+// LDS slot of bridge dimension d on a level of G >= 2 blocks of 16 dates: the
+// skeleton's dimensions (d < G) keep their index, and the 15 in-block
+// dimensions of block i (G 2^lv + 2^lv i + q at in-block depth lv) sit together
+// at slots G + 15 i + 2^lv - 1 + q, so one block's records are read from one
+// base address with immediate offsets.
+__device__ __forceinline__ int book_slot(int d, int G) {
+    if (d < G) return d;
+    const int lv = 31 - __builtin_clz((uint32_t)d / (uint32_t)G);
+    const int r = d - (G << lv);
+    return G + 15 * (r >> lv) + (1 << lv) - 1 + (r & ((1 << lv) - 1));
+}
 
 // Greeks carried as forward tangents along the path (GM = 2, payoffs with a
 // pathwise derivative).
@@ -272,11 +294,11 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
             gf = gc = Tan3{0.0f, 0.0f, 0.0f};
     }
     if (valid) {
-        acc[0] += (double)(pf - pc);
+        acc[0] += (double)((pf - pc) * g.unit);
         if constexpr (ADJ || TAN) {     // no pathwise Greeks for the barrier: the sums stay 0
             acc[1] += (double)(gf.s - gc.s);
-            acc[2] += (double)(gf.v - gc.v);
-            acc[3] += (double)(gf.m - gc.m);
+            acc[2] += (double)((gf.v - gc.v) * g.unit);
+            acc[3] += (double)((gf.m - gc.m) * g.unit);
         }
     }
 }
@@ -375,16 +397,16 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
     // d/2 + y; dc receives the two-date intervals' increments (coarse steps)
     auto gen_block = [&](int i, float WL, float WR, float (&dw)[16], float (&dc)[8]) {
         dw[0] = WR - WL;
+        const int slot0 = G + 15 * i;       // the block's records (book_slot)
 #pragma unroll
         for (int lv = 0; lv < 4; lv++) {
             if (lv == 3) {
 #pragma unroll
                 for (int q = 0; q < 8; q++) dc[q] = dw[q];
             }
-            const int j0 = (G << lv) + (i << lv);
 #pragma unroll
             for (int q = (1 << lv) - 1; q >= 0; q--) {
-                const float y = icdf_u32_tab_scaled_neg(sobol_coord(sl, j0 + q), itab, sdb[lv]);
+                const float y = icdf_u32_tab_scaled_neg(sobol_coord(sl, slot0 + (1 << lv) - 1 + q), itab, sdb[lv]);
                 const float d = dw[q];
                 dw[2 * q] = fmaf(d, 0.5f, -y);
                 dw[2 * q + 1] = fmaf(d, 0.5f, y);
@@ -592,6 +614,13 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         P.K = C.K;
         P.B = C.B;
         P.discount = C.discount;
+        g.unit = 1.0f;
+        if constexpr (book_scaled<MK>) {
+            P.S0 *= P.inv_sref;
+            P.K *= P.inv_sref;
+            P.B *= P.inv_sref;
+            g.unit = 1.0f / P.inv_sref;
+        }
         const int D = g.nf;
         const int slot = C.slot[E.l], t_idx = C.t_idx;
         const uint32_t* grec = a.sdir[E.l] + ((size_t)slot * a.R + E.r) * (size_t)D * SDIR_WORDS;
@@ -612,10 +641,13 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         // Levels of at most 16 dates (dimensions) also keep the direction
         // numbers of bits 7..31 in LDS after the 16 records (hl[d][25]).
         uint32_t* hl = rec + BOOK_SHORT_D * 20;
+        // records in LDS slots: identity for levels of at most 16 dates,
+        // book_slot (block-contiguous) above
+        const int Gs = E.l > BOOK_SHORT_L ? g.G : D;
         if (lo != last_e) {
             for (int i = tid; i < D * 16; i += 256) {
                 const int d = i >> 4, k = i & 15;
-                rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
+                rec[book_slot(d, Gs) * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
             }
             if (E.l <= BOOK_SHORT_L)
                 for (int i = tid; i < D * 25; i += 256) hl[i] = grec[(int64_t)(i / 25) * SDIR_WORDS + 39 + i % 25];
@@ -628,7 +660,7 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
                 uint32_t s = 0;
 #pragma unroll
                 for (int j = 7; j < 32; j++) s ^= ((gbase >> j) & 1u) ? gr[32 + j] : 0u;
-                *(uint4*)(rec + d * 20 + 16) = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
+                *(uint4*)(rec + book_slot(d, Gs) * 20 + 16) = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
             }
         }
         __syncthreads();
@@ -651,7 +683,7 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
                     for (int d = tid; d < D; d += 256) {
                         const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
                         const uint32_t dl = gr[32 + 7] ^ gr[32 + 8 + c];
-                        uint4* r = (uint4*)(rec + d * 20 + 16);
+                        uint4* r = (uint4*)(rec + book_slot(d, Gs) * 20 + 16);
                         const uint4 v = *r;
                         *r = make_uint4(v.x ^ dl, v.y ^ dl, v.z ^ dl, v.w ^ dl);
                     }

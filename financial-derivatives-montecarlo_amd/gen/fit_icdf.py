@@ -6,10 +6,14 @@ g(q) = -Phi^{-1}(q) >= 0 (the sign comes from s).  g is read from a table of
 cubics indexed directly by q's FP32 encoding: one row per 1/16 of a binade of q,
 the row index being bits 19..31 of q's encoding (the biased exponent and the top
 4 mantissa bits) minus the first row's, the variable the remaining 19 mantissa
-bits as an integer.  q lies in [2^-33, 1/2] (2^-33 for s = 0; FP32 rounds q to
-exactly 1/2 for s within 64 of 2^31), biased exponents 94..126: 33 binades and
-528 rows, the last binade holding only q = 1/2, where g = 0.  No logarithm and
+bits under the exponent of 1.0, i.e. the float 1 + x 2^-23 in [1, 1 + 2^-4)
+(one AND-OR, no integer-to-float conversion).  q lies in [2^-33, 1/2] (2^-33
+for s = 0; FP32 rounds q to exactly 1/2 for s within 64 of 2^31), biased
+exponents 94..126: 33 binades and 528 rows, the last binade holding only
+q = 1/2, where g = 0.  No logarithm and
 no branch: a 16-byte LDS read, three FMAs and a multiply by the caller's scale.
+The cubic's variable is not centred, so its FP32 evaluation loses a little to
+cancellation: 2.1e-7 at most against the exact g (max(1, g)-weighted).
 
 The coefficients are NOT taken from any library: they are fitted here by
 weighted least squares per row against scipy.special.ndtri evaluated in FP64
@@ -25,15 +29,14 @@ from scipy.special import ndtri
 
 TAB_SUB = 4           # 2^4 segments per binade of q
 TAB_DEG = 3
-TAB_XBITS = 23 - TAB_SUB
 E_MIN, E_MAX = 94, 126  # biased FP32 exponents of q
 
 
 def fit_table():
     """Row r = (E - E_MIN) 16 + j covers q = 2^(E-127) (1 + (j + x 2^-19) / 16)
-    for the integer x in [0, 2^19); its cubic is fitted in X = x 2^-19 in [0, 1)
-    and stored scaled to x.  The binade of E_MAX = 126 holds only q = 1/2: its
-    rows are zero (g(1/2) = 0)."""
+    for the integer x in [0, 2^19); its cubic is fitted in the kernels' variable
+    V = 1 + x 2^-23 = 1 + X / 16, X = x 2^-19 in [0, 1).  The binade of
+    E_MAX = 126 holds only q = 1/2: its rows are zero (g(1/2) = 0)."""
     rows = (E_MAX - E_MIN + 1) << TAB_SUB
     tab = np.zeros((rows, TAB_DEG + 1))
     worst = 0.0
@@ -47,9 +50,10 @@ def fit_table():
         q = 2.0 ** (E - 127) * (1.0 + (j + X) / (1 << TAB_SUB))
         g = -ndtri(q)
         w = 1.0 / np.maximum(1.0, g)
-        a = np.polyfit(X, g, TAB_DEG, w=w)[::-1]                  # a_0 .. a_3 in X
-        worst = max(worst, np.max(np.abs(np.polyval(a[::-1], X) - g) * w))
-        tab[r] = a * 2.0 ** (-TAB_XBITS * np.arange(TAB_DEG + 1))   # in x = X 2^19
+        V = 1.0 + X / (1 << TAB_SUB)
+        a = np.polyfit(V, g, TAB_DEG, w=w)[::-1]                  # a_0 .. a_3 in V
+        worst = max(worst, np.max(np.abs(np.polyval(a[::-1], V) - g) * w))
+        tab[r] = a
     return tab, worst
 
 
@@ -60,7 +64,7 @@ def main():
         f.write("/* Generated by gen/fit_icdf.py -- do not edit.\n")
         f.write(" * Piecewise-cubic table of g = -Phi^-1(q), q in [2^-33, 1/2], in the bits of q:\n")
         f.write(" * row r = (bits(q) >> 19) - FDMC_ICDF_ROW0 holds {c0, c1, c2, c3} of\n")
-        f.write(" * g = c0 + x (c1 + x (c2 + x c3)), x = bits(q) & 0x7FFFF.\n")
+        f.write(" * g = c0 + v (c1 + v (c2 + v c3)), v = float(bits(q) & 0x7FFFF | bits(1.0f)).\n")
         f.write(" * max weighted fit error |g - p| / max(1, g) %.3e (FP64 evaluation). */\n" % err)
         f.write("#pragma once\n")
         f.write("#define FDMC_ICDF_TAB_ROWS %d\n" % len(tab))

@@ -1,8 +1,9 @@
 """The FP32 inverse normal CDF's generated table (csrc/icdf_coeffs.h), evaluated
 on the CPU the way icdf_u32_tab_scaled_neg (csrc/device_math.hpp) evaluates it:
 q = |(float)(int32)s 2^-32 + 2^-33| (the fold to the lower half),
-row = (bits(q) >> 19) - FDMC_ICDF_ROW0, x = bits(q) & 0x7FFFF, FP32 Horner
-steps, the sign taken from s.  numpy rounds each multiply and add where the
+row = (bits(q) >> 19) - FDMC_ICDF_ROW0, the variable v = 1 + (bits(q) & 0x7FFFF)
+2^-23 (the mantissa bits under the exponent of 1.0), FP32 Horner steps, the sign
+taken from s.  numpy rounds each multiply and add where the
 device fuses them, so the bound checked here is looser than the device's error
 and tighter than the GPU test's 4e-7."""
 import os
@@ -36,10 +37,10 @@ def _icdf(s, tab):
     q = _fold(s)
     b = q.view(np.uint32)
     c = tab[(b >> 19) - ROW0]
-    x = (b & 0x7FFFF).astype(np.float32)
+    v = ((b & 0x7FFFF) | np.uint32(0x3F800000)).view(np.float32)
     p = c[:, 3]
     for k in (2, 1, 0):
-        p = (p * x + c[:, k]).astype(np.float32)
+        p = (p * v + c[:, k]).astype(np.float32)
     neg = (p.view(np.uint32) | (s & np.uint32(0x80000000))).view(np.float32)   # -Phi^-1
     return -neg.astype(np.float64)
 
@@ -57,7 +58,7 @@ def test_fp32_evaluation_error():
     got = _icdf(s, tab)
     ref = ndtri((s.astype(np.float64) + 0.5) / 2.0 ** 32)
     err = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
-    assert err.max() < 2.5e-7, (err.max(), s[np.argmax(err)])
+    assert err.max() < 3e-7, (err.max(), s[np.argmax(err)])
 
 
 def test_rows_reached_are_fitted():
