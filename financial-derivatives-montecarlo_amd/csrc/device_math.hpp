@@ -135,20 +135,27 @@ __device__ __forceinline__ void block_sum(double (&v)[NA], double* lds) {
 }
 
 // ---- steppers (methods_cuda.cu:271-291, 388-406) ---------------------------
-// One-factor Euler step at time t with step h and increment dw.
+// One-factor Euler step at time t with step h and increment dw,
+// S + mu S h + sigma_loc S dw (methods_cuda.cu:271-291) with mu h hoisted out
+// of the loops (the term-by-term form takes four operations: without
+// reassociation nothing of mu S h can be hoisted): local vols as
+// S + S (mu h + sigma_loc dw), two FMAs; GBM as (sigma S) dw + (S + S mu h),
+// three (its two-FMA chain with a constant sigma sends the compiler into a very
+// long optimisation of the unrolled paths, and S (1 + mu h + sigma dw) would
+// round mu h against 1 on every step: a drift bias of order 1e-3 relative).
 template <int MK>
 __device__ __forceinline__ float step_1f(const LegParams& P, float S, float dw, float t, float h) {
+    float sl;
     if constexpr (MK == MK_GBM) {
-        return S + P.mu * S * h + P.sigma * S * dw;
+        return fmaf(P.sigma * S, dw, fmaf(S, P.mu * h, S));
     } else if constexpr (MK == MK_DUPIRE) {
         // sigma0 exp(-alpha t) (S/S0)^(beta-1)
-        const float sl = P.sigma0 * fast_exp(-P.alpha * t) * fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));
-        return S + P.mu * S * h + sl * S * dw;
+        sl = P.sigma0 * fast_exp(-P.alpha * t) * fast_exp2(P.beta_m1 * fast_log2(S * P.inv_S0));
     } else {  // MK_LVQ: a + b exp(-c t) ln(S/s_ref)^2
         const float l = fast_ln(S * P.inv_sref);
-        const float sl = P.lv_a + P.lv_b * fast_exp(-P.lv_c * t) * l * l;
-        return S + P.mu * S * h + sl * S * dw;
+        sl = P.lv_a + P.lv_b * fast_exp(-P.lv_c * t) * l * l;
     }
+    return fmaf(S, fmaf(sl, dw, P.mu * h), S);
 }
 
 // Full-truncation exponential-Euler Heston step (methods_cuda.cu:278-284) with

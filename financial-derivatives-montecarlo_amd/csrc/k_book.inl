@@ -55,7 +55,7 @@ __device__ __forceinline__ float book_sigma(const LegParams& P, float S, float v
         return P.sigma0 * v * aux;
     } else {
         aux = fast_log2(S);                                       // log2(S/s_ref): S in units of s_ref
-        return fmaf(v, aux * aux, P.lv_a);
+        return fmaf(v * aux, aux, P.lv_a);                        // v aux is shared with book_partials
     }
 }
 
@@ -73,7 +73,8 @@ __device__ __forceinline__ float book_euler(const LegParams& P, float S, float d
 }
 
 // Partial derivatives of book_euler: fS = dS'/dS, fvol = dS'/d(sigma, sigma0
-// or a), fmu = dS'/dmu, fS0 = dS'/dS0 through the Dupire normalisation.
+// or a), fmu = dS'/dmu, fS0 = dS'/dS0 through the Dupire normalisation (the only
+// model whose local vol depends on S0; book_s0_tangent).
 template <int MK>
 __device__ __forceinline__ void book_partials(const LegParams& P, float S, float dw, float v, float h, float muh,
                                               float& fS, float& fvol, float& fmu, float& fS0) {
@@ -96,6 +97,11 @@ __device__ __forceinline__ void book_partials(const LegParams& P, float S, float
     }
 }
 
+// Models whose step depends on S0 other than through S (Dupire's (S/S0)^(beta-1)):
+// only they carry the fS0 term in the tangents.
+template <int MK>
+constexpr bool book_s0_tangent = MK == MK_DUPIRE;
+
 // One Euler step S -> S' and, with TAN, the tangent update
 // t' = fS t + (f0, fv, fm) (step_partials are the step's partial derivatives).
 template <int MK, bool TAN>
@@ -104,7 +110,7 @@ __device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t,
     if constexpr (TAN) {
         float fS, fv, fm, f0;
         book_partials<MK>(P, S, dw, v, h, muh, fS, fv, fm, f0);
-        t.s = fS * t.s + f0;
+        t.s = book_s0_tangent<MK> ? fmaf(fS, t.s, f0) : fS * t.s;
         t.v = fS * t.v + fv;
         t.m = fS * t.m + fm;
     }
@@ -221,13 +227,13 @@ __device__ __forceinline__ void book_block_adj(const LegParams& P, float S, cons
             jS = fS * jS;
             jv = fS * jv + fv;
             jm = fS * jm + fm;
-            j0 = fS * j0 + f0;
+            if constexpr (book_s0_tangent<MK>) j0 = fS * j0 + f0;
             if constexpr (DIRECT) {
                 const float d = (PK == PK_ASIAN) ? each : (gi + 1 == kmin ? cmin : 0.0f);
                 dS += d * jS;
                 dv += d * jv;
                 dm += d * jm;
-                d0 += d * j0;
+                if constexpr (book_s0_tangent<MK>) d0 += d * j0;
             }
         }
     }

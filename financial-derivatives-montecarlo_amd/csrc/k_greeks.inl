@@ -139,6 +139,8 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                 if constexpr (NZ == NZ_SOBOL_BB) {
                     if constexpr (B == 1) {
                         dw[0] = sk[(i + 1) * 256] - sk[i * 256];
+                    } else if constexpr (UNI) {
+                        bb_block_uniform<true>(a, sl, itab, G, lgG, i, sk[i * 256], sk[(i + 1) * 256], dw);
                     } else {
                         float W[B + 1];
                         W[0] = sk[i * 256];

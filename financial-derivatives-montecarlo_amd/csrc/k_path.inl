@@ -114,6 +114,29 @@ __device__ __forceinline__ float bb_terminal(const PathLaunch& a, const SobolLan
     else return ((const float4*)a.bbw)[0].z * icdf_u32_tab(bb_coord<LDS>(sl, 0), itab);
 }
 
+// The 16 increments of block i of a uniform-grid bridge (weights 1/2, one sd
+// per depth) from W at the block's ends: each interval's increment d splits at
+// its midpoint into d/2 - y and d/2 + y (y = -sd z of the node), which is
+// W_M = (W_L + W_R)/2 + sd z written for the increments (no W inside the block,
+// no differences).  Node j0 + q at in-block depth l, j0 = (G + i) 2^l.
+template <bool LDS>
+__device__ __forceinline__ void bb_block_uniform(const PathLaunch& a, const SobolLane& sl, const float4* itab, int G,
+                                                 int lgG, int i, float WL, float WR, float (&dw)[16]) {
+    dw[0] = WR - WL;
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        const int j0 = (G << l) + (i << l);
+        const float sd = a.bb_sd[lgG + l + 1];
+#pragma unroll
+        for (int q = (1 << l) - 1; q >= 0; q--) {
+            const float y = icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j0 + q), itab, sd);
+            const float d = dw[q];
+            dw[2 * q] = fmaf(d, 0.5f, -y);
+            dw[2 * q + 1] = fmaf(d, 0.5f, y);
+        }
+    }
+}
+
 template <int MK, int PK>
 struct Leg {
     float S, V, run;
@@ -331,21 +354,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                             rt[c * 256] = rt[(c + 1) * 256];
                             for (int k = c - 1; k >= 0; k--) rt[k * 256] = node(k, i, Wp, rt[(k + 1) * 256]);
                         }
-                        float W[B + 1];
-                        W[0] = Wp;
-                        W[B] = rt[0];
+                        if constexpr (UNI && B == 16) {
+                            float dw[16];
+                            bb_block_uniform<LDS>(a, sl, itab, G, lgG, i, Wp, rt[0], dw);
 #pragma unroll
-                        for (int l = 0; l < LB; l++) {
-                            const int j0 = (G << l) + (i << l);
+                            for (int k = 0; k < B; k++) ps.step(a, dw[k], i * B + k);
+                        } else {
+                            float W[B + 1];
+                            W[0] = Wp;
+                            W[B] = rt[0];
 #pragma unroll
-                            for (int q = 0; q < (1 << l); q++) {
-                                const int s = B >> l;
-                                const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
-                                W[Mp] = mid(W[Lp], W[Rp], j0 + q, lgG + l + 1);
+                            for (int l = 0; l < LB; l++) {
+                                const int j0 = (G << l) + (i << l);
+#pragma unroll
+                                for (int q = 0; q < (1 << l); q++) {
+                                    const int s = B >> l;
+                                    const int Lp = q * s, Rp = Lp + s, Mp = Lp + (s >> 1);
+                                    W[Mp] = mid(W[Lp], W[Rp], j0 + q, lgG + l + 1);
+                                }
                             }
-                        }
 #pragma unroll
-                        for (int k = 0; k < B; k++) ps.step(a, W[k + 1] - W[k], i * B + k);
+                            for (int k = 0; k < B; k++) ps.step(a, W[k + 1] - W[k], i * B + k);
+                        }
                     }
                 }
             };
