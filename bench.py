@@ -42,6 +42,9 @@ sys.path.insert(0, os.path.join(ROOT, "financial-derivatives-montecarlo_amd"))
 BOOK_FWD_FLOPS = 36
 BOOK_REV_FLOPS = 44
 C2_FLOPS_PER_PATH_STEP = 31     # C2 forward: Euler 3 + BB 6 + Asian 1 + geo 1 + ICDF 20
+C2_ADJ_FLOPS_PER_PATH_STEP = 39  # C2 reverse sweep: reverse Euler/Asian 8 + forward recomputed 31
+C1_FLOPS_PER_PATH_STEP = 23     # C1: Euler 3 + normal generation 20 (Philox integer ops excluded)
+C4_FLOPS_PER_PATH_STEP = 63     # C4 per fine step: fine 18 + coarse share 5 + 2 normals x 20
 VALU_PEAK_TFLOPS = 157.3        # MI355X FP32 vector peak (packed)
 MFMA_PEAK_TFLOPS = 2500.0       # MI355X dense FP16/BF16 matrix peak
 # transcendentals (exp, log, sqrt, rcp, sin, cos) issue at a quarter of the FP32
@@ -54,32 +57,44 @@ BOOK_TRANS_PER_PATH_STEP = 2
 METRIC = "time-to-eps=1e-3 price+Greeks, batched vs plain (s); path-steps/s vs VALU peak"
 EPS = 1e-3
 BOOK_KERNEL = "k_book<LOCALVOL_QUAD, adjoint Greeks>"
-# HBM bytes per launch of the book kernel on this workload: FETCH_SIZE and
-# WRITE_SIZE of two rocprofv3 --pmc runs of `bench.py --roofline-only`, scaled by
-# the same counters on a 512 MiB device-to-device copy (reads x2.0, writes x1.0);
-# a counter run cannot be part of the timed process, so the figure is recorded.
-BOOK_TRAFFIC_BYTES_PER_LAUNCH = 1.892e11
+# HBM bytes of one run of the book kernel on this workload (6 launches):
+# FETCH_SIZE and WRITE_SIZE of two rocprofv3 --pmc runs of `bench.py
+# --roofline-only`, scaled by the same counters on a 512 MiB device-to-device
+# copy (reads x2.0, writes x1.0); a counter run cannot be part of the timed
+# process, so the figures are recorded.  The launches differ by 10^5x in size:
+# the per-run total is the meaningful figure, `traffic` is its per-launch mean.
+BOOK_TRAFFIC_READ_BYTES_PER_RUN = 2.673e10
+BOOK_TRAFFIC_WRITE_BYTES_PER_RUN = 3.434e10
+BOOK_TRAFFIC_LAUNCHES_PER_RUN = 6
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(budget_s=15.0, eps=1e-2):
+def cpu_baseline(res=None, n_sample=16, seed=16):
     """Oracle (FP32 paths, FP64 inverse CDF and sums, forward-tangent Greeks) on
-    all host threads: the allocation loops of the book's contracts 0, 1, 2, ...
-    at eps, until budget_s has been spent."""
+    all host threads: the allocation loops of a seeded sample of the book's
+    contracts at eps = 1e-3, timed to eps.  The sample is drawn among the
+    contracts other than the up-and-out barriers: one barrier contract alone
+    takes minutes on the host (6.4e9 path-steps for contract 0).  With the GPU
+    run's per-contract results `res`, the host time to eps of the whole book is
+    extrapolated by path-steps: sample time x (book path-steps / sample
+    path-steps), both counted by the GPU run on the same allocation loop."""
+    import numpy as np
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import fdmc
     import pyoracle as O
 
     model, book = fdmc.local_vol_book(1024, 2026)
     cfg = fdmc.BookConfig(R=16, greeks=True)
+    pool = [i for i, k in enumerate(book) if k.kind != fdmc.PAYOFF_BARRIER]
+    idx = sorted(np.random.default_rng(seed).choice(pool, n_sample, replace=False).tolist())
     t = time.perf_counter()
-    ps, n = 0, 0
-    while n < len(book) and time.perf_counter() - t < budget_s:
-        ps += O.book_contract_run(model, book[n], n, eps, cfg).path_steps
-        n += 1
+    ps = 0
+    for i in idx:
+        ps += O.book_contract_run(model, book[i], i, EPS, cfg).path_steps
     dt = time.perf_counter() - t
     cpu = ""
     try:
@@ -87,10 +102,19 @@ def cpu_baseline(budget_s=15.0, eps=1e-2):
             cpu = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": ps / dt, "unit": "path-steps/s", "cores": O.num_threads(), "kind": "port", "cpu": cpu,
-            "host_cpus": os.cpu_count(),
-            "sample": "contracts 0..%d of the book at eps=%g (price + Greeks), %.3g path-steps in %.1f s"
-                      % (n - 1, eps, ps, dt)}
+    out = {"value": ps / dt, "unit": "path-steps/s", "cores": O.num_threads(), "kind": "port", "cpu": cpu,
+           "host_cpus": os.cpu_count(), "sample_time_to_eps_s": dt, "sample_path_steps": ps,
+           "sample": "time to eps=%g (price + Greeks) of %d contracts %s drawn with seed %d from the book's "
+                     "%d non-barrier contracts: %.3g path-steps in %.1f s"
+                     % (EPS, n_sample, idx, seed, len(pool), ps, dt)}
+    if res is not None:
+        gpu_all = sum(r.path_steps for r in res)
+        gpu_sub = sum(res[i].path_steps for i in idx)
+        out["book_time_to_eps_s_extrapolated"] = dt * gpu_all / gpu_sub
+        out["extrapolation"] = ("sample time x %.4g book path-steps / %.4g sample path-steps (GPU run's counts); "
+                                "assumes the barrier contracts' host cost per path-step equals the sample's"
+                                % (gpu_all, gpu_sub))
+    return out
 
 
 def c2_asian(eng, steps=5):
@@ -108,7 +132,7 @@ def c2_asian(eng, steps=5):
         return eng.qmc_accumulate(m, asian, n_steps, R, 42, 0, N, fdmc.NOISE_BB, dev_bb=bb,
                                   ctrl_model=m, ctrl_payoff=geo, E_ctrl=E, beta=1.0)
 
-    for _ in range(3):      # the clocks settle over the first passes (5.7 -> 4.7 ms)
+    for _ in range(10):     # the clocks settle over the first passes (5.3 -> 4.2 ms over eight)
         step()
     eng.synchronize()
     eng.timing_enable(True)
@@ -129,11 +153,17 @@ def c2_asian(eng, steps=5):
         r = eng.run_qmc_cv_cuda(m, m, asian, geo, E, 1.0, EPS, n_steps, cfg, fdmc.NOISE_BB, dev_bb=bb)
         tte.append(r.time_s)
     dE = fdmc.geom_asian_greeks(100, 100, 1, 0.05, 0.3, n_steps)
+
+    def greeks_run():
+        return eng.run_qmc_greeks(m, asian, EPS, n_steps, cfg, fdmc.NOISE_BB, bb, ctrl_model=m, ctrl_payoff=geo,
+                                  E_ctrl=E, dE_ctrl=dE, beta=1.0)
+
     ttg = []
     for _ in range(4):
-        rg, g = eng.run_qmc_greeks(m, asian, EPS, n_steps, cfg, fdmc.NOISE_BB, bb, ctrl_model=m, ctrl_payoff=geo,
-                                   E_ctrl=E, dE_ctrl=dE, beta=1.0)
+        rg, g = greeks_run()
         ttg.append(rg.time_s)
+    rl_adj = timed_roofline(eng, greeks_run, C2_FLOPS_PER_PATH_STEP + C2_ADJ_FLOPS_PER_PATH_STEP,
+                            "k_greeks<GBM,Asian,CV_SHARED,GeomAsian,SOBOL_BB,LB=4> (forward + reverse sweep)")
     bb.free()
     return {"workload": "GBM S0=K=100 r=0.05 sigma=0.3 T=1, D=256 bridge, R=32 x 2^20 points, geometric CV",
             "path_steps_per_s": path_steps * steps / wall, "ms_per_pass": 1e3 * wall / steps,
@@ -144,7 +174,8 @@ def c2_asian(eng, steps=5):
                          "kernel_avg_ms": k_avg_ms, "flops_per_path_step": C2_FLOPS_PER_PATH_STEP},
             "time_to_eps": {"eps": EPS, "price_s": sorted(tte[1:])[1], "price": r.price, "std_error": r.std_error,
                             "price_and_greeks_s": sorted(ttg[1:])[1],
-                            "greeks": {"delta": g.delta, "vega": g.vega, "rho": g.rho}}}
+                            "greeks": {"delta": g.delta, "vega": g.vega, "rho": g.rho},
+                            "greeks_roofline": rl_adj}}
 
 
 def c3_gemm(eng, reps=5):
@@ -220,6 +251,24 @@ def gemm_error(eng, pca, M, bf16, n=4096):
     return float(np.linalg.norm(dw.cpu().numpy() - ref) / np.linalg.norm(ref))
 
 
+def timed_roofline(eng, run, flops_per_path_step, kernel):
+    """One run with HIP events around every launch of the path kernel on the
+    engine stream: algorithmic flops of the run's path-steps / kernel time."""
+    eng.synchronize()
+    eng.timing_enable(True)
+    eng.timing_read(reset=True)
+    r = run()
+    launches, kms = eng.timing_read(reset=True)
+    eng.timing_enable(False)
+    if not hasattr(r, "path_steps"):      # (result, greeks)
+        r = r[0]
+    achieved = r.path_steps * flops_per_path_step / (kms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / VALU_PEAK_TFLOPS, "kernel": kernel, "kernel_launches": launches,
+            "kernel_total_ms": kms, "path_steps": r.path_steps, "flops_per_path_step": flops_per_path_step,
+            "path_steps_per_s_in_kernel": r.path_steps / (kms * 1e-3)}
+
+
 def c1_mc(eng):
     """Config 1: plain MC European call under GBM, 64 Euler steps, pilot 1e4 (seed
     123), N = ceil(2 Var / eps^2) paths at eps = 1e-2, against Black-Scholes."""
@@ -233,8 +282,11 @@ def c1_mc(eng):
         ts.append(r.time_s)
     t = sorted(ts)[1]
     bs = fdmc.bs_call(100, 100, 1, 0.05, 0.2)
+    rl = timed_roofline(eng, lambda: eng.run_mc_cuda(g, e, 1e-2, 64, fdmc.MCConfig()), C1_FLOPS_PER_PATH_STEP,
+                        "k_path<GBM,European,PHILOX>")
     return {"eps": 1e-2, "time_to_eps_s": t, "price": r.price, "std_error": r.std_error, "n_paths": r.n_samples,
-            "path_steps_per_s": r.path_steps / t, "black_scholes": bs, "err_vs_black_scholes": r.price - bs}
+            "path_steps_per_s": r.path_steps / t, "black_scholes": bs, "err_vs_black_scholes": r.price - bs,
+            "roofline": rl}
 
 
 def c4_heston_mlmc(eng):
@@ -258,13 +310,25 @@ def c4_heston_mlmc(eng):
     # decay too slowly for the bias test to fire by L = 8, so the run ends at the
     # level cap with the not-converged flag (variance target met on every level)
     bar = fdmc.Barrier(100.0, 130.0, 0.2, 0.05, 1.0)
+    rl = timed_roofline(eng, lambda: eng.run_mlmc_cuda(hes, fdmc.European(100, 0.05, 1), 5e-4, cfg),
+                        C4_FLOPS_PER_PATH_STEP, "k_levels<HESTON,European,PHILOX>")
+    eng.timing_enable(True)
+    eng.timing_read(reset=True)
     rb = eng.run_mlmc_cuda(hes, bar, 5e-4, cfg)
+    b_launches, b_kms = eng.timing_read(reset=True)
+    eng.timing_enable(False)
+    b_ach = rb.path_steps * C4_FLOPS_PER_PATH_STEP / (b_kms * 1e-3) / 1e12
     return {"eps": 5e-4, "time_to_eps_s": sorted(ts)[1], "price": r.price, "std_error": r.std_error,
             "levels": r.levels, "path_steps": r.path_steps, "path_steps_per_s": r.path_steps / sorted(ts)[1],
             "closed_form": cf, "err_vs_closed_form": r.price - cf,
             "barrier": {"time_s": rb.time_s, "price": rb.price, "std_error": rb.std_error, "levels": rb.levels,
                         "flags": rb.flags, "converged": not (rb.flags & fdmc.FLAG_NOT_CONVERGED),
-                        "path_steps": rb.path_steps, "path_steps_per_s": rb.path_steps / rb.time_s}}
+                        "path_steps": rb.path_steps, "path_steps_per_s": rb.path_steps / rb.time_s,
+                        "roofline": {"bound": "valu", "achieved": b_ach, "peak": VALU_PEAK_TFLOPS,
+                                     "unit": "TFLOP/s", "frac": b_ach / VALU_PEAK_TFLOPS,
+                                     "kernel": "k_levels<HESTON,Barrier,PHILOX>", "kernel_launches": b_launches,
+                                     "kernel_total_ms": b_kms, "flops_per_path_step": C4_FLOPS_PER_PATH_STEP}},
+            "roofline": rl}
 
 
 def book_flops(contracts, res):
@@ -288,7 +352,12 @@ def book_roofline(eng, book, contracts):
     flops = book_flops(contracts, res)
     achieved = flops / (kms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / VALU_PEAK_TFLOPS, "traffic": BOOK_TRAFFIC_BYTES_PER_LAUNCH, "kernel": BOOK_KERNEL,
+            "frac": achieved / VALU_PEAK_TFLOPS,
+            "traffic": (BOOK_TRAFFIC_READ_BYTES_PER_RUN + BOOK_TRAFFIC_WRITE_BYTES_PER_RUN)
+                       / BOOK_TRAFFIC_LAUNCHES_PER_RUN,
+            "traffic_per_run": {"read": BOOK_TRAFFIC_READ_BYTES_PER_RUN, "write": BOOK_TRAFFIC_WRITE_BYTES_PER_RUN,
+                                "launches": BOOK_TRAFFIC_LAUNCHES_PER_RUN, "unit": "B"},
+            "kernel": BOOK_KERNEL,
             "kernel_avg_ms": k_avg_ms, "kernel_launches": launches, "kernel_total_ms": kms,
             "path_steps": st.path_steps, "flops": flops, "flops_per_path_step": flops / st.path_steps,
             "flops_forward_reverse": [BOOK_FWD_FLOPS, BOOK_REV_FLOPS],
@@ -424,7 +493,7 @@ def main():
     eng.close()
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(res)
     print(json.dumps(out))
 
 
