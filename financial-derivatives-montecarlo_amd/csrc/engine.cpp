@@ -498,12 +498,12 @@ int run_basket_wide(fdmc_context* c, const PassSpec& s, std::vector<double>& out
         for (int r = 0; r < s.R; r++) salts[r] = s.salt_base + (uint32_t)r;
         b.sdir = get_sdir(c, salts, b.D);
     }
-    const void* fn = basket_wide_kernel(mk, src, corr);
+    const void* fn = basket_wide_kernel(mk, src, corr, n);
     if (!fn) fail(FDMC_E_UNSUPPORTED, "wide basket kernel not available");
-    const int wg = basket_wide_wg(corr);
-    const int64_t t_first = s.first / wg, t_last = (s.first + s.count - 1) / wg;
+    const int wg = basket_wide_block(corr), tile = basket_wide_tile(corr);
+    const int64_t t_first = s.first / tile, t_last = (s.first + s.count - 1) / tile;
     const int64_t ntiles = t_last - t_first + 1;
-    const int64_t per_rep = std::max<int64_t>(1, (int64_t)c->n_cu * (corr ? 8 : 16) / s.R);
+    const int64_t per_rep = std::max<int64_t>(1, (int64_t)c->n_cu * (corr ? 4 : 16) / s.R);
     const int64_t tpw = std::max<int64_t>(1, (ntiles + per_rep - 1) / per_rep);
     const int64_t nwg = (ntiles + tpw - 1) / tpw;
     if (nwg > 0x7FFFFFFF || tpw > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many workgroups");

@@ -45,8 +45,9 @@ hipError_t launch_is_gbm(const ISLaunch& a, int nwg, hipStream_t st);
 // k_basket.hip
 const void* basket_kernel(int mk, int src, int na);
 size_t basket_lds_bytes(int src, int D, int na);
-const void* basket_wide_kernel(int mk, int src, bool corr);
-int basket_wide_wg(bool corr);
+const void* basket_wide_kernel(int mk, int src, bool corr, int n);
+int basket_wide_block(bool corr);   // lanes per workgroup
+int basket_wide_tile(bool corr);    // paths per tile
 size_t basket_wide_lds_bytes(int n, bool corr);
 
 // k_book.hip

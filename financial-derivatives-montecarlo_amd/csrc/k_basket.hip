@@ -160,9 +160,8 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
 // L2/MALL resident) or Philox normal j of the path.
 //  - uncorrelated (128-lane workgroups): assets one after the other -- an asset's
 //    path needs only its own increments, so a lane carries only the basket sum;
-//  - correlated (64-lane workgroups, n <= WIDE_CORR_MAX): dates one after the
-//    other; the date's n scaled normals and the lane's n asset prices sit in LDS
-//    ([n][64] floats each), dW = L z row by row with L read uniformly.
+//  - correlated (n <= WIDE_CORR_MAX): k_basket_corr below, dates one after the
+//    other with dW = Z L^T on FP32 MFMA.
 template <int MK>
 __device__ __forceinline__ float wide_step(const BasketWide& b, float S, float sg, float dw, int k) {
     if constexpr (MK == MK_MULTI_GBM) {
@@ -174,10 +173,9 @@ __device__ __forceinline__ float wide_step(const BasketWide& b, float S, float s
     }
 }
 
-template <int MK, int SRC, bool CORR>
-__global__ __launch_bounds__(CORR ? 64 : WIDE_WG) void k_basket_wide(BasketWide b) {
-    constexpr int WG = CORR ? 64 : WIDE_WG;
-    extern __shared__ __align__(16) unsigned char smem[];
+template <int MK, int SRC>
+__global__ __launch_bounds__(WIDE_WG) void k_basket_wide(BasketWide b) {
+    constexpr int WG = WIDE_WG;
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = blockIdx.y;
     const int n = b.n, D = b.D;
@@ -185,8 +183,6 @@ __global__ __launch_bounds__(CORR ? 64 : WIDE_WG) void k_basket_wide(BasketWide 
     __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
     if constexpr (ITAB) icdf_tab_load(itab, tid, WG);
     __shared__ double red[(WG / 64) * 2];
-    float* zs = (float*)smem;                       // CORR: [n][WG] scaled normals of the date
-    float* Ss = zs + (size_t)n * WG;                // CORR: [n][WG] asset prices
     __syncthreads();
     SobolLane sl;
     {
@@ -209,49 +205,22 @@ __global__ __launch_bounds__(CORR ? 64 : WIDE_WG) void k_basket_wide(BasketWide 
             sl.hi_gray = hi ^ (hi >> 1);
         }
         float B = 0.0f;
-        if constexpr (!CORR) {
-            for (int a = 0; a < n; a++) {
-                float S = b.S0[a];
-                const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
-                for (int k = 0; k < b.n_steps; k++) {
-                    const int j = k * n + a;
-                    float z;
-                    if constexpr (SRC == SRC_SOBOL) {
-                        z = icdf_u32_tab(sobol_coord(sl, j), itab);
-                    } else {
-                        float q[4];
-                        prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j >> 2), q);
-                        z = q[j & 3];
-                    }
-                    S = wide_step<MK>(b, S, sg, z * b.sqrt_h, k);
-                }
-                B += b.w[a] * S;
-            }
-        } else {
-            for (int a = 0; a < n; a++) Ss[a * WG + tid] = b.S0[a];
+        for (int a = 0; a < n; a++) {
+            float S = b.S0[a];
+            const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
             for (int k = 0; k < b.n_steps; k++) {
-                float q[4];
-                for (int c = 0; c < n; c++) {
-                    const int j = k * n + c;
-                    float z;
-                    if constexpr (SRC == SRC_SOBOL) {
-                        z = icdf_u32_tab(sobol_coord(sl, j), itab);
-                    } else {
-                        if (c == 0 || (j & 3) == 0)
-                            prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j >> 2), q);
-                        z = q[j & 3];
-                    }
-                    zs[c * WG + tid] = z * b.sqrt_h;
+                const int j = k * n + a;
+                float z;
+                if constexpr (SRC == SRC_SOBOL) {
+                    z = icdf_u32_tab(sobol_coord(sl, j), itab);
+                } else {
+                    float q[4];
+                    prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j >> 2), q);
+                    z = q[j & 3];
                 }
-                for (int a = 0; a < n; a++) {
-                    const float* La = b.L + (int64_t)a * n;
-                    float dw = 0.0f;
-                    for (int c = 0; c <= a; c++) dw += La[c] * zs[c * WG + tid];
-                    const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
-                    Ss[a * WG + tid] = wide_step<MK>(b, Ss[a * WG + tid], sg, dw, k);
-                }
+                S = wide_step<MK>(b, S, sg, z * b.sqrt_h, k);
             }
-            for (int a = 0; a < n; a++) B += b.w[a] * Ss[a * WG + tid];
+            B += b.w[a] * S;
         }
         const float y = fmaxf(B - b.K, 0.0f) * b.discount;
         if (valid) {
@@ -267,17 +236,209 @@ __global__ __launch_bounds__(CORR ? 64 : WIDE_WG) void k_basket_wide(BasketWide 
     }
 }
 
-const void* basket_wide_kernel(int mk, int src, bool corr) {
+// ---- correlated baskets of more than BASKET_MAX assets on FP32 MFMA ---------
+// The reference correlates a date's normals with a batched GEMM dW = Z L^T
+// (correlate_multi_dupire, methods_cuda.cu:1133-1148).  Here each wave takes 16
+// paths and, per date, forms their increments with v_mfma_f32_16x16x4_f32 (FP32
+// operands, FP32 accumulation): A = the paths' scaled normals, B = L^T, in
+// asset blocks of 16.  The paths' asset prices stay in the accumulator layout
+// (lane l: asset 16 j + l % 16 of paths 4 (l / 16) + i, i = 0..3), so the Euler
+// step is elementwise on the MFMA output and nothing per path touches memory.
+// Only the block pairs (q, j) with q <= j carry work (L is lower triangular);
+// the B fragments of all pairs sit in LDS as one float4 per lane (the four k
+// steps of one 16-asset block).
+//
+// The A operand of one MFMA holds, in lane l, row (path) l % 16 and k slot
+// l / 16.  The k order inside an asset block q is permuted so that a lane's
+// four k steps t = 0..3 are the consecutive assets 16 q + 4 (l / 16) + t: one
+// Philox call (four normals of consecutive index) or four consecutive Sobol
+// dimensions per lane and block.  B follows the same order.
+constexpr int CORR_TILE = 64;      // paths per tile: 4 waves x 16
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int corr_pair(int q, int j, int NB) { return q * NB - ((q * (q - 1)) >> 1) + (j - q); }
+
+template <int MK, int SRC, int NB>
+__global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float4* lt = (float4*)smem;                      // [pair][64 lanes]: L^T B fragments for t = 0..3
+    constexpr bool ITAB = SRC == SRC_SOBOL;
+    __shared__ __align__(16) float4 itab[ITAB ? FDMC_ICDF_TAB_ROWS : 1];
+    if constexpr (ITAB) icdf_tab_load(itab, threadIdx.x, 256);
+    __shared__ double red[4 * 2];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int col = lane & 15, ks = lane >> 4;
+    const int r = blockIdx.y;
+    const int n = b.n, nb = (n + 15) >> 4;           // nb <= NB (checked on the host)
+    // B fragments: lane (ks, col) of pair (q, j) holds L[16 j + col][16 q + 4 ks + t]
+    // for t = 0..3, zero above the diagonal and outside n
+    constexpr int NPAIRS = NB * (NB + 1) / 2;
+    for (int e = tid; e < NPAIRS * 64; e += 256) {
+        int p = e >> 6, q = 0;
+        while (p >= NB - q) { p -= NB - q; q++; }
+        const int j = q + p, ln = e & 63;
+        const int arow = 16 * j + (ln & 15), c0 = 16 * q + 4 * (ln >> 4);
+        float v[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int c = c0 + t;
+            v[t] = (arow < n && c <= arow) ? b.L[(int64_t)arow * n + c] : 0.0f;
+        }
+        lt[e] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    __syncthreads();
+    const uint32_t* grec = (SRC == SRC_SOBOL) ? b.sdir + (int64_t)r * b.D * SDIR_WORDS : nullptr;
+    const float muh = b.mu * b.h;
+    double acc[2] = {0.0, 0.0};
+    const int64_t last = b.first + b.count;
+    const int64_t tile0 = (b.first / CORR_TILE) + (int64_t)blockIdx.x * b.tiles_per_wg;
+    for (int it = 0; it < b.tiles_per_wg; it++) {
+        const int64_t base = (tile0 + it) * CORR_TILE;
+        if (base >= last) break;                     // the same for every lane
+        const int64_t wbase = base + 16 * wv;
+        // the A-layout path of this lane: row col
+        const uint64_t pa = (uint64_t)(wbase + col);
+        SobolLane sl;
+        {
+            const uint32_t x = (uint32_t)pa, g = x ^ (x >> 1);
+            sl.rec = nullptr; sl.grec = grec; sl.ga = g & 7u; sl.gb = (g >> 3) & 7u; sl.w = 0;
+            sl.hi_gray = g & ~63u; sl.lds_mode = false;
+        }
+        f32x4 S[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const int a = 16 * j + col;
+            const float s0 = a < n ? b.S0[a] : 1.0f;      // outside n: a finite dummy with weight 0
+            S[j] = f32x4{s0, s0, s0, s0};
+        }
+        for (int k = 0; k < b.n_steps; k++) {
+            f32x4 dw[NB];
+#pragma unroll
+            for (int j = 0; j < NB; j++) dw[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            const int jk = k * n;
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                if (q >= nb) break;                  // uniform
+                const int c0 = 16 * q + 4 * ks;      // this lane's four assets c0 .. c0 + 3
+                float za[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (c0 < n) {
+                    if constexpr (SRC == SRC_SOBOL) {
+#pragma unroll
+                        for (int t = 0; t < 4; t++)
+                            if (c0 + t < n) za[t] = icdf_u32_tab(sobol_coord_glb(sl, jk + c0 + t), itab) * b.sqrt_h;
+                    } else {
+                        // normals jk + c0 + t of the path, in groups of four from
+                        // index 4 g (the offset jk & 3 is the same for every lane)
+                        const int j0 = jk + c0, off = j0 & 3;
+                        float g0[4], g1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                        prng_normals4(b.seed + (uint32_t)r, b.tag, pa, (uint32_t)(j0 >> 2), g0);
+                        if (off) prng_normals4(b.seed + (uint32_t)r, b.tag, pa, (uint32_t)(j0 >> 2) + 1u, g1);
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            const int u = off + t;
+                            float z = g0[0];
+                            z = (u == 1) ? g0[1] : z;
+                            z = (u == 2) ? g0[2] : z;
+                            z = (u == 3) ? g0[3] : z;
+                            z = (u == 4) ? g1[0] : z;
+                            z = (u == 5) ? g1[1] : z;
+                            z = (u == 6) ? g1[2] : z;
+                            if (c0 + t < n) za[t] = z * b.sqrt_h;
+                        }
+                    }
+                }
+                float4 bf[NB];
+#pragma unroll
+                for (int j = q; j < NB; j++)
+                    if (j < nb) bf[j] = lt[corr_pair(q, j, NB) * 64 + lane];
+                // k step outside, blocks inside: consecutive MFMAs are independent
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int j = q; j < NB; j++) {
+                        if (j >= nb) break;          // uniform
+                        const float bv = t == 0 ? bf[j].x : t == 1 ? bf[j].y : t == 2 ? bf[j].z : bf[j].w;
+                        dw[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(za[t], bv, dw[j], 0, 0, 0);
+                    }
+            }
+            // Euler step of every (path, asset) the lane holds (kernel_multi_dupire's
+            // S + mu S h + sigma S dW, methods_cuda.cu:526-551)
+            float eterm = 0.0f;
+            if constexpr (MK != MK_MULTI_GBM) eterm = b.sigma0 * fast_exp(-b.alpha * ((float)k * b.h));
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                if (j >= nb) break;
+                const int a = 16 * j + col;
+                const float sg = (MK == MK_MULTI_GBM && a < n) ? b.sig[a] : 0.0f;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float Sv = S[j][i], d = dw[j][i];
+                    if constexpr (MK == MK_MULTI_GBM) {
+                        S[j][i] = Sv + b.mu * Sv * b.h + sg * Sv * d;
+                    } else {
+                        const float sl_ = eterm * fast_exp2(b.beta_m1 * fast_log2(Sv * b.inv_S0));
+                        S[j][i] = Sv + b.mu * Sv * b.h + sl_ * Sv * d;
+                    }
+                }
+            }
+            (void)muh;
+        }
+        // basket value per path: the lane's column, then the 16 lanes of its row group
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float B = 0.0f;
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                const int a = 16 * j + col;
+                if (j < nb) B += (a < n ? b.w[a] : 0.0f) * S[j][i];
+            }
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) B += __shfl_xor(B, o, 64);
+            const int64_t idx = wbase + 4 * ks + i;
+            if (col == 0 && idx >= b.first && idx < last) {
+                const float y = fmaxf(B - b.K, 0.0f) * b.discount;
+                acc[0] += (double)y;
+                acc[1] += (double)y * (double)y;
+            }
+        }
+    }
+    block_sum<2, 4>(acc, red);
+    if (tid == 0) {
+        double* out = b.partials + ((int64_t)r * b.nwg_x + blockIdx.x) * 2;
+        out[0] = acc[0];
+        out[1] = acc[1];
+    }
+}
+
+// Column blocks of 16 assets the correlated kernel is compiled for: n <= 16 NB.
+static int corr_nb(int n) { return n <= 64 ? 4 : n <= 128 ? 8 : 16; }
+
+template <int MK, int SRC>
+static const void* pick_corr(int n) {
+    switch (corr_nb(n)) {
+    case 4: return (const void*)&k_basket_corr<MK, SRC, 4>;
+    case 8: return (const void*)&k_basket_corr<MK, SRC, 8>;
+    default: return (const void*)&k_basket_corr<MK, SRC, 16>;
+    }
+}
+
+const void* basket_wide_kernel(int mk, int src, bool corr, int n) {
 #define W(MKv, SRCv)                                                                              \
     if (mk == MKv && src == SRCv)                                                                 \
-        return corr ? (const void*)&k_basket_wide<MKv, SRCv, true> : (const void*)&k_basket_wide<MKv, SRCv, false>;
+        return corr ? pick_corr<MKv, SRCv>(n) : (const void*)&k_basket_wide<MKv, SRCv>;
     W(MK_MULTI_GBM, SRC_SOBOL) W(MK_MULTI_GBM, SRC_PRNG) W(MK_MULTI_DUPIRE, SRC_SOBOL) W(MK_MULTI_DUPIRE, SRC_PRNG)
 #undef W
     return nullptr;
 }
 
-int basket_wide_wg(bool corr) { return corr ? 64 : WIDE_WG; }
-size_t basket_wide_lds_bytes(int n, bool corr) { return corr ? (size_t)2 * n * 64 * sizeof(float) : 0; }
+int basket_wide_block(bool corr) { return corr ? 256 : WIDE_WG; }
+int basket_wide_tile(bool corr) { return corr ? CORR_TILE : WIDE_WG; }
+size_t basket_wide_lds_bytes(int n, bool corr) {
+    if (!corr) return 0;
+    const int nb = corr_nb(n);
+    return (size_t)nb * (nb + 1) / 2 * 64 * sizeof(float4);
+}
 
 template <int MK, int SRC>
 static const void* pick_na(int na) {

@@ -141,7 +141,7 @@ struct BasketLaunch {
 // Baskets of more than BASKET_MAX assets (k_basket.hip, k_basket_wide): per-asset
 // parameters in device arrays.
 constexpr int WIDE_WG = 128;        // lanes (paths) per workgroup (uncorrelated; correlated: 64)
-constexpr int WIDE_CORR_MAX = 256;  // correlated: the date's normals and prices sit in LDS (512 B per asset)
+constexpr int WIDE_CORR_MAX = 256;  // correlated: 16 column blocks of 16 assets in k_basket_corr
 struct BasketWide {
     int n, n_steps, D;
     float h, sqrt_h, mu, K, discount;

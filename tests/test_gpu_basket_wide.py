@@ -57,6 +57,45 @@ def test_wide_basket_prng_matches_oracle(engine, model, payoff, n_steps):
     assert abs(gv - rv) <= 1e-3 * rv
 
 
+# shapes of the MFMA correlation kernel's edges: asset counts that are not a
+# multiple of 4 (Philox groups of four normals straddle asset blocks and dates)
+# or of 16 (a partial last block), offsets inside a 64-path tile, and the
+# 16-block compilation above 128 assets
+GBM37 = fdmc.MultiGBMParams(37, [95.0 + a * 0.25 for a in range(37)], [0.12 + 0.005 * a for a in range(37)], 0.04, 1.0,
+                            L=np.linalg.cholesky(_corr_100(37, 5)).ravel())
+PAY37 = fdmc.Basket(101.0, 0.04, 1.0, 37, weights=[1.0 / 37 + (a - 18) * 2e-4 for a in range(37)])
+DUP150 = fdmc.MultiDupireParams(n=150, L=np.linalg.cholesky(_corr_100(150, 7)).ravel(),
+                                S0=[100.0 + 0.1 * (a % 7) for a in range(150)])
+PAY150 = fdmc.Basket(100.0, 0.05, 1.0, 150)
+GBM113 = fdmc.MultiGBMParams(113, [100.0] * 113, [0.2 + 0.001 * a for a in range(113)], 0.05, 1.0,
+                             L=np.linalg.cholesky(_corr_100(113, 9)).ravel())
+PAY113 = fdmc.Basket(100.0, 0.05, 1.0, 113)
+
+
+@pytest.mark.parametrize("model,payoff,n_steps,first,npts", [
+    (GBM37, PAY37, 7, 37, 300),
+    (GBM113, PAY113, 9, 64, 128),
+    (DUP150, PAY150, 5, 5, 200),
+], ids=["gbm37", "gbm113", "dupire150"])
+def test_corr_basket_edge_shapes_qmc(engine, model, payoff, n_steps, first, npts):
+    R = 2
+    print(model.n, "assets,", n_steps, "dates, points", first, "+", npts, flush=True)
+    got = np.array(engine.qmc_accumulate(model, payoff, n_steps, R, 42, first, npts, fdmc.NOISE_RAW))
+    ref = O.qmc_accumulate(model, payoff, n_steps, R, 42, first, npts, fdmc.NOISE_RAW)
+    assert np.allclose(got[:, 0] / npts, ref[:, 0] / npts, rtol=2e-5, atol=1e-6), (got[:, 0], ref[:, 0])
+    assert np.allclose(got[:, 1] / npts, ref[:, 1] / npts, rtol=5e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("model,payoff,n_steps", [(GBM37, PAY37, 6), (GBM113, PAY113, 3), (DUP150, PAY150, 4)],
+                         ids=["gbm37", "gbm113", "dupire150"])
+def test_corr_basket_edge_shapes_prng(engine, model, payoff, n_steps):
+    print(model.n, "assets,", n_steps, "dates, Philox", flush=True)
+    gm, gv = engine.run_mc_fixed(model, payoff, n_steps, 1000, 7)
+    rm, rv = O.run_mc_fixed(model, payoff, n_steps, 1000, 7)
+    assert abs(gm - rm) <= 2e-5 * abs(rm) + 1e-6, (gm, rm)
+    assert abs(gv - rv) <= 1e-3 * rv
+
+
 def test_ejemplo07_qmc_and_mc_vs_report(engine):
     """Config of examples/ejemplo07_basket_uncorr.cpp: run_qmc_cuda Raw and
     run_mc_cuda at eps = 0.01 with 20 dates against the report's 5000-path
