@@ -205,22 +205,50 @@ __global__ __launch_bounds__(WIDE_WG) void k_basket_wide(BasketWide b) {
             sl.hi_gray = hi ^ (hi >> 1);
         }
         float B = 0.0f;
-        for (int a = 0; a < n; a++) {
-            float S = b.S0[a];
-            const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
-            for (int k = 0; k < b.n_steps; k++) {
-                const int j = k * n + a;
-                float z;
-                if constexpr (SRC == SRC_SOBOL) {
-                    z = icdf_u32_tab(sobol_coord(sl, j), itab);
-                } else {
-                    float q[4];
-                    prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j >> 2), q);
-                    z = q[j & 3];
+        if constexpr (SRC == SRC_SOBOL) {
+            for (int a = 0; a < n; a++) {
+                float S = b.S0[a];
+                const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
+                for (int k = 0; k < b.n_steps; k++) {
+                    const float z = icdf_u32_tab(sobol_coord(sl, k * n + a), itab);
+                    S = wide_step<MK>(b, S, sg, z * b.sqrt_h, k);
                 }
-                S = wide_step<MK>(b, S, sg, z * b.sqrt_h, k);
+                B += b.w[a] * S;
             }
-            B += b.w[a] * S;
+        } else {
+            // four assets at a time: normals k n + a0 .. k n + a0 + 3 of a date are
+            // consecutive, so one Philox block (two when k n + a0 is not a multiple
+            // of 4; the offset is the same for every lane) serves all four
+            for (int a0 = 0; a0 < n; a0 += 4) {
+                float S[4], sg[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int a = min(a0 + t, n - 1);
+                    S[t] = b.S0[a];
+                    sg[t] = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
+                }
+                for (int k = 0; k < b.n_steps; k++) {
+                    const int j0 = k * n + a0, off = j0 & 3;
+                    float g0[4], g1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j0 >> 2), g0);
+                    if (off) prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j0 >> 2) + 1u, g1);
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const int u = off + t;
+                        float z = g0[0];
+                        z = (u == 1) ? g0[1] : z;
+                        z = (u == 2) ? g0[2] : z;
+                        z = (u == 3) ? g0[3] : z;
+                        z = (u == 4) ? g1[0] : z;
+                        z = (u == 5) ? g1[1] : z;
+                        z = (u == 6) ? g1[2] : z;
+                        S[t] = wide_step<MK>(b, S[t], sg[t], z * b.sqrt_h, k);
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (a0 + t < n) B += b.w[a0 + t] * S[t];
+            }
         }
         const float y = fmaxf(B - b.K, 0.0f) * b.discount;
         if (valid) {

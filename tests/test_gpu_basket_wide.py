@@ -48,8 +48,13 @@ def test_wide_basket_qmc_matches_oracle(engine, model, payoff, n_steps, npts):
     assert np.allclose(got[:, 1] / npts, ref[:, 1] / npts, rtol=5e-5, atol=1e-6)
 
 
-@pytest.mark.parametrize("model,payoff,n_steps", [(UNCORR, PAY1000, 10), (CORR100, PAY100, 24)],
-                         ids=["uncorr1000", "corr100"])
+UNCORR37 = fdmc.MultiDupireParams(n=37, uncorrelated=True, S0=[100.0 + 0.5 * (a % 5) for a in range(37)])
+PAYU37 = fdmc.Basket(101.0, 0.05, 1.0, 37, weights=[1.0 / 37 + (a - 18) * 2e-4 for a in range(37)])
+
+
+@pytest.mark.parametrize("model,payoff,n_steps", [(UNCORR, PAY1000, 10), (CORR100, PAY100, 24),
+                                                  (UNCORR37, PAYU37, 7)],
+                         ids=["uncorr1000", "corr100", "uncorr37"])
 def test_wide_basket_prng_matches_oracle(engine, model, payoff, n_steps):
     gm, gv = engine.run_mc_fixed(model, payoff, n_steps, 1000, 99)
     rm, rv = O.run_mc_fixed(model, payoff, n_steps, 1000, 99)
