@@ -66,6 +66,12 @@ __device__ __forceinline__ float icdf_u32_tab(uint32_t s, const float4* tab) {
     return -icdf_u32_tab_scaled_neg(s, tab, 1.0f);
 }
 
+// a ^ b ^ c in one VALU instruction: gfx950's three-input bitwise operation
+// v_bitop3_b32 with truth table 0x96 (the parity of the three input bits).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // ---- Philox4x32-10 -------------------------------------------------------
 struct u32x4 { uint32_t x, y, z, w; };
 
@@ -78,7 +84,7 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
         const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
         const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        c = u32x4{xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }

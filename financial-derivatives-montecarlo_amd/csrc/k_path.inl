@@ -37,7 +37,7 @@ struct SobolLane {
 // LDS mode only (callers whose dimension never exceeds SOBOL_LDS_MAX_D).
 __device__ __forceinline__ uint32_t sobol_coord_lds(const SobolLane& s, int d) {
     const uint32_t* r = s.rec + d * 20;
-    return r[s.ga] ^ r[8 + s.gb] ^ r[16 + s.w];
+    return xor3(r[s.ga], r[8 + s.gb], r[16 + s.w]);
 }
 
 // Global mode: the lane's Gray bits from the record, the higher bits of the
@@ -45,14 +45,20 @@ __device__ __forceinline__ uint32_t sobol_coord_lds(const SobolLane& s, int d) {
 __device__ __forceinline__ uint32_t sobol_coord_glb(const SobolLane& s, int d) {
     const uint32_t* r = s.grec + (int64_t)d * SDIR_WORDS;
     uint32_t x = r[s.ga] ^ r[8 + s.gb];
-    for (uint32_t g = s.hi_gray; g; g &= g - 1u) x ^= r[32 + __builtin_ctz(g)];
+    uint32_t g = s.hi_gray;
+    for (; g & (g - 1u); g &= g - 1u) {          // two set bits at a time
+        const uint32_t b0 = __builtin_ctz(g);
+        g &= g - 1u;
+        x = xor3(x, r[32 + b0], r[32 + __builtin_ctz(g)]);
+    }
+    if (g) x ^= r[32 + __builtin_ctz(g)];
     return x;
 }
 
 __device__ __forceinline__ uint32_t sobol_coord(const SobolLane& s, int d) {
     if (s.lds_mode) {
         const uint32_t* r = s.rec + d * 20;
-        return r[s.ga] ^ r[8 + s.gb] ^ r[16 + s.w];
+        return xor3(r[s.ga], r[8 + s.gb], r[16 + s.w]);
     }
     return sobol_coord_glb(s, d);
 }
