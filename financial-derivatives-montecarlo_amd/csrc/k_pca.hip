@@ -400,22 +400,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
         for (int k = 0; k < BASKET_MAX; k++) sg = (k == a) ? E.sig[k] : sg;
         float* Pp = E.P + (int64_t)((n0 + wn) >> 7) * Mrows * E.n;
+        float prod[4][16];
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int e = 0; e < 16; e++) {
-                float prod = 1.0f;
+                float p = 1.0f;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const int col = n0 + wn + j * 32 + lr;
-                    prod *= (col < E.D) ? fmaf(sg, acc[i][j][e], E.c0) : 1.0f;
+                    p *= (col < E.D) ? fmaf(sg, acc[i][j][e], E.c0) : 1.0f;
                 }
-                for (int sft = E.n; sft < 32; sft <<= 1) prod *= __shfl_xor(prod, sft);
-                if (lr < E.n) {
-                    const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                    Pp[(int64_t)row * E.n + a] = prod;
-                }
+                prod[i][e] = p;
             }
+        // all 64 products of one shuffle distance are exchanged back to back, so
+        // the cross-lane round trips overlap (one wait per distance instead of
+        // one per product: 5 % of the launch)
+        for (int sft = E.n; sft < 32; sft <<= 1) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) prod[i][e] *= __shfl_xor(prod[i][e], sft);
+        }
+        if (lr < E.n) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                    Pp[(int64_t)row * E.n + a] = prod[i][e];
+                }
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < 4; i++)
