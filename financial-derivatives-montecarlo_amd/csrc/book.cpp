@@ -476,7 +476,10 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
             // local-vol time factor of every fine step: b e^{-c t} ln(2)^2 (quadratic
             // local vol, evaluated in log2), e^{-alpha t} (Dupire), t = k h in FP32 as
             // the steppers form it
-            std::vector<float> vt((size_t)BOOK_MAX_T * BOOK_VT_STRIDE, 0.0f);
+            // Planes 1 and 2 hold the same factors times 1/16 and 1/8 (exact: powers
+            // of two), for paths whose fine and coarse increments are built 16 and
+            // 8 times their size (the scaled bridge blocks of k_book.inl).
+            std::vector<float> vt((size_t)3 * BOOK_VT_PLANE, 0.0f);
             for (size_t ti = 0; ti < Ts.size(); ti++)
                 for (int l = 0; l < BOOK_NL; l++) {
                     const int nf = 1 << l;
@@ -486,6 +489,8 @@ fdmc_status fdmc_book_create(fdmc_context* ctx, const fdmc_model* model, const f
                         const float t = (float)k * h;
                         if (b->mk == MK_LVQ) row[k] = b->leg.lv_b * std::exp(-b->leg.lv_c * t) * 0.480453014f;  // ln(2)^2
                         else if (b->mk == MK_DUPIRE) row[k] = std::exp(-b->leg.alpha * t);
+                        row[BOOK_VT_PLANE + k] = row[k] * 0.0625f;
+                        row[2 * BOOK_VT_PLANE + k] = row[k] * 0.125f;
                     }
                 }
             upload(b->voltab, vt.data(), vt.size() * sizeof(float));

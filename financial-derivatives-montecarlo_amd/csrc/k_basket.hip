@@ -317,7 +317,6 @@ __global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
     }
     __syncthreads();
     const uint32_t* grec = (SRC == SRC_SOBOL) ? b.sdir + (int64_t)r * b.D * SDIR_WORDS : nullptr;
-    const float muh = b.mu * b.h;
     double acc[2] = {0.0, 0.0};
     const int64_t last = b.first + b.count;
     const int64_t tile0 = (b.first / CORR_TILE) + (int64_t)blockIdx.x * b.tiles_per_wg;
@@ -410,7 +409,6 @@ __global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
                     }
                 }
             }
-            (void)muh;
         }
         // basket value per path: the lane's column, then the 16 lanes of its row group
 #pragma unroll

@@ -398,6 +398,21 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
     float sdb[4];
 #pragma unroll
     for (int lv = 0; lv < 4; lv++) sdb[lv] = sdt[m + lv + 1];
+    // Barrier payoffs under the quadratic local vol (the book's bulk: no pathwise
+    // Greeks) build their blocks in scaled increments: an increment at in-block
+    // depth lv is kept 2^lv times its size, so the children of d are
+    // d -+ 2^(lv+1) sd z with no halving and no product sd z of their own (one
+    // FMA each, one VALU less per normal); the fine steps' increments come out 16
+    // and the coarse steps' 8 times their size, and the steps use the vol's level
+    // and time factors times 1/16 and 1/8 (the voltab's planes 1 and 2; powers of
+    // two, so sigma dw is the same product).
+    constexpr bool SC = PK == PK_BARRIER && MK == MK_LVQ;
+    float sds[4];
+#pragma unroll
+    for (int lv = 0; lv < 4; lv++) sds[lv] = sdb[lv] * (float)(2 << lv);
+    LegParams P16 = P, P8 = P;
+    P16.lv_a = P.lv_a * 0.0625f;
+    P8.lv_a = P.lv_a * 0.125f;
     // the block's increments from W at its ends, level by level in increments
     // (as the short path): each interval's increment d splits into d/2 - y and
     // d/2 + y; dc receives the two-date intervals' increments (coarse steps)
@@ -412,10 +427,16 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
             }
 #pragma unroll
             for (int q = (1 << lv) - 1; q >= 0; q--) {
-                const float y = icdf_u32_tab_scaled_neg(sobol_coord(sl, slot0 + (1 << lv) - 1 + q), itab, sdb[lv]);
                 const float d = dw[q];
-                dw[2 * q] = fmaf(d, 0.5f, -y);
-                dw[2 * q + 1] = fmaf(d, 0.5f, y);
+                if constexpr (SC) {
+                    const float y = icdf_u32_tab_scaled_neg(sobol_coord(sl, slot0 + (1 << lv) - 1 + q), itab, 1.0f);
+                    dw[2 * q] = fmaf(-sds[lv], y, d);
+                    dw[2 * q + 1] = fmaf(sds[lv], y, d);
+                } else {
+                    const float y = icdf_u32_tab_scaled_neg(sobol_coord(sl, slot0 + (1 << lv) - 1 + q), itab, sdb[lv]);
+                    dw[2 * q] = fmaf(d, 0.5f, -y);
+                    dw[2 * q + 1] = fmaf(d, 0.5f, y);
+                }
             }
         }
     };
@@ -448,17 +469,33 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
             }
         }
         float dw[16], dc[8], vb[16];
-        load_vb(i, vb);
         gen_block(i, Wp, rt[0], dw, dc);
         if constexpr (ADJ) {
             ckf[i * 256] = F.S;
             ckc[i * 256] = Cc.S;
         }
+        if constexpr (SC) {
+            cst<float>* v16 = vt + BOOK_VT_PLANE + i * 16;
+            cst<float>* v8 = vt + 2 * BOOK_VT_PLANE + i * 16;
+            float vc[8];
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const int gi = i * 16 + k;
-            F.step(P, dw[k], vb[k], g.hf, g.muh_f, gi + 1);
-            if (k & 1) Cc.step(P, dc[k >> 1], vb[k - 1], g.hc, g.muh_c, (gi >> 1) + 1);
+            for (int k = 0; k < 16; k++) vb[k] = v16[k];
+#pragma unroll
+            for (int k = 0; k < 8; k++) vc[k] = v8[2 * k];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int gi = i * 16 + k;
+                F.step(P16, dw[k], vb[k], g.hf, g.muh_f, gi + 1);
+                if (k & 1) Cc.step(P8, dc[k >> 1], vc[k >> 1], g.hc, g.muh_c, (gi >> 1) + 1);
+            }
+        } else {
+            load_vb(i, vb);
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int gi = i * 16 + k;
+                F.step(P, dw[k], vb[k], g.hf, g.muh_f, gi + 1);
+                if (k & 1) Cc.step(P, dc[k >> 1], vb[k - 1], g.hc, g.muh_c, (gi >> 1) + 1);
+            }
         }
     }
     book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,

@@ -216,6 +216,7 @@ constexpr int BOOK_NACC = 4;                      // P_f - P_c, delta, vega, rho
 constexpr int BOOK_MAX_ITEMS = 128;               // work items per entry and pass
 constexpr int BOOK_MAX_T = 8;                     // distinct maturities (bridge tables)
 constexpr int BOOK_VT_STRIDE = 2 * BOOK_DMAX - 1; // time factors of levels 0..9 per maturity
+constexpr int BOOK_VT_PLANE = BOOK_MAX_T * BOOK_VT_STRIDE;   // planes: factors, x 1/16, x 1/8
 
 struct BookContract {
     int kind;                 // PK_EUROPEAN / PK_ASIAN / PK_LOOKBACK / PK_BARRIER
@@ -261,7 +262,7 @@ struct BookLaunch {
     BookCtl* ctl;             // [contracts]
     const uint32_t* sdir[BOOK_NL];  // level l: [slots][R][2^l][SDIR_WORDS]
     const float* bbtab;       // bridge sd per depth: [maturities][BOOK_NL levels][BOOK_NL depths]
-    const float* voltab;      // local-vol time factor per fine step: [maturities][BOOK_VT_STRIDE]
+    const float* voltab;      // local-vol time factor per fine step: [3 planes][maturities][BOOK_VT_STRIDE]
     unsigned int* queue;
     int* total_items;
     int* active;
