@@ -21,6 +21,7 @@ struct BookGeom {
     int l, nf, nc, G;
     float hf, hc, muh_f, muh_c, bgk_f, bgk_c, T;
     float unit;   // price units per path unit: s_ref for the local vol (see book_body), else 1
+    float vs_f, vs_c;   // vega of the fine / coarse leg per carried vol-level derivative (scaled blocks)
 };
 
 // The quadratic local vol's paths are stepped in units of s_ref (S0, K and B
@@ -303,7 +304,7 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
         acc[0] += (double)((pf - pc) * g.unit);
         if constexpr (ADJ || TAN) {     // no pathwise Greeks for the barrier: the sums stay 0
             acc[1] += (double)(gf.s - gc.s);
-            acc[2] += (double)((gf.v - gc.v) * g.unit);
+            acc[2] += (double)((gf.v * g.vs_f - gc.v * g.vs_c) * g.unit);
             acc[3] += (double)((gf.m - gc.m) * g.unit);
         }
     }
@@ -398,15 +399,16 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
     float sdb[4];
 #pragma unroll
     for (int lv = 0; lv < 4; lv++) sdb[lv] = sdt[m + lv + 1];
-    // Barrier payoffs under the quadratic local vol (the book's bulk: no pathwise
-    // Greeks) build their blocks in scaled increments: an increment at in-block
-    // depth lv is kept 2^lv times its size, so the children of d are
-    // d -+ 2^(lv+1) sd z with no halving and no product sd z of their own (one
-    // FMA each, one VALU less per normal); the fine steps' increments come out 16
-    // and the coarse steps' 8 times their size, and the steps use the vol's level
-    // and time factors times 1/16 and 1/8 (the voltab's planes 1 and 2; powers of
-    // two, so sigma dw is the same product).
-    constexpr bool SC = PK == PK_BARRIER && MK == MK_LVQ;
+    // Under the quadratic local vol the blocks are built in scaled increments: an
+    // increment at in-block depth lv is kept 2^lv times its size, so the children
+    // of d are d -+ 2^(lv+1) sd z with no halving and no product sd z of their own
+    // (one FMA each, one VALU less per normal); the fine steps' increments come
+    // out 16 and the coarse steps' 8 times their size, and the steps use the
+    // vol's level and time factors times 1/16 and 1/8 (the voltab's planes 1 and
+    // 2; powers of two, so sigma dw is the same product).  Of the Euler step's
+    // partial derivatives only d/d(vol level) = S dw carries the scale: the
+    // legs' vegas are multiplied by 1/16 and 1/8 at the end (BookGeom::vs_*).
+    constexpr bool SC = MK == MK_LVQ;
     float sds[4];
 #pragma unroll
     for (int lv = 0; lv < 4; lv++) sds[lv] = sdb[lv] * (float)(2 << lv);
@@ -498,7 +500,12 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
             }
         }
     }
-    book_tail<MK, PK, GM>(P, g, F, Cc, valid, acc,
+    BookGeom gt = g;
+    if constexpr (SC) {
+        gt.vs_f = 0.0625f;
+        gt.vs_c = 0.125f;
+    }
+    book_tail<MK, PK, GM>(P, gt, F, Cc, valid, acc,
                           [&](float ef, float mf, float ec, float mc, BookAdj& Af, BookAdj& Ac) {
                               float lt[MMAX + 1];
                               float WR = WG;        // W at the current block's right end i + 1
@@ -519,12 +526,27 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
                                       }
                                   }
                                   float dw[16], dc[8], vb[16];
-                                  load_vb(i, vb);
                                   gen_block(i, lt[0], WR, dw, dc);
-                                  book_block_adj<MK, PK, 8, true>(P, ckc[i * 256], dc, 8, i * 8, vb, g.hc,
-                                                                  g.muh_c, ec, mc, Cc.kmin, Ac);
-                                  book_block_adj<MK, PK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vb, g.hf,
-                                                                    g.muh_f, ef, mf, F.kmin, Af);
+                                  if constexpr (SC) {
+                                      float v8[16];
+                                      cst<float>* p16 = vt + BOOK_VT_PLANE + i * 16;
+                                      cst<float>* p8 = vt + 2 * BOOK_VT_PLANE + i * 16;
+#pragma unroll
+                                      for (int k = 0; k < 16; k++) {
+                                          vb[k] = p16[k];
+                                          v8[k] = (k & 1) ? 0.0f : p8[k];    // the coarse steps read the even ones
+                                      }
+                                      book_block_adj<MK, PK, 8, true>(P8, ckc[i * 256], dc, 8, i * 8, v8, g.hc,
+                                                                      g.muh_c, ec, mc, Cc.kmin, Ac);
+                                      book_block_adj<MK, PK, 16, false>(P16, ckf[i * 256], dw, 16, i * 16, vb,
+                                                                        g.hf, g.muh_f, ef, mf, F.kmin, Af);
+                                  } else {
+                                      load_vb(i, vb);
+                                      book_block_adj<MK, PK, 8, true>(P, ckc[i * 256], dc, 8, i * 8, vb, g.hc,
+                                                                      g.muh_c, ec, mc, Cc.kmin, Ac);
+                                      book_block_adj<MK, PK, 16, false>(P, ckf[i * 256], dw, 16, i * 16, vb,
+                                                                        g.hf, g.muh_f, ef, mf, F.kmin, Af);
+                                  }
                               }
                           });
 }
@@ -658,6 +680,8 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         P.B = C.B;
         P.discount = C.discount;
         g.unit = 1.0f;
+        g.vs_f = 1.0f;
+        g.vs_c = 1.0f;
         if constexpr (book_scaled<MK>) {
             P.S0 *= P.inv_sref;
             P.K *= P.inv_sref;
