@@ -97,6 +97,10 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
 #pragma unroll
     for (int i = 0; i < NACC; i++) acc[i] = 0.0;
     const LegParams& P = a.leg[0];
+    // the leg for the 16-date bridge blocks' scaled increments (bb_block_uniform);
+    // of the step's partial derivatives only d/d(vol) = S dw carries the scale,
+    // so the vega of such paths is multiplied by 1/16 at the end
+    const LegParams P16 = vol_scaled(P, 0.0625f);
     const float h = a.h;
 
     const int64_t last = a.first + a.count;
@@ -123,6 +127,8 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
         // the rest of the tile, for a uniform-grid bridge (UNI) or bb_precompute's weights
         auto tile = [&](auto uniform_tag) {
             constexpr bool UNI = decltype(uniform_tag)::value;
+            constexpr bool SC = NZ == NZ_SOBOL_BB && UNI && B == 16;
+            const LegParams& PS = SC ? P16 : P;
             if constexpr (NZ == NZ_SOBOL_BB) {
                 sk[0] = 0.0f;
                 sk[G * 256] = bb_terminal<UNI, true>(a, sl, itab);
@@ -186,7 +192,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                 for (int k = 0; k < B; k++) {
                     const int g = i * B + k;
                     if (g < n) {
-                        S = step_1f<MK>(P, S, dw[k], (float)g * h, h);
+                        S = step_1f<MK>(PS, S, dw[k], (float)g * h, h);
                         if constexpr (PK == PK_LOOKBACK) {
                             if (S < run) { run = S; kmin = g + 1; }
                         } else if constexpr (is_path_dep<PK>()) {
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
     #pragma unroll
                 for (int k = 0; k < B; k++) {
                     const int g = i * B + k;
-                    Sl[k + 1] = (g < n) ? step_1f<MK>(P, Sl[k], dw[k], (float)g * h, h) : Sl[k];
+                    Sl[k + 1] = (g < n) ? step_1f<MK>(PS, Sl[k], dw[k], (float)g * h, h) : Sl[k];
                 }
     #pragma unroll
                 for (int k = B - 1; k >= 0; k--) {
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                         if (g + 1 == n) adj += c_last;
                         if (g + 1 == kmin) adj += c_min;
                         float fS, fv, fm, f0;
-                        step_partials<MK>(P, Sl[k], dw[k], (float)g * h, h, fS, fv, fm, f0);
+                        step_partials<MK>(PS, Sl[k], dw[k], (float)g * h, h, fS, fv, fm, f0);
                         vb += adj * fv;
                         mb += adj * fm;
                         s0b += adj * f0;
@@ -247,6 +253,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                 }
             }
             if (kmin == 0) adj += c_min;     // running minimum at S0
+            if constexpr (SC) vb *= 0.0625f;
             float delta = adj + s0b;
             float rho = mb;
             if constexpr (PK == PK_EUROPEAN) rho -= a.T * ym;   // d/dr of the discount e^{-rT}

@@ -125,6 +125,11 @@ __device__ __forceinline__ float bb_terminal(const PathLaunch& a, const SobolLan
 // its midpoint into d/2 - y and d/2 + y (y = -sd z of the node), which is
 // W_M = (W_L + W_R)/2 + sd z written for the increments (no W inside the block,
 // no differences).  Node j0 + q at in-block depth l, j0 = (G + i) 2^l.
+// The increments are kept 2^l times their size at depth l, so a child of d is
+// d -+ 2^(l+1) sd z: one FMA, with no halving and no product sd z of its own;
+// the 16 increments come out 16 times their size, and the steps take them with
+// the vol parameters times 1/16 (vol_scaled: a power of two, so sigma dw is the
+// same product).
 template <bool LDS>
 __device__ __forceinline__ void bb_block_uniform(const PathLaunch& a, const SobolLane& sl, const float4* itab, int G,
                                                  int lgG, int i, float WL, float WR, float (&dw)[16]) {
@@ -132,15 +137,25 @@ __device__ __forceinline__ void bb_block_uniform(const PathLaunch& a, const Sobo
 #pragma unroll
     for (int l = 0; l < 4; l++) {
         const int j0 = (G << l) + (i << l);
-        const float sd = a.bb_sd[lgG + l + 1];
+        const float sds = a.bb_sd[lgG + l + 1] * (float)(2 << l);
 #pragma unroll
         for (int q = (1 << l) - 1; q >= 0; q--) {
-            const float y = icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j0 + q), itab, sd);
+            const float y = icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j0 + q), itab, 1.0f);
             const float d = dw[q];
-            dw[2 * q] = fmaf(d, 0.5f, -y);
-            dw[2 * q + 1] = fmaf(d, 0.5f, y);
+            dw[2 * q] = fmaf(-sds, y, d);
+            dw[2 * q + 1] = fmaf(sds, y, d);
         }
     }
+}
+
+// The vol parameters of a leg times s (GBM sigma, Dupire sigma0, the quadratic
+// local vol's a and b): the leg for increments that are 1/s times too large.
+__device__ __forceinline__ LegParams vol_scaled(LegParams P, float s) {
+    P.sigma *= s;
+    P.sigma0 *= s;
+    P.lv_a *= s;
+    P.lv_b *= s;
+    return P;
 }
 
 template <int MK, int PK>
@@ -242,6 +257,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     double acc[NACC];
 #pragma unroll
     for (int i = 0; i < NACC; i++) acc[i] = 0.0;
+    // the legs for the 16-date bridge blocks' scaled increments (bb_block_uniform)
+    PathLaunch a16 = a;
+    if constexpr (BRIDGE && LB == 4) {
+        a16.leg[0] = vol_scaled(a.leg[0], 0.0625f);
+        a16.leg[1] = vol_scaled(a.leg[1], 0.0625f);
+    }
 
     const int64_t last = a.first + a.count;
     const int64_t tile0 = (a.first >> 8) + (int64_t)blockIdx.x * a.tiles_per_wg;
@@ -364,7 +385,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                             float dw[16];
                             bb_block_uniform<LDS>(a, sl, itab, G, lgG, i, Wp, rt[0], dw);
 #pragma unroll
-                            for (int k = 0; k < B; k++) ps.step(a, dw[k], i * B + k);
+                            for (int k = 0; k < B; k++) ps.step(a16, dw[k], i * B + k);
                         } else {
                             float W[B + 1];
                             W[0] = Wp;
