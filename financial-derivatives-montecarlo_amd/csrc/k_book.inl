@@ -312,7 +312,9 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
 
 // Levels of NF <= 16 dates (NF = 1, 2, 4, 8, 16): the whole bridge in
 // registers, unrolled, with no skeleton in memory; the same arithmetic, in the
-// same order, as one block of the long path.
+// same order, as one block of the long path in its unscaled form (the long
+// path's blocks of the quadratic local vol keep scaled increments instead,
+// book_path_long; the increments differ from these by rounding only).
 // dm[d]: XOR of the current tile's high Sobol words relative to the words in
 // the LDS records (the work item's first tile), per dimension.
 template <int MK, int PK, int GM, int NF>
