@@ -13,11 +13,14 @@
 //               double buffering; tiles are assigned so the 8 column tiles of
 //               one row block run on the same XCD (its L2 keeps the A rows).
 //   k_gemm_nt256: the same with 256 x 256 tiles (4 waves of 128 x 128), used
-//               whenever the shape allows it; with EPI = true it stores no C
-//               and instead multiplies the Euler factors of a GBM basket per
-//               row, asset and 128-column slab (ProdEpi), so dW never reaches
-//               memory.
+//               whenever the shape allows it;
+//   k_gemm_prod: 256 x 256 tiles that store no C: they multiply the Euler
+//               factors of a GBM basket per row, asset and 128-column slab
+//               (ProdEpi), so dW never reaches memory.  One persistent
+//               workgroup per CU, operands streamed straight into LDS
+//               (global_load_lds_dwordx4) four 32-deep k stages ahead of use.
 #include "k_path.inl"
+#include <algorithm>
 
 namespace fdmc {
 
@@ -215,9 +218,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const T* __restrict__ A, const 
 constexpr int G2_M = 256, G2_N = 256, G2_K = 64, G2_LD = G2_K + 8;
 constexpr size_t G2_LDS_BYTES = (size_t)2 * (G2_M + G2_N) * G2_LD * 2;
 
-template <typename T, bool EPI>
+template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_gemm_nt256(const T* __restrict__ A, const T* __restrict__ Bt,
-                                                    float* __restrict__ C, int Mrows, int N, int K, ProdEpi E) {
+                                                    float* __restrict__ C, int Mrows, int N, int K) {
     extern __shared__ __align__(16) unsigned char gsm[];
     T* As = (T*)gsm;                          // [2][G2_M * G2_LD]
     T* Bs = As + 2 * G2_M * G2_LD;            // [2][G2_N * G2_LD]
@@ -391,58 +394,243 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #undef FDMC_G2_STORE
     // D layout of 32x32 MFMA: lane l holds column l % 32, element e at row
     // (e / 4) * 8 + (l / 32) * 4 + e % 4.
-    if constexpr (EPI) {
-        // the lane's columns n0 + wn + 32 j + l % 32 all belong to asset
-        // a = l % n; lanes l ^ n, l ^ 2n, ... < 32 apart hold the same rows and
-        // asset at other dates
-        const int a = lr & (E.n - 1);
-        float sg = 0.0f;
 #pragma unroll
-        for (int k = 0; k < BASKET_MAX; k++) sg = (k == a) ? E.sig[k] : sg;
-        float* Pp = E.P + (int64_t)((n0 + wn) >> 7) * Mrows * E.n;
-        float prod[4][16];
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                const int col = n0 + wn + j * 32 + lr;
+                __builtin_nontemporal_store(acc[i][j][e], C + (int64_t)row * N + col);   // streamed once
+            }
+}
+
+
+// ---- GEMM with the basket-product epilogue: persistent, operands DMA'd to LDS --
+// dW = Z M^T for 256 x 256 tiles (4 waves of 128 x 128, as k_gemm_nt256), but
+// each tile ends in the Euler-factor products of ProdEpi instead of a C store,
+// and the kernel is shaped around that:
+//   - one workgroup per CU walks its XCD's contiguous tile range with a stride of
+//     the XCD's workgroup count (the 4 column tiles of 8 row blocks in flight on
+//     an XCD share its L2), so a tile's epilogue and stores overlap the next
+//     tile's loads instead of holding the CU's LDS until the stores drain;
+//   - operands go from global memory straight into LDS with
+//     global_load_lds_dwordx4 (no staging registers, no LDS store instructions)
+//     in stages of BK = 32: stage = 256 A rows + 256 B rows of 64 bytes, 16-byte
+//     chunks XOR-swizzled by ((row >> 1) ^ (row >> 2)) & 3 (conflict-free
+//     fragment reads), four stages in four LDS objects (128 KB);
+//   - the DMA is one stream of stages across tiles: stage s + 2's B half is
+//     issued during stage s's first k16 step, its A half (stage s + 3) after the
+//     stage's barrier, spaced one instruction per four MFMAs; a tile's last
+//     stages already load the next tile's first ones;
+//   - per stage one barrier: each wave waits for its own loads of stage s + 1
+//     (s_waitcnt vmcnt), then s_barrier makes stage s + 1 visible and releases
+//     stage s - 1's buffer; fragments are read one k16 step ahead, across stage
+//     boundaries;
+//   - the epilogue runs band by band (32 rows), so it needs few registers inside
+//     the tile loop; for 16 assets one v_permlane16_swap pairs the two half-row
+//     partial products of two rows at once (NA = 16), other asset counts
+//     combine lanes with shuffles.
+// Accumulation order is that of k_gemm_nt256 (k ascending, one MFMA per k16
+// step and 32 x 32 block), so the products are bit-identical to forming them
+// from k_gemm_nt256's dW.  Requires Mrows % 256 == 0, N % 256 == 0,
+// K % 128 == 0 and a grid that is a multiple of 8 (checked on the host).
+constexpr int GP_BK = 32, GP_RB = 64, GP_STAGE = 512 * GP_RB, GP_NS = 4;
+
+__device__ __forceinline__ int gp_swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+// s_waitcnt vmcnt(N) with expcnt and lgkmcnt left free (gfx9 encoding)
+template <int N_>
+__device__ __forceinline__ void gp_wait_vm() {
+    static_assert(N_ >= 0 && N_ < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N_ & 15) | ((N_ >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <typename T, int NA>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_gemm_prod(
+    const T* __restrict__ A, const T* __restrict__ Bt, int Mrows, int N, int K, ProdEpi E) {
+    __shared__ __align__(1024) unsigned char b0[GP_STAGE], b1[GP_STAGE], b2[GP_STAGE], b3[GP_STAGE];
+    // stage buffers are distinct objects, so the compiler can tell a DMA into one
+    // from fragment reads of another (no blanket vmcnt(0) before LDS reads)
+    auto stage = [&](int j) -> unsigned char* { return j == 0 ? b0 : j == 1 ? b1 : j == 2 ? b2 : b3; };
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ntn = N / G2_N, ntiles = ntn * (Mrows / G2_M);
+    const int nx = gridDim.x >> 3;                      // workgroups per XCD
+    const int per = (ntiles + 7) / 8;
+    const int tend = min(ntiles, (int)(blockIdx.x & 7) * per + per);
+    int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (t >= tend) return;
+    const int wm = (w >> 1) * 128, wn = (w & 1) * 128;
+    // DMA: a wave instruction fills one 1 KB block of 16 rows; lane l writes
+    // bytes 16 l .. 16 l + 15 of it = row l / 4, physical chunk l % 4, i.e.
+    // logical chunk (l % 4) ^ swz(row).  Wave w fills blocks w + 4 q of A and B.
+    const int drow = lane >> 2;
+    const uint32_t voff = (uint32_t)(drow * K + (((lane & 3) ^ gp_swz(drow)) * 8)) * (uint32_t)sizeof(T);
+    const int64_t bstep = (int64_t)64 * K * (int64_t)sizeof(T);         // 4 blocks further
+    const int nk = K / GP_BK;
+    auto baseA = [&](int tt) { return (const char*)(A + (int64_t)((tt / ntn) * G2_M + 16 * w) * K); };
+    auto baseB = [&](int tt) { return (const char*)(Bt + (int64_t)((tt % ntn) * G2_N + 16 * w) * K); };
+    const char* gA = baseA(t);
+    const char* gB = baseB(t);
+    int tn = t + nx;
+    const char* gAn = tn < tend ? baseA(tn) : gA;       // after the last tile: reloads never read
+    const char* gBn = tn < tend ? baseB(tn) : gB;
+    auto dma_half = [&](const char* p, int ks, unsigned char* st) {
+        const int64_t k0 = (int64_t)ks * GP_BK * (int64_t)sizeof(T);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            __builtin_amdgcn_global_load_lds((const void*)(p + (q * bstep + k0) + voff),
+                                             (void __attribute__((address_space(3)))*)(st + (w + 4 * q) * 1024), 16, 0, 0);
+    };
+    // fragments: lane l reads row 32 i + l % 32 at logical chunk 2 step + l / 32
+    const int lr = lane & 31, hi = lane >> 5, fs = gp_swz(lr);
+    const int fo0 = lr * GP_RB + (((0 + hi) ^ fs) << 4), fo1 = lr * GP_RB + (((2 + hi) ^ fs) << 4);
+    using V8 = typename Frag<T>::type;
+    auto ldfrag = [&](const unsigned char* st, int f, V8* fa, V8* fb) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            fa[i] = *(const V8*)(st + (wm + 32 * i) * GP_RB + f);
+            fb[i] = *(const V8*)(st + 256 * GP_RB + (wn + 32 * i) * GP_RB + f);
+        }
+    };
+    f32x16 acc[4][4];
+    auto mma = [&](const V8* fa, const V8* fb, bool zero) {
+        const f32x16 z = {};
 #pragma unroll
         for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma32v<T>(fa[i], fb[j], zero ? z : acc[i][j]);
+    };
+    // 16 MFMAs with 8 fragment reads between them and `nv` DMA instructions
+    // spaced evenly; nothing crosses the group
+#define FDMC_GP_ORDER(nv)                                                            \
+    do {                                                                             \
+        for (int m_ = 0; m_ < 16; m_++) {                                            \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                       \
+            if (m_ % 2 == 0) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);      \
+            if (m_ % (16 / (nv)) == 16 / (nv) - 1) __builtin_amdgcn_sched_group_barrier(0x010, 1, 0); \
+        }                                                                            \
+        __builtin_amdgcn_sched_barrier(0);                                           \
+    } while (0)
+    V8 fa[2][4], fb[2][4];
+    // one stage s in buffer j: step 0 (B half of stage s + 2 issued, step 1's
+    // fragments read), wait for stage s + 1 (own loads: 4 B + 4 A instructions
+    // of stage s + 2 may still be in flight), barrier, A half of stage s + 3
+    // issued, step 1 (step 0 fragments of stage s + 1 read)
+    auto run_stage = [&](int s, int j, bool zero) {
+        {
+            const int g = s + 2;
+            const bool cur = g < nk;
+            dma_half(cur ? gB : gBn, cur ? g : g - nk, stage((j + 2) % GP_NS) + 256 * GP_RB);
+        }
+        ldfrag(stage(j), fo1, fa[1], fb[1]);
+        mma(fa[0], fb[0], zero);
+        FDMC_GP_ORDER(4);
+        gp_wait_vm<8>();
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            const int g = s + 3;
+            const bool cur = g < nk;
+            dma_half(cur ? gA : gAn, cur ? g : g - nk, stage((j + 3) % GP_NS));
+        }
+        ldfrag(stage((j + 1) % GP_NS), fo0, fa[0], fb[0]);
+        mma(fa[1], fb[1], false);
+        FDMC_GP_ORDER(4);
+    };
+    // prologue: stages 0 and 1 whole, stage 2's A half
+    dma_half(gA, 0, stage(0));
+    dma_half(gB, 0, stage(0) + 256 * GP_RB);
+    dma_half(gA, 1, stage(1));
+    dma_half(gB, 1, stage(1) + 256 * GP_RB);
+    dma_half(gA, 2, stage(2));
+    bool first = true;
+    const int a_n = lr & ((NA ? NA : E.n) - 1);        // this lane's asset
+    float sg = 0.0f;
+#pragma unroll
+    for (int k = 0; k < BASKET_MAX; k++) sg = (k == a_n) ? E.sig[k] : sg;
+    while (true) {
+        const int m0 = (t / ntn) * G2_M, n0 = (t % ntn) * G2_N;
+        // stage 0 loaded in every wave; after an epilogue its stores are in
+        // flight too, and stores and loads may complete out of order: wait for all
+        if (first) gp_wait_vm<12>(); else gp_wait_vm<0>();
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        ldfrag(stage(0), fo0, fa[0], fb[0]);
+        // the first four stages peeled: the first MFMA of a tile starts from 0
+#pragma unroll
+        for (int j = 0; j < GP_NS; j++) run_stage(j, j, j == 0);
+        for (int sb = GP_NS; sb < nk; sb += GP_NS) {
+#pragma unroll
+            for (int j = 0; j < GP_NS; j++) run_stage(sb + j, j, false);
+        }
+        // epilogue.  D layout of 32x32 MFMA: lane l holds column l % 32, element
+        // e at row (e / 4) * 8 + (l / 32) * 4 + e % 4.  The lane's columns
+        // n0 + wn + 32 j + l % 32 all belong to asset l % n; lanes l ^ n,
+        // l ^ 2n, ... < 32 apart hold the same rows and asset at other dates.
+        float* Pp = E.P + (int64_t)((n0 + wn) >> 7) * Mrows * (NA ? NA : E.n);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            __builtin_amdgcn_sched_barrier(0);
+            // this band's accumulators moved to VGPRs here (left to itself the
+            // compiler copies all 256 before the first band and spills)
+            float x[4][16];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int e = 0; e < 16; e++)
+                    asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(x[j][e]) : "a"(acc[i][j][e]));
+            float prod[16];
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 float p = 1.0f;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const int col = n0 + wn + j * 32 + lr;
-                    p *= (col < E.D) ? fmaf(sg, acc[i][j][e], E.c0) : 1.0f;
+                    p *= (col < E.D) ? fmaf(sg, x[j][e], E.c0) : 1.0f;   // padding columns count as 1
                 }
-                prod[i][e] = p;
+                prod[e] = p;
             }
-        // all 64 products of one shuffle distance are exchanged back to back, so
-        // the cross-lane round trips overlap (one wait per distance instead of
-        // one per product: 5 % of the launch)
-        for (int sft = E.n; sft < 32; sft <<= 1) {
+            if constexpr (NA == 16) {
+                // one swap of the odd 16-lane rows of prod[e] with the even rows of
+                // prod[e + 1] puts e's two partials side by side in lanes 0-15 and
+                // e + 1's in lanes 16-31: one product gives lanes 0-15 row e and
+                // lanes 16-31 row e + 1 (one row further), which they store
+                float* pb = Pp + (int64_t)(m0 + wm + i * 32 + hi * 4 + (lr >> 4)) * 16 + a_n;
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) prod[i][e] *= __shfl_xor(prod[i][e], sft);
-        }
-        if (lr < E.n) {
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                    Pp[(int64_t)row * E.n + a] = prod[i][e];
+                for (int q = 0; q < 8; q++) {
+                    const int e = 2 * q;
+                    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(prod[e]), __float_as_uint(prod[e + 1]),
+                                                              false, false);
+                    pb[((e >> 2) * 8 + (e & 3)) * 16] = __uint_as_float(r[0]) * __uint_as_float(r[1]);
                 }
-        }
-    } else {
+            } else {
+                for (int sft = E.n; sft < 32; sft <<= 1) {
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const int row = m0 + wm + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-                    const int col = n0 + wn + j * 32 + lr;
-                    __builtin_nontemporal_store(acc[i][j][e], C + (int64_t)row * N + col);   // streamed once
+                    for (int e = 0; e < 16; e++) prod[e] *= __shfl_xor(prod[e], sft);
                 }
+                if (lr < E.n) {
+#pragma unroll
+                    for (int e = 0; e < 16; e++) {
+                        const int row = m0 + wm + i * 32 + (e >> 2) * 8 + hi * 4 + (e & 3);
+                        Pp[(int64_t)row * E.n + a_n] = prod[e];
+                    }
+                }
+            }
+        }
+        if (tn >= tend) break;
+        t = tn;
+        tn += nx;
+        gA = gAn;
+        gB = gBn;
+        gAn = tn < tend ? baseA(tn) : gA;
+        gBn = tn < tend ? baseB(tn) : gB;
+        first = false;
     }
+    gp_wait_vm<0>();   // no DMA into LDS outlives the workgroup
+#undef FDMC_GP_ORDER
 }
 
 // ---- host launchers ------------------------------------------------------------
@@ -471,25 +659,35 @@ hipError_t launch_gen_z(const uint32_t* sdir, int D, int ld, int R, int64_t firs
 
 hipError_t launch_gemm_nt(const void* A, const void* Bt, float* C, int Mrows, int N, int K, bool bf16,
                           hipStream_t st, const ProdEpi* epi) {
+    if (epi) {
+        // the product epilogue: persistent k_gemm_prod, one workgroup per CU
+        // (a multiple of 8, for the per-XCD tile ranges)
+        if (Mrows % G2_M || N % G2_N || K % (GP_NS * GP_BK)) return hipErrorInvalidValue;
+        ProdEpi E = *epi;
+        if (E.n < 2 || E.n > BASKET_MAX || (E.n & (E.n - 1)) || !E.P) return hipErrorInvalidValue;
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const int64_t tiles = (int64_t)(Mrows / G2_M) * (N / G2_N);
+        const int64_t grid = std::max<int64_t>(8, std::min<int64_t>((ncu / 8) * 8, (tiles + 7) / 8 * 8));
+        const void* fn = bf16 ? (E.n == 16 ? (const void*)&k_gemm_prod<__bf16, 16> : (const void*)&k_gemm_prod<__bf16, 0>)
+                              : (E.n == 16 ? (const void*)&k_gemm_prod<_Float16, 16> : (const void*)&k_gemm_prod<_Float16, 0>);
+        void* args[] = {(void*)&A, (void*)&Bt, (void*)&Mrows, (void*)&N, (void*)&K, (void*)&E};
+        hipError_t e = hipLaunchKernel(fn, dim3((unsigned)grid), dim3(256), args, 0, st);
+        if (e != hipSuccess) return e;
+        return hipGetLastError();
+    }
     if (Mrows % G2_M == 0 && N % G2_N == 0 && K % (2 * G2_K) == 0) {
         const int64_t tiles = (int64_t)(Mrows / G2_M) * (N / G2_N);
         const int64_t grid = (tiles + 7) / 8 * 8;
         if (grid > 0x7FFFFFFF) return hipErrorInvalidValue;
-        ProdEpi E{};
-        if (epi) {
-            E = *epi;
-            if (E.n < 2 || E.n > BASKET_MAX || (E.n & (E.n - 1)) || !E.P) return hipErrorInvalidValue;
-        }
-        const void* fn = bf16 ? (epi ? (const void*)&k_gemm_nt256<__bf16, true> : (const void*)&k_gemm_nt256<__bf16, false>)
-                              : (epi ? (const void*)&k_gemm_nt256<_Float16, true> : (const void*)&k_gemm_nt256<_Float16, false>);
+        const void* fn = bf16 ? (const void*)&k_gemm_nt256<__bf16> : (const void*)&k_gemm_nt256<_Float16>;
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G2_LDS_BYTES);
         if (e != hipSuccess) return e;
-        void* args[] = {(void*)&A, (void*)&Bt, (void*)&C, (void*)&Mrows, (void*)&N, (void*)&K, (void*)&E};
+        void* args[] = {(void*)&A, (void*)&Bt, (void*)&C, (void*)&Mrows, (void*)&N, (void*)&K};
         e = hipLaunchKernel(fn, dim3((unsigned)grid), dim3(256), args, G2_LDS_BYTES, st);
         if (e != hipSuccess) return e;
         return hipGetLastError();
     }
-    if (epi) return hipErrorInvalidValue;   // the product epilogue needs the 256 x 256 tiling
     if (Mrows % GB_M || N % GB_N || K % GB_K) return hipErrorInvalidValue;
     const int64_t tiles = (int64_t)(Mrows / GB_M) * (N / GB_N);
     const int64_t grid = (tiles + 7) / 8 * 8;
