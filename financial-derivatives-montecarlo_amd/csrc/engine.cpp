@@ -1572,6 +1572,39 @@ fdmc_status fdmc_pca_transform(fdmc_context* c, const fdmc_pca* pca, uint32_t sa
     });
 }
 
+fdmc_status fdmc_pca_products(fdmc_context* c, const fdmc_pca* pca, uint32_t salt, int64_t offset,
+                              int64_t n_points, int32_t bf16, int32_t n_assets, float c0, const float* sig,
+                              float* d_out) {
+    return guarded([&] {
+        if (!c || !pca || !sig || !d_out) fail(FDMC_E_INVALID, "null argument");
+        if (n_points <= 0 || offset < 0 || offset + n_points > (int64_t)4294967296LL)
+            fail(FDMC_E_RANGE, "point range outside the 32-bit Sobol index");
+        if (n_assets < 2 || n_assets > BASKET_MAX || (n_assets & (n_assets - 1)))
+            fail(FDMC_E_INVALID, "n_assets must be a power of two in [2, 32]");
+        const int D = pca->m, ld = pca->mp;
+        if (D > D_MAX_SOBOL) fail(FDMC_E_RANGE, "PCA dimension above 20000");
+        if (ld % 256) fail(FDMC_E_INVALID, "the product epilogue needs m rounded up to 128 to be a multiple of 256");
+        ProdEpi e{};
+        e.c0 = c0;
+        for (int a = 0; a < n_assets; a++) e.sig[a] = sig[a];
+        e.n = n_assets;
+        e.D = D;
+        std::vector<uint32_t> salts{salt};
+        const uint32_t* sdir = get_sdir(c, salts, D);
+        const int64_t mrows = pca_prod(c, pca, sdir, D, 1, offset, n_points, bf16 != 0, e);
+        const size_t row = (size_t)n_assets * sizeof(float);
+        HIPC(hipMemcpy2DAsync(d_out, (size_t)n_points * row, c->pca_dw.p, (size_t)mrows * row, (size_t)n_points * row,
+                              (size_t)(ld / 128), hipMemcpyDeviceToDevice, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        if (c->timing) {
+            float ms = 0.0f;
+            HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            c->timed_ms += ms;
+            c->timed_launches++;
+        }
+    });
+}
+
 fdmc_status fdmc_run_qmc(fdmc_context* c, const fdmc_model* model, const fdmc_payoff* payoff, double eps,
                          int32_t n_steps, const fdmc_qmc_config* cfg, int32_t mode, const fdmc_bb* bb,
                          const fdmc_pca* pca, fdmc_result* out) {

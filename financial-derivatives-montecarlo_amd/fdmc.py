@@ -435,6 +435,8 @@ _SIGS = {
                                       C.c_double, C.c_int32, C.POINTER(CQMCConfig), C.c_int32, _P,
                                       C.POINTER(CResult), C.POINTER(CGreeks)]),
     "fdmc_pca_transform": (C.c_int, [_P, _P, C.c_uint32, C.c_int64, C.c_int64, C.c_int32, _P]),
+    "fdmc_pca_products": (C.c_int, [_P, _P, C.c_uint32, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                                    _P, _P]),
     "fdmc_sobol_scrambled_u32": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_sobol_normal": (C.c_int, [_P, C.c_uint32, C.c_int32, C.c_int64, C.c_int64, _P]),
     "fdmc_icdf_u32": (C.c_int, [_P, _P, C.c_int64, _P]),
@@ -704,6 +706,12 @@ class Engine:
     def pca_transform(self, dev_pca, salt, offset, n_points, bf16, d_out):
         _check(self.lib, self.lib.fdmc_pca_transform(self.ctx, dev_pca.handle, salt, offset, n_points,
                                                      1 if bf16 else 0, d_out))
+
+    def pca_products(self, dev_pca, salt, offset, n_points, bf16, c0, sig, d_out):
+        """GBM-basket slab products of the PCA path, [mp / 128][n_points][len(sig)] FP32 at d_out."""
+        sv = (C.c_float * len(sig))(*sig)
+        _check(self.lib, self.lib.fdmc_pca_products(self.ctx, dev_pca.handle, salt, offset, n_points,
+                                                    1 if bf16 else 0, len(sig), c0, sv, d_out))
 
     # -- generator hooks (device pointers)
     def sobol_scrambled_u32(self, salt, D, n, offset, d_out):

@@ -394,6 +394,20 @@ fdmc_status fdmc_run_qmc_greeks(fdmc_context* ctx, const fdmc_model* model,
 fdmc_status fdmc_pca_transform(fdmc_context* ctx, const fdmc_pca* pca, uint32_t salt, int64_t offset,
                                int64_t n_points, int32_t bf16, float* d_out);
 
+/* The same PCA noise pushed through the Euler factors of an n_assets GBM basket
+ * the way the basket path forms them on matrix cores (phase1_pca,
+ * methods_cuda.cu:1151-1166, then the GBM case of kernel_multi_dupire's steps,
+ * methods_cuda.cu:526-551, fused into the GEMM's epilogue): column k of dW
+ * belongs to asset k % n_assets, and per point, asset and 128-column slab the
+ * output is the product of c0 + sig[asset] * dW over the slab's columns of that
+ * asset below m (columns m .. of the padding count as 1).  d_out: [mp / 128]
+ * [n_points][n_assets] FP32 device memory.  Needs mp (m rounded up to 128) a
+ * multiple of 256 and n_assets a power of two in [2, 32].  A diagnostic of the
+ * fused path (the basket kernels consume these products directly). */
+fdmc_status fdmc_pca_products(fdmc_context* ctx, const fdmc_pca* pca, uint32_t salt, int64_t offset,
+                              int64_t n_points, int32_t bf16, int32_t n_assets, float c0, const float* sig,
+                              float* d_out);
+
 /* ---- scrambled Sobol generator (methods_cuda.cu:753-844) ----
  * d_out is [D][n] (dimension-major, as kernel_gen_hh_scrambled_sobol_normal writes).
  * Point p of dimension d is the Hong-Hickernell-scrambled Gray-code Sobol point

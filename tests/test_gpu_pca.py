@@ -241,3 +241,77 @@ def test_run_qmc_pca_chunks_large_doublings(engine):
     means = tot / (1 << 17)
     assert abs(r.price - means.mean()) <= 1e-12 * abs(r.price)
     assert abs(r.std_error - means.std(ddof=1) / np.sqrt(R)) <= 1e-9 * r.std_error
+
+
+def _slab_products_ref(dw, D, n, c0, sig):
+    """The basket's slab products from FP32 dW rows [points][ld], in the order
+    the GEMM's product epilogue forms them: lane l of a 128-column slab holds
+    columns 32 j + l (asset l % n) and multiplies their factors c0 + sig dW
+    (one FP32 fused multiply-add each; padding columns >= D count as 1) for
+    j = 0..3, then lanes l and l ^ n, l ^ 2n, .. 16 combine pairwise."""
+    npts, ld = dw.shape
+    lanes = np.arange(32)
+    # the kernel receives sig and c0 as FP32
+    sg = np.asarray(sig, np.float32).astype(np.float64)[lanes % n]
+    c0 = np.float64(np.float32(c0))
+    out = np.empty((ld // 128, npts, n), np.float32)
+    for s in range(ld // 128):
+        p = np.ones((npts, 32), np.float32)
+        for j in range(4):
+            c = s * 128 + j * 32 + lanes
+            f = (sg * dw[:, c].astype(np.float64) + c0).astype(np.float32)   # exact fma, one rounding
+            p = p * np.where(c < D, f, np.float32(1.0))
+        sft = n
+        while sft < 32:
+            p = p * p[:, lanes ^ sft]
+            sft <<= 1
+        out[s] = p[:, :n]
+    return out
+
+
+@pytest.mark.parametrize("n,steps,bf16", [(16, 64, False), (16, 64, True), (8, 120, False), (32, 8, True),
+                                          (2, 384, False)],
+                         ids=["c3-16x64-fp16", "c3-16x64-bf16", "8x120-padded", "32x8", "2x384"])
+def test_gemm_product_epilogue_rows_match_products_of_dw(engine, torch_cuda, n, steps, bf16):
+    """The fused product epilogue (k_gemm_prod) against the same products formed
+    on the host from the plain GEMM's dW rows of the same points: both GEMMs
+    accumulate each output in the same k order, so the products agree to the
+    last bit or two (the host's fused multiply-add is emulated in FP64).  A
+    mistake in the epilogue's row, lane or asset mapping moves whole factors:
+    basket prices would not show it (paths assembled from other rows' slabs have
+    the same law), this comparison does.  Point counts not a multiple of 256
+    leave padding rows in the GEMM; 8 x 120 = 960 dates leave padding columns."""
+    torch = torch_cuda
+    sig, C = c3_parameters(n, seed=3 + n)
+    M = O.pca_kron_compute(steps, 1.0, C)
+    D = n * steps
+    pca = engine.pca_upload(D, M.T.ravel())
+    ld = (D + 127) // 128 * 128
+    npts, off = 1000, 768
+    c0 = 1.0 + 0.05 / steps
+    dw = torch.empty((npts, D), dtype=torch.float32, device="cuda")
+    engine.pca_transform(pca, 42, off, npts, bf16, dw.data_ptr())
+    prod = torch.empty((ld // 128, npts, n), dtype=torch.float32, device="cuda")
+    engine.pca_products(pca, 42, off, npts, bf16, c0, list(sig), prod.data_ptr())
+    torch.cuda.synchronize()
+    dw_full = np.zeros((npts, ld), np.float32)
+    dw_full[:, :D] = dw.cpu().numpy()
+    ref = _slab_products_ref(dw_full, D, n, c0, sig)
+    got = prod.cpu().numpy()
+    rel = np.abs(got.astype(np.float64) - ref) / np.abs(ref)
+    assert rel.max() < 3e-7, (rel.max(), np.unravel_index(rel.argmax(), rel.shape))
+    assert np.mean(got == ref) > 0.99
+    pca.free()
+
+
+def test_pca_products_rejects_bad_shapes(engine, torch_cuda, c3):
+    """Shapes the product epilogue cannot take are rejected on the host before
+    any device work (the output buffer is a real one all the same)."""
+    sig, C, M, pca = c3
+    buf = torch_cuda.empty((8, 256, 4), dtype=torch_cuda.float32, device="cuda")
+    with pytest.raises(fdmc.FdmcError):
+        engine.pca_products(pca, 42, 0, 256, False, 1.0, [0.2] * 3, buf.data_ptr())   # 3 assets
+    small = engine.pca_upload(128, np.eye(128).ravel())                                # mp = 128
+    with pytest.raises(fdmc.FdmcError):
+        engine.pca_products(small, 42, 0, 256, False, 1.0, [0.2] * 2, buf.data_ptr())
+    small.free()
