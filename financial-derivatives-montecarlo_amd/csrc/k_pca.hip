@@ -37,12 +37,16 @@ __device__ __forceinline__ _Float16 to_half<_Float16>(float x) { return (_Float1
 
 // ---- normals, point-major ----------------------------------------------------
 // Grid (workgroups, R); a workgroup runs `tpw` consecutive 256-point tiles of
-// replica r (lane = point) one 64-dimension chunk at a time: the chunk's direction
-// records (64 x 20 words, 5 KB) stay in LDS while the tiles are generated
-// (tile words updated incrementally, two XORs per dimension and tile), and the
-// normals are staged through LDS so that global stores are 16-byte and
-// row-contiguous.  49 KB of LDS: three workgroups per CU.  Columns D..ld-1
-// are written as zeros.
+// replica r (lane = point) one 64-dimension chunk at a time.  At a chunk's start
+// its direction records and the first tile's words go to LDS and each lane forms
+// the scrambled Sobol words of its point in the chunk's 64 dimensions
+// (sobol_coord_lds); they stay in registers for the chunk's tiles: from tile n - 1
+// to tile n, gray(256 n) changes in bits 7 and 8 + ctz(n) only, so every word
+// takes one three-input XOR with two direction numbers, which are the same for
+// all lanes and come through scalar loads (no LDS read per coordinate but the
+// inverse-CDF table row).  The normals are staged through LDS so that global
+// stores are 16-byte and row-contiguous.  49 KB of LDS: three workgroups per
+// CU.  Columns D..ld-1 are written as zeros.
 constexpr int GZ_TW = 72;   // staged row: 64 values + 8 of padding
 
 template <typename T>
@@ -65,18 +69,33 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
     for (int d0 = 0; d0 < ld; d0 += 64) {
         const int dn = D - d0 < 64 ? D - d0 : 64;     // <= 0 on padding chunks
         const uint32_t* gchunk = grec + (int64_t)d0 * SDIR_WORDS;
-        __syncthreads();                              // the previous chunk's records are no longer read
+        __syncthreads();                              // the previous chunk's records and staging are no longer read
         for (int i = tid; i < dn * 16; i += 256) rec[(i >> 4) * 20 + (i & 15)] = gchunk[(int64_t)(i >> 4) * SDIR_WORDS + (i & 15)];
+        if (dn > 0) sobol_tile_words(rec, gchunk, dn, (uint32_t)tfirst, true, tid);
+        __syncthreads();
+        uint32_t cur[64];
+#pragma unroll
+        for (int j = 0; j < 64; j++) cur[j] = (j < dn) ? sobol_coord_lds(sl, j) : 0u;
         for (int64_t t = tfirst; t <= tend; t++) {
-            __syncthreads();                          // tile words and staging free again
-            if (dn > 0) sobol_tile_words(rec, gchunk, dn, (uint32_t)t, t == tfirst, tid);
-            __syncthreads();
+            if (t > tfirst && dn > 0) {
+                const int c = 32 + 8 + __builtin_ctz((uint32_t)t);
+                if (dn == 64) {
+#pragma unroll
+                    for (int j = 0; j < 64; j++)
+                        cur[j] = xor3(cur[j], gchunk[j * SDIR_WORDS + 32 + 7], gchunk[j * SDIR_WORDS + c]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 64; j++)
+                        if (j < dn) cur[j] = xor3(cur[j], gchunk[j * SDIR_WORDS + 32 + 7], gchunk[j * SDIR_WORDS + c]);
+                }
+            }
+            __syncthreads();                          // staging free again
 #pragma unroll
             for (int j0 = 0; j0 < 64; j0 += 8) {
                 union { T h[8]; uint4 u; } v;
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    const float z = (j0 + j < dn) ? icdf_u32_tab(sobol_coord_lds(sl, j0 + j), itab) : 0.0f;
+                    const float z = (j0 + j < dn) ? icdf_u32_tab(cur[j0 + j], itab) : 0.0f;
                     v.h[j] = to_half<T>(z);
                 }
                 *(uint4*)(tile + tid * GZ_TW + j0) = v.u;
