@@ -12,10 +12,13 @@
 //             1{e^G>K} e^G/(n S_k) for the geometric mean, disc 1{S_T>K} at
 //             maturity, -bgk at the running minimum for the lookback);
 //   reverse : blocks i = G-1 .. 0: regenerate the block's increments from the
-//             Sobol index (or Philox counter) and the bridge skeleton, recompute
-//             S inside the block from its checkpoint, then sweep
-//             Sbar_k = direct_k + Sbar_{k+1} dS_{k+1}/dS_k,
-//             theta_bar += Sbar_{k+1} dS_{k+1}/dtheta.
+//             Sobol index (or Philox counter) and the bridge skeleton, and run
+//             the block forward from its checkpoint carrying J = dS_k/dS_start
+//             and the tangents T_k = dS_k/dtheta of this block's steps; then
+//             Sbar_start = sum_k direct_k J_k + Sbar_end J_B and
+//             theta_bar += sum_k direct_k T_k + Sbar_end T_B (the chain rule
+//             of Sbar_k = direct_k + Sbar_{k+1} dS_{k+1}/dS_k summed in the
+//             other order: no per-step state is stored).
 //
 // No path is stored: memory per lane is G + (bridge skeleton) floats.
 #pragma once
@@ -50,6 +53,9 @@ __device__ __forceinline__ void step_partials(const LegParams& P, float S, float
 }
 
 template <int MK, int PK, int CV, int CPK, int NZ, int LB>
+// The compiler's own allocation (about 210 VGPRs, two waves per SIMD): the
+// three-waves-per-SIMD build spills 34-46 VGPRs and ran the C2 adjoint pass
+// 6 % slower (0.280 against 0.263 ms), measured on MI355X.
 __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
     constexpr int NP = (CV == CV_NONE) ? 2 : 7;
     constexpr int NACC = NP + 6;
@@ -224,33 +230,44 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                 c_log -= a.beta_cv * ((e > a.leg[1].K) ? e * inv_n : 0.0f);
             }
 
-            // ---- reverse sweep
-            float adj = 0.0f, vb = 0.0f, mb = 0.0f, s0b = 0.0f;
+            // ---- reverse sweep, block by block.  The state is one-dimensional,
+            // so a block's vector-Jacobian product needs no per-step storage: one
+            // forward sweep from the block's checkpoint carries J = dS_k/dS_start
+            // and the tangents T of S_k in the parameters through this block's
+            // steps, and sums the direct terms d_k against them:
+            //   adj(S_start)  = sum_k d_k J_k + adj(S_end) J_B,
+            //   theta_bar    += sum_k d_k T_k + adj(S_end) T_B.
+            float adj = 0.0f, vb = 0.0f, mb = 0.0f, s0b = 0.0f;   // adj: d/dS at the block's end
             for (int i = G - 1; i >= 0; i--) {
-                float dw[B], Sl[B + 1];
+                float dw[B];
                 gen_block(i, dw);
-                Sl[0] = ck[i * 256];
+                float S = ck[i * 256];
+                float J = 1.0f, Tv = 0.0f, Tm = 0.0f, T0 = 0.0f;
+                float A = 0.0f, Av = 0.0f, Am = 0.0f, A0 = 0.0f;
     #pragma unroll
                 for (int k = 0; k < B; k++) {
                     const int g = i * B + k;
-                    Sl[k + 1] = (g < n) ? step_1f<MK>(PS, Sl[k], dw[k], (float)g * h, h) : Sl[k];
-                }
-    #pragma unroll
-                for (int k = B - 1; k >= 0; k--) {
-                    const int g = i * B + k;
                     if (g < n) {
-                        const float Sn = Sl[k + 1];
-                        adj += c_each + c_log / Sn;
-                        if (g + 1 == n) adj += c_last;
-                        if (g + 1 == kmin) adj += c_min;
                         float fS, fv, fm, f0;
-                        step_partials<MK>(PS, Sl[k], dw[k], (float)g * h, h, fS, fv, fm, f0);
-                        vb += adj * fv;
-                        mb += adj * fm;
-                        s0b += adj * f0;
-                        adj *= fS;
+                        step_partials<MK>(PS, S, dw[k], (float)g * h, h, fS, fv, fm, f0);
+                        S = step_1f<MK>(PS, S, dw[k], (float)g * h, h);
+                        J *= fS;
+                        Tv = fmaf(fS, Tv, fv);
+                        Tm = fmaf(fS, Tm, fm);
+                        T0 = fmaf(fS, T0, f0);
+                        float d = c_each + c_log / S;             // direct term of S_{g+1}
+                        if (g + 1 == n) d += c_last;
+                        if (g + 1 == kmin) d += c_min;
+                        A = fmaf(d, J, A);
+                        Av = fmaf(d, Tv, Av);
+                        Am = fmaf(d, Tm, Am);
+                        A0 = fmaf(d, T0, A0);
                     }
                 }
+                vb += fmaf(adj, Tv, Av);
+                mb += fmaf(adj, Tm, Am);
+                s0b += fmaf(adj, T0, A0);
+                adj = fmaf(adj, J, A);
             }
             if (kmin == 0) adj += c_min;     // running minimum at S0
             if constexpr (SC) vb *= 0.0625f;
