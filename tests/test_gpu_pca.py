@@ -269,10 +269,10 @@ def _slab_products_ref(dw, D, n, c0, sig):
     return out
 
 
-@pytest.mark.parametrize("n,steps,bf16", [(16, 64, False), (16, 64, True), (8, 120, False), (32, 8, True),
-                                          (2, 384, False)],
-                         ids=["c3-16x64-fp16", "c3-16x64-bf16", "8x120-padded", "32x8", "2x384"])
-def test_gemm_product_epilogue_rows_match_products_of_dw(engine, torch_cuda, n, steps, bf16):
+@pytest.mark.parametrize("n,steps,bf16,npts", [(16, 64, False, 1000), (16, 64, True, 1000), (8, 120, False, 1000),
+                                               (32, 8, True, 1000), (2, 384, False, 1000), (16, 64, False, 70000)],
+                         ids=["c3-16x64-fp16", "c3-16x64-bf16", "8x120-padded", "32x8", "2x384", "c3-multitile"])
+def test_gemm_product_epilogue_rows_match_products_of_dw(engine, torch_cuda, n, steps, bf16, npts):
     """The fused product epilogue (k_gemm_prod) against the same products formed
     on the host from the plain GEMM's dW rows of the same points: both GEMMs
     accumulate each output in the same k order, so the products agree to the
@@ -280,14 +280,17 @@ def test_gemm_product_epilogue_rows_match_products_of_dw(engine, torch_cuda, n, 
     mistake in the epilogue's row, lane or asset mapping moves whole factors:
     basket prices would not show it (paths assembled from other rows' slabs have
     the same law), this comparison does.  Point counts not a multiple of 256
-    leave padding rows in the GEMM; 8 x 120 = 960 dates leave padding columns."""
+    leave padding rows in the GEMM; 8 x 120 = 960 dates leave padding columns;
+    70,000 points are 1,096 tiles of 256 x 256, about four per persistent
+    workgroup, so later tiles (and the operand stream that runs from one tile
+    into the next) are checked too."""
     torch = torch_cuda
     sig, C = c3_parameters(n, seed=3 + n)
     M = O.pca_kron_compute(steps, 1.0, C)
     D = n * steps
     pca = engine.pca_upload(D, M.T.ravel())
     ld = (D + 127) // 128 * 128
-    npts, off = 1000, 768
+    off = 768
     c0 = 1.0 + 0.05 / steps
     dw = torch.empty((npts, D), dtype=torch.float32, device="cuda")
     engine.pca_transform(pca, 42, off, npts, bf16, dw.data_ptr())
