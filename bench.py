@@ -27,6 +27,7 @@ c3_pca_gemm, c4_heston_mlmc).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -460,6 +461,11 @@ def main():
     tangent_s = time.perf_counter() - t1
     tb.free()
     nonconv = sum(1 for r in res if r.flags)
+    # every contract's price and standard error, and the Greeks of those that carry them, must be finite
+    nonfinite = sum(1 for k, r in zip(contracts, res)
+                    if not all(math.isfinite(v) for v in
+                               ((r.price, r.std_error) if k.kind == fdmc.PAYOFF_BARRIER else
+                                (r.price, r.std_error, r.delta, r.vega, r.rho, r.delta_se, r.vega_se, r.rho_se))))
     max_se_ratio = max(r.std_error for r in res) / (EPS / 2 ** 0.5)
     lip = [r for r in res if r.delta == r.delta]
     out = {
@@ -478,7 +484,8 @@ def main():
                         "batched_graph_replays": st.graph_replays, "batched_equals_plain": same,
                         "tangent_greeks_batched_s": tangent_s, "autotune_s": tune_s,
                         "variant": st.variant, "wg_per_cu": st.wg_per_cu,
-                        "contracts_not_converged": nonconv, "max_se_over_target": max_se_ratio,
+                        "contracts_not_converged": nonconv, "contracts_not_finite": nonfinite,
+                        "max_se_over_target": max_se_ratio,
                         "contracts_with_greeks": len(lip)},
     }
     if not args.no_extras:

@@ -204,7 +204,12 @@ __device__ __forceinline__ float payoff(const LegParams& P, float S_T, float run
     } else if constexpr (PK == PK_GEOM_ASIAN) {
         return fmaxf(fast_exp(run / (float)n) - P.K, 0.0f);
     } else if constexpr (PK == PK_LOOKBACK) {
-        return S_T - run * bgk;
+        // S_T - min * bgk (methods_cuda.cu:346-347), and 0 for a path whose
+        // terminal value is not positive or NaN: it left the Euler scheme's
+        // domain (S <= 0, where a local vol's log is NaN), and every payoff pays
+        // 0 there, as the calls do through fmaxf.  For S_T > 0 the value is
+        // unchanged: min <= S_T and bgk <= 1.
+        return (S_T > 0.0f) ? S_T - run * bgk : 0.0f;
     } else {  // PK_BARRIER
         return (run / bgk >= P.B) ? 0.0f : fmaxf(S_T - P.K, 0.0f) * P.discount;
     }

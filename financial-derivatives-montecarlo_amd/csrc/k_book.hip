@@ -245,6 +245,9 @@ __global__ void k_book_final(BookLaunch a) {
         int64_t extra[BOOK_NL];
         conv = book_decide(a, C, s, k, extra) == 1;
     }
+    // an estimate that is not finite never counts as converged
+    for (int q = 0; q < BOOK_NACC; q++)
+        if (!(fabs(val[q]) < INFINITY && var[q] < INFINITY)) conv = false;
     o.flags = conv ? 0 : 2;
     a.out[c] = o;
 }

@@ -183,9 +183,9 @@ __device__ __forceinline__ void book_direct(const LegParams& P, float S_T, float
     cmin = 0.0f;
     if constexpr (PK == PK_ASIAN) each = (run / (float)n > P.K) ? 1.0f / (float)n : 0.0f;
     if constexpr (PK == PK_EUROPEAN) last = (S_T > P.K) ? P.discount : 0.0f;
-    if constexpr (PK == PK_LOOKBACK) {
-        last = 1.0f;
-        cmin = -bgk;
+    if constexpr (PK == PK_LOOKBACK) {   // the payoff is 0 unless S_T > 0
+        last = (S_T > 0.0f) ? 1.0f : 0.0f;
+        cmin = (S_T > 0.0f) ? -bgk : 0.0f;
     }
 }
 
@@ -273,7 +273,7 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
                     r = Tan3{Lg.acc.s * w, Lg.acc.v * w, Lg.acc.m * w};
                 }
             } else if constexpr (PK == PK_LOOKBACK) {
-                r = Tan3{Lg.t.s - bgk * Lg.acc.s, Lg.t.v - bgk * Lg.acc.v, Lg.t.m - bgk * Lg.acc.m};
+                if (Lg.S > 0.0f) r = Tan3{Lg.t.s - bgk * Lg.acc.s, Lg.t.v - bgk * Lg.acc.v, Lg.t.m - bgk * Lg.acc.m};
             }
             return r;
         };
@@ -295,10 +295,13 @@ __device__ __forceinline__ void book_tail(const LegParams& P, const BookGeom& g,
             gc.m -= g.T * pc;
         }
         if (nc == 0) gc = Tan3{0.0f, 0.0f, 0.0f};
-        // a path that left the Euler scheme's domain (S <= 0 under the local vol)
-        // pays 0 through fmaxf and contributes 0 to the Greeks too
-        if (!(fabsf(gf.s) + fabsf(gf.v) + fabsf(gf.m) + fabsf(gc.s) + fabsf(gc.v) + fabsf(gc.m) < INFINITY))
-            gf = gc = Tan3{0.0f, 0.0f, 0.0f};
+    }
+    if constexpr (ADJ || TAN) {
+        // a leg that left the Euler scheme's domain (S <= 0 under the local vol,
+        // after which its states are NaN) pays 0 and contributes 0 to the
+        // Greeks; the other leg of the pair counts as usual
+        if (!(fabsf(gf.s) + fabsf(gf.v) + fabsf(gf.m) < INFINITY)) gf = Tan3{0.0f, 0.0f, 0.0f};
+        if (!(fabsf(gc.s) + fabsf(gc.v) + fabsf(gc.m) < INFINITY)) gc = Tan3{0.0f, 0.0f, 0.0f};
     }
     if (valid) {
         acc[0] += (double)((pf - pc) * g.unit);

@@ -224,7 +224,10 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                 c_log = (e > P.K) ? e * inv_n : 0.0f;
             }
             if constexpr (PK == PK_EUROPEAN) c_last = (S > P.K) ? P.discount : 0.0f;
-            if constexpr (PK == PK_LOOKBACK) { c_last = 1.0f; c_min = -a.bgk[0]; }
+            if constexpr (PK == PK_LOOKBACK) {    // the payoff is 0 unless S_T > 0
+                c_last = (S > 0.0f) ? 1.0f : 0.0f;
+                c_min = (S > 0.0f) ? -a.bgk[0] : 0.0f;
+            }
             if constexpr (CV == CV_SHARED && CPK == PK_GEOM_ASIAN) {
                 const float e = fast_exp(crun * inv_n);
                 c_log -= a.beta_cv * ((e > a.leg[1].K) ? e * inv_n : 0.0f);

@@ -787,9 +787,9 @@ static void book_leg(const orc_params* P, int kind, const float* dW, int n, int 
         if (A > P->K) for (int k = 0; k < 3; k++) g[k] = ds[k] / (float)n;
         break;
     }
-    case FDMC_PAYOFF_LOOKBACK:
-        *y = S - mn * bgk;
-        for (int k = 0; k < 3; k++) g[k] = d[k] - bgk * dmn[k];
+    case FDMC_PAYOFF_LOOKBACK:     /* 0 for a path that left the domain, as in payoff_eval */
+        *y = (S > 0.0f) ? S - mn * bgk : 0.0f;
+        if (S > 0.0f) for (int k = 0; k < 3; k++) g[k] = d[k] - bgk * dmn[k];
         break;
     default:  /* barrier: not Lipschitz, no pathwise Greeks */
         *y = (mx / bgk >= P->B) ? 0.0f : fmaxf(S - P->K, 0.0f) * P->discount;
@@ -827,10 +827,11 @@ void orc_book_path(const fdmc_model* m, const fdmc_book_contract* k, const float
         book_leg(&P, k->kind, dW, nf / 2, 1, hc, book_bgk(k, hc), (float)k->T, &yc, gc);
     }
     out[0] = yf - yc;
-    /* a path that left the Euler scheme's domain contributes 0 to the Greeks */
-    float mag = 0.0f;
-    for (int i = 0; i < 3; i++) mag += fabsf(gf[i]) + fabsf(gc[i]);
-    for (int i = 0; i < 3; i++) out[1 + i] = (mag < INFINITY) ? gf[i] - gc[i] : 0.0f;
+    /* a leg that left the Euler scheme's domain (its states NaN after S <= 0)
+       contributes 0 to the Greeks; the other leg counts as usual */
+    float mf = 0.0f, mc = 0.0f;
+    for (int i = 0; i < 3; i++) { mf += fabsf(gf[i]); mc += fabsf(gc[i]); }
+    for (int i = 0; i < 3; i++) out[1 + i] = ((mf < INFINITY) ? gf[i] : 0.0f) - ((mc < INFINITY) ? gc[i] : 0.0f);
 }
 
 static int book_levels(const fdmc_book_contract* k) {
@@ -954,6 +955,9 @@ int orc_book_contract_run(const fdmc_model* m, const fdmc_book_contract* k, int 
     out->rho_se = lip ? sqrt(v[3]) : NAN;
     out->levels = L + 1;
     out->iterations = iter;
+    /* an estimate that is not finite never counts as converged */
+    for (int q = 0; q < 4; q++)
+        if (!(fabs(val[q]) < INFINITY && v[q] < INFINITY)) conv = 0;
     out->flags = conv ? 0u : FDMC_FLAG_NOT_CONVERGED;
     free(s); free(rm);
     return 0;

@@ -120,7 +120,10 @@ static inline float payoff_eval(const orc_params* P, float S_T, float run, int n
     case FDMC_PAYOFF_EUROPEAN: return fmaxf(S_T - P->K, 0.0f) * P->discount;
     case FDMC_PAYOFF_ASIAN: return fmaxf(run / n - P->K, 0.0f);
     case FDMC_PAYOFF_GEOM_ASIAN: return fmaxf(expf(run / n) - P->K, 0.0f);
-    case FDMC_PAYOFF_LOOKBACK: return S_T - run;
+    /* methods_cuda.cu:346-347 returns S_T - running, NaN for a path that left the
+       Euler scheme's domain (S <= 0 under a local vol): such a path pays 0 here,
+       as it does under the calls through fmaxf; for S_T > 0 nothing changes */
+    case FDMC_PAYOFF_LOOKBACK: return (S_T > 0.0f) ? S_T - run : 0.0f;
     case FDMC_PAYOFF_BARRIER: return (run >= P->B) ? 0.0f : fmaxf(S_T - P->K, 0.0f) * P->discount;
     default: return fmaxf(S_T - P->K, 0.0f) * P->discount;
     }

@@ -146,7 +146,7 @@ def _leg64(mkind, par, kind, K, B, T, r, inc):
         return np.maximum(S - K, 0.0) * np.exp(-r * T)
     if kind == fdmc.PAYOFF_ASIAN:
         return np.maximum(acc / n - K, 0.0)
-    return S - mn * bgk                # lookback
+    return np.where(S > 0, S - mn * bgk, 0.0)     # lookback; 0 for a path that left the domain
 
 
 def _diff64(mkind, par, kind, K, T, r, dW):
@@ -211,3 +211,83 @@ def test_stopping_rule_and_barrier_greeks():
         assert math.isnan(b.delta) == (kind == fdmc.PAYOFF_BARRIER)
     b = po.book_contract_run(m, contract(fdmc.PAYOFF_ASIAN, 16, T=0.5), 2, 1e-2, cfg(R=16, max_iter=1))
     assert b.iterations == 1 and b.flags == 2      # FDMC_FLAG_NOT_CONVERGED: out of passes
+
+
+# ------------------------------------------------------------------ paths that leave the domain
+# An Euler path under a local vol can step to S <= 0, where ln S is NaN and so is
+# every later state.  Every payoff of such a path is 0: the calls through fmaxf
+# (as in the reference's kernels), the lookback by the explicit S_T > 0 test
+# (the reference's S_T - min would be NaN), and its Greeks are 0.
+@pytest.mark.parametrize("kind", [fdmc.PAYOFF_EUROPEAN, fdmc.PAYOFF_ASIAN, fdmc.PAYOFF_LOOKBACK,
+                                  fdmc.PAYOFF_BARRIER])
+def test_book_paths_that_leave_the_domain_pay_zero(kind):
+    k = contract(kind, 4, K=90.0)
+    # h = 1/4: 1 + mu h + 0.2 (-6) < 0, so S_1 < 0 and S_2.. are NaN; the coarse
+    # leg's first step (-5.9) leaves too
+    out = po.book_path(lvq(), k, np.array([-6.0, 0.1, 0.2, -0.1], np.float32))
+    assert list(out) == [0.0, 0.0, 0.0, 0.0]
+    # one date: S_T < 0 and finite; the lookback's S_T - min would be 0 here but
+    # with the BGK factor negative
+    out = po.book_path(lvq(), contract(kind, 1, K=90.0), np.array([-6.0], np.float32))
+    assert list(out) == [0.0, 0.0, 0.0, 0.0]
+    # inside the domain the lookback pays S_T - min bgk > 0
+    if kind == fdmc.PAYOFF_LOOKBACK:
+        out = po.book_path(lvq(), k, np.array([0.3, -0.2, 0.1, 0.05], np.float32))
+        assert out[0] != 0.0 and all(math.isfinite(x) for x in out)
+
+
+@pytest.mark.parametrize("kind", [fdmc.PAYOFF_EUROPEAN, fdmc.PAYOFF_ASIAN, fdmc.PAYOFF_LOOKBACK])
+def test_book_pair_with_one_leg_outside_the_domain(kind):
+    """Fine leg out (its first step -6 takes S below 0), coarse leg in (its one
+    step is the sum -0.05): the pair is 0 - (coarse leg), price and Greeks, and
+    the coarse leg of a two-date contract is the one-date contract's only leg."""
+    k2, k1 = contract(kind, 2, K=90.0), contract(kind, 1, K=90.0)
+    pair = po.book_path(lvq(), k2, np.array([-6.0, 5.95], np.float32))
+    leg = po.book_path(lvq(), k1, np.array([np.float32(-6.0) + np.float32(5.95)], np.float32))
+    assert leg[0] > 0.0 and all(math.isfinite(x) and x != 0.0 for x in leg)
+    assert list(pair) == [-x for x in leg]
+
+
+@pytest.mark.parametrize("payoff", [fdmc.European(90.0, 0.05, 1.0), fdmc.Asian(90.0), fdmc.GeomAsian(90.0),
+                                    fdmc.Lookback(0.2), fdmc.Barrier(90.0, 130.0, 0.2, 0.05, 1.0)],
+                         ids=["euro", "asian", "geo", "look", "barrier"])
+def test_reference_paths_that_leave_the_domain_pay_zero(payoff):
+    P = po.make_params(lvq(), payoff, 4)
+    assert po.path_single(P, np.array([-6.0, 0.1, 0.2, -0.1], np.float32), 4) == 0.0
+    P1 = po.make_params(lvq(), payoff, 1)
+    assert po.path_single(P1, np.array([-6.0], np.float32), 1) == 0.0
+
+
+def _exit_fraction(a, b, T, n, P=100000, seed=5):
+    """Fraction of FP32 Euler paths of the quadratic local vol a + b e^{-t}
+    ln(S/100)^2 that reach S <= 0 (or NaN) within n steps."""
+    rng = np.random.default_rng(seed)
+    h = np.float32(T / n)
+    S = np.full(P, 100.0, np.float32)
+    out = np.zeros(P, bool)
+    with np.errstate(all="ignore"):
+        for j in range(n):
+            l = np.log(S / np.float32(100.0))
+            vol = np.float32(a) + np.float32(b) * np.float32(math.exp(-j * float(h))) * l * l
+            S = S + np.float32(0.05) * S * h + vol * S * (rng.standard_normal(P).astype(np.float32) * np.sqrt(h))
+            out |= ~(S > 0)
+    return out.mean()
+
+
+# a = 1 over four dates of T = 1: about 9 % of the fine paths (and more of the
+# coarser levels') leave the domain
+STEEP = dict(a=1.0, b=0.05, T=1.0, n=4)
+
+
+def test_steep_local_vol_book_is_finite():
+    """A book contract whose paths leave the domain often: the estimate and its
+    Greeks are finite and the stopping rule is met, in both differentiation
+    modes (the lookback's price was NaN before paths outside the domain paid 0)."""
+    assert _exit_fraction(STEEP["a"], STEEP["b"], STEEP["T"], STEEP["n"]) > 0.05
+    m = fdmc.LocalVolQuadParams(S0=100.0, mu=0.05, a=STEEP["a"], b=STEEP["b"], c=1.0, s_ref=100.0, T=1.0)
+    for kind in (fdmc.PAYOFF_LOOKBACK, fdmc.PAYOFF_EUROPEAN, fdmc.PAYOFF_ASIAN):
+        for greeks in (1, 2):
+            b = po.book_contract_run(m, contract(kind, STEEP["n"], T=STEEP["T"]), 0, 5e-2, cfg(greeks=greeks))
+            assert b.flags == 0 and b.std_error <= 5e-2 / math.sqrt(2) * (1 + 1e-9)
+            vals = (b.price, b.delta, b.vega, b.rho, b.delta_se, b.vega_se, b.rho_se)
+            assert all(math.isfinite(x) for x in vals), (kind, greeks, b)

@@ -136,6 +136,40 @@ def test_batched_and_plain_are_bit_identical_and_match_oracle(engine):
         b.free()
 
 
+def test_steep_local_vol_book_matches_oracle(engine):
+    """Vol level 1: on four dates over T = 1 about 9 % of the Euler paths step
+    to S <= 0 (more on the coarser levels), on 32 dates over T = 2 about 0.2 %.
+    Such a path pays 0 under every payoff and adds 0 to the Greeks, so the
+    estimates are finite, converge, and match the oracle's contract loop, in
+    the register (4 and 16 dates) and in the block (32 dates) kernels; the
+    tangent-mode book agrees with the adjoint one."""
+    m = fdmc.LocalVolQuadParams(S0=100.0, mu=0.05, a=1.0, b=0.05, c=1.0, s_ref=100.0, T=1.0)
+    book = [C(LB, 4), C(E, 4), C(A, 4), C(LB, 16), C(A, 32, T=2.0), C(LB, 32, T=2.0)]
+    eps = 0.1
+    ba = engine.book_create(m, book, fdmc.BookConfig(R=16, greeks=1))
+    bt = engine.book_create(m, book, fdmc.BookConfig(R=16, greeks=2))
+    try:
+        adj, _ = ba.run(eps, fdmc.BOOK_BATCHED)
+        tan, _ = bt.run(eps, fdmc.BOOK_BATCHED)
+    finally:
+        ba.free()
+        bt.free()
+    cfg = fdmc.BookConfig(R=16, greeks=1)
+    for c, (k, g, t) in enumerate(zip(book, adj, tan)):
+        o = po.book_contract_run(m, k, c, eps, cfg)
+        print("contract", c, g.price, o.price, g.std_error, g.n_samples, o.n_samples, flush=True)
+        for x in (g, t):
+            assert x.flags == 0 and x.std_error <= eps / math.sqrt(2) * (1 + 1e-9)
+            assert all(math.isfinite(v) for v in (x.price, x.delta, x.vega, x.rho, x.delta_se, x.vega_se, x.rho_se))
+        assert o.flags == 0
+        assert abs(g.price - o.price) <= 0.25 * g.std_error + 1e-5
+        assert abs(g.n_samples - o.n_samples) <= 0.02 * o.n_samples
+        for a_, b_, s_ in ((g.delta, o.delta, g.delta_se), (g.vega, o.vega, g.vega_se), (g.rho, o.rho, g.rho_se)):
+            assert abs(a_ - b_) <= 0.25 * s_ + 1e-4 * max(1.0, abs(b_)), (c, a_, b_, s_)
+        assert abs(t.price - g.price) <= 0.25 * g.std_error + 1e-5
+        assert abs(t.delta - g.delta) <= 0.25 * g.delta_se + 1e-4
+
+
 def test_gbm_european_contract_matches_black_scholes(engine):
     S0, mu, sig, T, K = 100.0, 0.05, 0.25, 1.0, 100.0
     m = fdmc.GBMParams(S0, mu, sig, T)

@@ -23,6 +23,7 @@ HES_MIL = fdmc.HestonParams(100.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04, 1.0,
                             scheme=fdmc.HESTON_MILSTEIN).compute_cholesky()
 DUP = fdmc.DupireLocalParams(100.0, 0.05, 0.2, 0.5, 0.7, 1.0)
 LVQ = fdmc.LocalVolQuadParams(100.0, 0.05, 0.2, 0.1, 1.0, 100.0, 1.0)
+LVQ_STEEP = fdmc.LocalVolQuadParams(100.0, 0.05, 1.0, 0.05, 1.0, 100.0, 1.0)   # many paths reach S <= 0
 EURO = fdmc.European(100.0, 0.05, 1.0)
 ASIAN = fdmc.Asian(100.0)
 LOOK = fdmc.Lookback(0.2)
@@ -45,8 +46,9 @@ def _check_sums(got, ref, n, barrier):
     (GBM, ASIAN, 2, (0, 1, 2, 5, 7)),
     (DUP, LOOK, 2, (1, 4)),
     (LVQ, EURO, 4, (2, 3)),
+    (LVQ_STEEP, LOOK, 2, (0, 1, 2)),
 ], ids=["heston-euro-M4", "heston-barrier-M4", "heston-barrier-M4-deep", "heston-milstein-euro-M4", "gbm-asian-M2", "dupire-look-M2",
-        "lvq-euro-M4"])
+        "lvq-euro-M4", "lvq-steep-look-M2"])
 def test_mlmc_level_sums_parity(engine, model, payoff, M, levels):
     reqs = [(l, 0, 1000 * l, 700 + 37 * l) for l in levels]   # unaligned ranges
     got = engine.levels_accumulate(model, payoff, M, reqs, qmc=False, salt_or_seed=42)

@@ -26,6 +26,9 @@ HES_MIL = fdmc.HestonParams(100.0, 0.05, 2.0, 0.04, 0.5, -0.7, 0.04, 1.0,
                             scheme=fdmc.HESTON_MILSTEIN).compute_cholesky()
 DUP = fdmc.DupireLocalParams(100.0, 0.05, 0.2, 0.5, 0.7, 1.0)
 LVQ = fdmc.LocalVolQuadParams(100.0, 0.05, 0.2, 0.1, 1.0, 100.0, 1.0)
+# vol level 1: about 9 % of the four-date Euler paths step to S <= 0, where the
+# local vol's log is NaN; such a path pays 0 under every payoff
+LVQ_STEEP = fdmc.LocalVolQuadParams(100.0, 0.05, 1.0, 0.05, 1.0, 100.0, 1.0)
 
 EURO = fdmc.European(100.0, 0.05, 1.0)
 ASIAN = fdmc.Asian(100.0)
@@ -56,6 +59,9 @@ QMC_CASES = [
     ("dupire-look-raw", DUP, LOOK, 16, fdmc.NOISE_RAW),
     ("lvq-asian-bb", LVQ, ASIAN, 128, fdmc.NOISE_BB),
     ("lvq-bar-bb", LVQ, BAR, 64, fdmc.NOISE_BB),
+    ("lvq-steep-look-raw", LVQ_STEEP, LOOK, 4, fdmc.NOISE_RAW),
+    ("lvq-steep-look-bb", LVQ_STEEP, LOOK, 8, fdmc.NOISE_BB),
+    ("lvq-steep-euro-bb", LVQ_STEEP, EURO, 4, fdmc.NOISE_BB),
 ]
 
 
