@@ -21,6 +21,7 @@
 //               (global_load_lds_dwordx4) four 32-deep k stages ahead of use.
 #include "k_path.inl"
 #include <algorithm>
+#include <type_traits>
 
 namespace fdmc {
 
@@ -48,6 +49,20 @@ __device__ __forceinline__ _Float16 to_half<_Float16>(float x) { return (_Float1
 // stores are 16-byte and row-contiguous.  49 KB of LDS: three workgroups per
 // CU.  Columns D..ld-1 are written as zeros.
 constexpr int GZ_TW = 72;   // staged row: 64 values + 8 of padding
+
+// icdf_u32_tab(s) with the sign applied by one v_bitop3: the table's magnitude
+// p >= 0 takes the sign bit where s has none (p | (~s & sign), truth table 0xF2
+// for sources p, s, sign), bit-identical to negating the scaled form.  As an
+// intrinsic the sign step is not merged across coordinates by the SLP
+// vectorizer, whose two-wide form of the negation the backend cannot select
+// when packed FP32 instructions are disabled.
+__device__ __forceinline__ float icdf_u32_tab_sgn(uint32_t s, const float4* tab) {
+    const uint32_t b = __float_as_uint(fmaf((float)(int32_t)s, 0x1p-32f, 0x1p-33f));
+    const float4 c = tab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
+    const float v = __uint_as_float((b & 0x7FFFFu) | 0x3F800000u);
+    const float p = fmaf(fmaf(fmaf(c.w, v, c.z), v, c.y), v, c.x);
+    return __uint_as_float(__builtin_amdgcn_bitop3_b32(__float_as_uint(p), s, 0x80000000u, 0xF2));
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir, int D, int ld, int64_t first,
@@ -90,16 +105,24 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
                 }
             }
             __syncthreads();                          // staging free again
+            // a full chunk takes a branch-free copy of the loop: under a
+            // per-coordinate (j < dn) test each table read sat in its own branch
+            // and was waited for alone; unconditional, the reads of a row of 8
+            // coordinates are in flight together
+            auto stage_row = [&](auto full) {
 #pragma unroll
-            for (int j0 = 0; j0 < 64; j0 += 8) {
-                union { T h[8]; uint4 u; } v;
+                for (int j0 = 0; j0 < 64; j0 += 8) {
+                    union { T h[8]; uint4 u; } v;
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const float z = (j0 + j < dn) ? icdf_u32_tab(cur[j0 + j], itab) : 0.0f;
-                    v.h[j] = to_half<T>(z);
+                    for (int j = 0; j < 8; j++) {
+                        const float z = (decltype(full)::value || j0 + j < dn) ? icdf_u32_tab_sgn(cur[j0 + j], itab) : 0.0f;
+                        v.h[j] = to_half<T>(z);
+                    }
+                    *(uint4*)(tile + tid * GZ_TW + j0) = v.u;
                 }
-                *(uint4*)(tile + tid * GZ_TW + j0) = v.u;
-            }
+            };
+            if (dn == 64) stage_row(std::true_type{});
+            else stage_row(std::false_type{});
             __syncthreads();
             // 256 rows x 64 values = 2048 chunks of 8 values
             const int64_t base = t << 8;
