@@ -43,11 +43,13 @@ __device__ __forceinline__ _Float16 to_half<_Float16>(float x) { return (_Float1
 // the scrambled Sobol words of its point in the chunk's 64 dimensions
 // (sobol_coord_lds); they stay in registers for the chunk's tiles: from tile n - 1
 // to tile n, gray(256 n) changes in bits 7 and 8 + ctz(n) only, so every word
-// takes one three-input XOR with two direction numbers, which are the same for
-// all lanes and come through scalar loads (no LDS read per coordinate but the
-// inverse-CDF table row).  The normals are staged through LDS so that global
-// stores are 16-byte and row-contiguous.  49 KB of LDS: three workgroups per
-// CU.  Columns D..ld-1 are written as zeros.
+// takes one XOR with v_7 ^ v_{8+ctz(n)} of its dimension, read from a table of
+// these rows that replaces the records in LDS once the words are formed (the
+// same for all lanes: broadcast reads).  The only other LDS read per
+// coordinate is its inverse-CDF table row, eight of them in flight.  The
+// normals are staged through LDS so that global stores are 16-byte and
+// row-contiguous.  49 KB of LDS: three workgroups per CU.  Columns D..ld-1 are
+// written as zeros.
 constexpr int GZ_TW = 72;   // staged row: 64 values + 8 of padding
 
 // icdf_u32_tab(s) with the sign applied by one v_bitop3: the table's magnitude
@@ -91,17 +93,40 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
         uint32_t cur[64];
 #pragma unroll
         for (int j = 0; j < 64; j++) cur[j] = (j < dn) ? sobol_coord_lds(sl, j) : 0u;
+        // The records are read; their LDS now holds the tile-to-tile updates
+        // dx[z][j] = v_7 ^ v_{8+z} of dimension d0 + j for z = ctz(n) < 16 (0 for
+        // the padding dimensions j >= dn), rows of 64 words that every lane
+        // reads alike (broadcast ds_read_b128); row 16 is filled when a tile
+        // has ctz(n) >= 16, row 17 stays 0 for the chunk's first tile.  One
+        // unconditional update per tile keeps cur[] in the same registers.
+        __syncthreads();
+        for (int i = tid; i < 18 * 64; i += 256) {
+            const int z = i >> 6, j = i & 63;
+            rec[i] = (j < dn && z < 16) ? gchunk[j * SDIR_WORDS + 32 + 7] ^ gchunk[j * SDIR_WORDS + 32 + 8 + z] : 0u;
+        }
+        __syncthreads();
         for (int64_t t = tfirst; t <= tend; t++) {
-            if (t > tfirst && dn > 0) {
-                const int c = 32 + 8 + __builtin_ctz((uint32_t)t);
-                if (dn == 64) {
+            int zr = 17;
+            if (t > tfirst) {
+                const int z = __builtin_ctz((uint32_t)t);   // t > tfirst >= 0, t < 2^24: (uint32_t)t != 0
+                zr = z < 16 ? z : 16;
+                if (z >= 16) {                            // uniform; row 16's last readers passed
+                    if (tid < 64)                         // the previous tile's barriers
+                        rec[16 * 64 + tid] = (tid < dn) ? gchunk[tid * SDIR_WORDS + 32 + 7] ^
+                                                              gchunk[tid * SDIR_WORDS + 32 + 8 + z]
+                                                        : 0u;
+                    __syncthreads();
+                }
+            }
+            {
+                const uint4* row = (const uint4*)(rec + zr * 64);
 #pragma unroll
-                    for (int j = 0; j < 64; j++)
-                        cur[j] = xor3(cur[j], gchunk[j * SDIR_WORDS + 32 + 7], gchunk[j * SDIR_WORDS + c]);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < 64; j++)
-                        if (j < dn) cur[j] = xor3(cur[j], gchunk[j * SDIR_WORDS + 32 + 7], gchunk[j * SDIR_WORDS + c]);
+                for (int q = 0; q < 16; q++) {
+                    const uint4 x = row[q];
+                    cur[4 * q] ^= x.x;
+                    cur[4 * q + 1] ^= x.y;
+                    cur[4 * q + 2] ^= x.z;
+                    cur[4 * q + 3] ^= x.w;
                 }
             }
             __syncthreads();                          // staging free again
@@ -113,10 +138,27 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
 #pragma unroll
                 for (int j0 = 0; j0 < 64; j0 += 8) {
                     union { T h[8]; uint4 u; } v;
+                    if constexpr (decltype(full)::value) {
+                        float4 c[8];
+                        float vv[8];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const float z = (decltype(full)::value || j0 + j < dn) ? icdf_u32_tab_sgn(cur[j0 + j], itab) : 0.0f;
-                        v.h[j] = to_half<T>(z);
+                        for (int j = 0; j < 8; j++) {
+                            const uint32_t b = __float_as_uint(fmaf((float)(int32_t)cur[j0 + j], 0x1p-32f, 0x1p-33f));
+                            c[j] = itab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
+                            vv[j] = __uint_as_float((b & 0x7FFFFu) | 0x3F800000u);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 8; j++) {
+                            const float p = fmaf(fmaf(fmaf(c[j].w, vv[j], c[j].z), vv[j], c[j].y), vv[j], c[j].x);
+                            v.h[j] = to_half<T>(__uint_as_float(
+                                __builtin_amdgcn_bitop3_b32(__float_as_uint(p), cur[j0 + j], 0x80000000u, 0xF2)));
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 8; j++) {
+                            const float z = (j0 + j < dn) ? icdf_u32_tab_sgn(cur[j0 + j], itab) : 0.0f;
+                            v.h[j] = to_half<T>(z);
+                        }
                     }
                     *(uint4*)(tile + tid * GZ_TW + j0) = v.u;
                 }

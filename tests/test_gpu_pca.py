@@ -320,18 +320,24 @@ def test_pca_products_rejects_bad_shapes(engine, torch_cuda, c3):
     small.free()
 
 
+@pytest.mark.parametrize("off", [256 * 7 + 100, 65525 * 256 + 37, ((1 << 23) - 11) * 256 + 37],
+                         ids=["low", "tile-65536", "tile-2^23"])
 @pytest.mark.parametrize("bf16", [False, True], ids=["fp16", "bf16"])
-def test_pca_normals_are_the_oracle_points(engine, torch_cuda, bf16):
+def test_pca_normals_are_the_oracle_points(engine, torch_cuda, bf16, off):
     """The GEMM's A operand itself: with M = I the transform returns the
     half-precision normals unchanged (products with 1, sums with 0), so they
     are compared with the oracle's scrambled-Sobol normals rounded the same
     way, row for row.  786,688 points are 3,073 tiles, more than the 3,072
     workgroups k_gen_z launches, so workgroups run two tiles and the rows
     checked include second tiles (Sobol words advanced in registers from the
-    first tile, two direction numbers per dimension); D = 160 has a partial
-    64-dimension chunk and a padding chunk (mp = 256)."""
+    first tile by the XOR of two direction numbers per dimension, v_7 and
+    v_{8 + ctz(n)} of tile n); D = 160 has a partial 64-dimension chunk and a
+    padding chunk (mp = 256).  The two high offsets start at an odd tile, so
+    that tile 2^16 (points 2^24..) and tile 2^23 (points 2^31..) are second
+    tiles (ctz 16 and 23, the update rows that are not precomputed); rows
+    2779.. are those tiles."""
     torch = torch_cuda
-    D, npts, off, salt = 160, 3073 * 256, 256 * 7 + 100, 77
+    D, npts, salt = 160, 3073 * 256, 77
     pca = engine.pca_upload(D, np.eye(D).ravel())
     out = torch.empty((npts, D), dtype=torch.float32, device="cuda")
     engine.pca_transform(pca, salt, off, npts, bf16, out.data_ptr())
@@ -339,7 +345,7 @@ def test_pca_normals_are_the_oracle_points(engine, torch_cuda, bf16):
     q = (lambda a: _round_bf16(a)) if bf16 else (lambda a: a.astype(np.float16).astype(np.float32))
     ulp = 2.0 ** (-7 if bf16 else -10)
     # rows in first and second tiles of several workgroups, and the last rows
-    for r0 in (0, 156, 256 * 1 + 156, 256 * 1000 + 156, 256 * 2047 + 156, npts - 256):
+    for r0 in (0, 156, 256 * 1 + 156, 2779, 256 * 1000 + 156, 256 * 2047 + 156, npts - 256):
         got = out[r0:r0 + 256].cpu().numpy()
         ref = q(O.sobol_normal_block(salt, D, off + r0, 256).T)
         diff = np.abs(got - ref)
