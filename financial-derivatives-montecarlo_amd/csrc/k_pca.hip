@@ -67,7 +67,7 @@ __device__ __forceinline__ float icdf_u32_tab_sgn(uint32_t s, const float4* tab)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir, int D, int ld, int64_t first,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_gen_z(const uint32_t* __restrict__ sdir, int D, int ld, int64_t first,
                                                int64_t count, int tpw, T* __restrict__ out) {
     __shared__ __align__(16) float4 itab[FDMC_ICDF_TAB_ROWS];
     __shared__ __align__(16) uint32_t rec[64 * 20];
@@ -166,16 +166,24 @@ __global__ __launch_bounds__(256) void k_gen_z(const uint32_t* __restrict__ sdir
             if (dn == 64) stage_row(std::true_type{});
             else stage_row(std::false_type{});
             __syncthreads();
-            // 256 rows x 64 values = 2048 chunks of 8 values
-            const int64_t base = t << 8;
-            for (int c = tid; c < 2048; c += 256) {
-                const int row = c >> 3, q = (c & 7) * 8;
-                const int64_t p = base + row;
-                if (p >= first && p < first + count && d0 + q < ld) {
-                    const int64_t orow = (int64_t)r * count + (p - first);
-                    *(uint4*)(out + orow * ld + d0 + q) = *(const uint4*)(tile + row * GZ_TW + q);
+            // 256 rows x 64 values = 2048 chunks of 8 values: thread tid stores
+            // chunk (tid & 7) of rows tid / 8 + 32 k (d0 + 64 <= ld: no column
+            // test).  A tile inside [first, first + count) takes a copy of the
+            // loop without the row test, whose reads and stores are not held
+            // in per-row branches.
+            const int64_t rel = (t << 8) - first;        // the tile's first row in the output
+            const int row0 = tid >> 3, q = (tid & 7) * 8;
+            T* ob = out + ((int64_t)r * count + rel + row0) * ld + d0 + q;
+            auto store_rows = [&](auto inner) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int64_t pr = rel + row0 + 32 * k;
+                    if (decltype(inner)::value || (pr >= 0 && pr < count))
+                        *(uint4*)(ob + (int64_t)32 * k * ld) = *(const uint4*)(tile + (row0 + 32 * k) * GZ_TW + q);
                 }
-            }
+            };
+            if (rel >= 0 && rel + 256 <= count) store_rows(std::true_type{});
+            else store_rows(std::false_type{});
         }
     }
 }
