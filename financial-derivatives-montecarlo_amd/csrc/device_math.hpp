@@ -46,6 +46,26 @@ __device__ __forceinline__ void icdf_tab_load(float4* lds, int tid, int nthreads
     for (int i = tid; i < FDMC_ICDF_TAB_ROWS; i += nthreads) lds[i] = ((const float4*)g_icdf_tab)[i];
 }
 
+// The table row of encoding b: bits 19..31 of b minus FDMC_ICDF_ROW0 rows.  A
+// kernel whose table sits at least FDMC_ICDF_ROW0 rows (24,064 bytes) into LDS
+// gets a shift and a mask, the constant folding into the ds_read's offset
+// field (BFE = false).  Below that address the constant is negative, which
+// that unsigned field cannot hold, and the compiler adds it separately: a
+// third instruction.  BFE = true costs two instructions wherever the table
+// sits: a bit-field extract, written as the instruction itself (as a builtin
+// the compiler turns it back into a shift and a mask), and one shift-and-add
+// onto the table's base minus the constant.
+template <bool BFE>
+__device__ __forceinline__ float4 icdf_row(const float4* tab, uint32_t b) {
+    if constexpr (BFE) {
+        uint32_t row;
+        asm("v_bfe_u32 %0, %1, 19, 12" : "=v"(row) : "v"(b));
+        return *(const float4*)((const char*)(tab - FDMC_ICDF_ROW0) + (row << 4));
+    } else {
+        return tab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
+    }
+}
+
 // -sd Phi^-1((s + 1/2) 2^-32) from the LDS table, for a scale sd > 0.  The
 // fold to the lower half comes from reading s as a signed integer i:
 // v = i 2^-32 + 2^-33 has |v| = q for both halves, and the row index and the
@@ -53,17 +73,19 @@ __device__ __forceinline__ void icdf_tab_load(float4* lds, int tid, int nthreads
 // (Within 64 of u = 1/2, FP32 rounds q to exactly 1/2, whose row gives g = 0.)
 // The result takes its sign bit from s with one OR, and callers fold the
 // negation into the sum they add it to.
+template <bool BFE = false>
 __device__ __forceinline__ float icdf_u32_tab_scaled_neg(uint32_t s, const float4* tab, float sd) {
     const uint32_t b = __float_as_uint(fmaf((float)(int32_t)s, 0x1p-32f, 0x1p-33f));
-    const float4 c = tab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
+    const float4 c = icdf_row<BFE>(tab, b);
     const float v = __uint_as_float((b & 0x7FFFFu) | 0x3F800000u);
     const float p = fmaf(fmaf(fmaf(c.w, v, c.z), v, c.y), v, c.x);
     const float y = sd * p;                                // sd g(q) >= 0
     return __uint_as_float(__float_as_uint(y) | (s & 0x80000000u));
 }
 
+template <bool BFE = false>
 __device__ __forceinline__ float icdf_u32_tab(uint32_t s, const float4* tab) {
-    return -icdf_u32_tab_scaled_neg(s, tab, 1.0f);
+    return -icdf_u32_tab_scaled_neg<BFE>(s, tab, 1.0f);
 }
 
 // a ^ b ^ c in one VALU instruction: gfx950's three-input bitwise operation

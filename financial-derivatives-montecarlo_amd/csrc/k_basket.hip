@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_basket(BasketLaunch b) {
                 float z[NA];
                 if constexpr (SRC == SRC_SOBOL) {
 #pragma unroll
-                    for (int a = 0; a < NA; a++) z[a] = (a < n) ? icdf_u32_tab(sobol_coord(sl, k * n + a), itab) * b.sqrt_h : 0.0f;
+                    for (int a = 0; a < NA; a++) z[a] = (a < n) ? icdf_u32_tab<true>(sobol_coord(sl, k * n + a), itab) * b.sqrt_h : 0.0f;
                 } else {
 #pragma unroll
                     for (int a0 = 0; a0 < NA; a0 += 4) {
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(WIDE_WG) void k_basket_wide(BasketWide b) {
                 float S = b.S0[a];
                 const float sg = (MK == MK_MULTI_GBM) ? b.sig[a] : 0.0f;
                 for (int k = 0; k < b.n_steps; k++) {
-                    const float z = icdf_u32_tab(sobol_coord(sl, k * n + a), itab);
+                    const float z = icdf_u32_tab<true>(sobol_coord(sl, k * n + a), itab);
                     S = wide_step<MK>(b, S, sg, z * b.sqrt_h, k);
                 }
                 B += b.w[a] * S;
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
                     if constexpr (SRC == SRC_SOBOL) {
 #pragma unroll
                         for (int t = 0; t < 4; t++)
-                            if (c0 + t < n) za[t] = icdf_u32_tab(sobol_coord_glb(sl, jk + c0 + t), itab) * b.sqrt_h;
+                            if (c0 + t < n) za[t] = icdf_u32_tab<true>(sobol_coord_glb(sl, jk + c0 + t), itab) * b.sqrt_h;
                     } else {
                         // normals jk + c0 + t of the path, in groups of four from
                         // index 4 g (the offset jk & 3 is the same for every lane)

@@ -123,17 +123,7 @@ __device__ __forceinline__ void book_step(const LegParams& P, float& S, Tan3& t,
 constexpr int BOOK_SHORT_L = 4;
 constexpr int BOOK_SHORT_D = 1 << BOOK_SHORT_L;
 
-// LDS slot of bridge dimension d on a level of G >= 2 blocks of 16 dates: the
-// skeleton's dimensions (d < G) keep their index, and the 15 in-block
-// dimensions of block i (G 2^lv + 2^lv i + q at in-block depth lv) sit together
-// at slots G + 15 i + 2^lv - 1 + q, so one block's records are read from one
-// base address with immediate offsets.
-__device__ __forceinline__ int book_slot(int d, int G) {
-    if (d < G) return d;
-    const int lv = 31 - __builtin_clz((uint32_t)d / (uint32_t)G);
-    const int r = d - (G << lv);
-    return G + 15 * (r >> lv) + (1 << lv) - 1 + (r & ((1 << lv) - 1));
-}
+// Records of a level of G >= 2 blocks of 16 dates sit at block_slot(d, G).
 
 // Greeks carried as forward tangents along the path (GM = 2, payoffs with a
 // pathwise derivative).
@@ -425,7 +415,7 @@ __device__ __forceinline__ void book_path_long(const LegParams& P, const BookGeo
     // d/2 + y; dc receives the two-date intervals' increments (coarse steps)
     auto gen_block = [&](int i, float WL, float WR, float (&dw)[16], float (&dc)[8]) {
         dw[0] = WR - WL;
-        const int slot0 = G + 15 * i;       // the block's records (book_slot)
+        const int slot0 = G + 15 * i;       // the block's records (block_slot)
 #pragma unroll
         for (int lv = 0; lv < 4; lv++) {
             if (lv == 3) {
@@ -714,12 +704,12 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
         // numbers of bits 7..31 in LDS after the 16 records (hl[d][25]).
         uint32_t* hl = rec + BOOK_SHORT_D * 20;
         // records in LDS slots: identity for levels of at most 16 dates,
-        // book_slot (block-contiguous) above
+        // block_slot (block-contiguous) above
         const int Gs = E.l > BOOK_SHORT_L ? g.G : D;
         if (lo != last_e) {
             for (int i = tid; i < D * 16; i += 256) {
                 const int d = i >> 4, k = i & 15;
-                rec[book_slot(d, Gs) * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
+                rec[block_slot(d, Gs) * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
             }
             if (E.l <= BOOK_SHORT_L)
                 for (int i = tid; i < D * 25; i += 256) hl[i] = grec[(int64_t)(i / 25) * SDIR_WORDS + 39 + i % 25];
@@ -732,7 +722,7 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
                 uint32_t s = 0;
 #pragma unroll
                 for (int j = 7; j < 32; j++) s ^= ((gbase >> j) & 1u) ? gr[32 + j] : 0u;
-                *(uint4*)(rec + book_slot(d, Gs) * 20 + 16) = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
+                *(uint4*)(rec + block_slot(d, Gs) * 20 + 16) = make_uint4(s ^ gr[16], s ^ gr[17], s ^ gr[18], s ^ gr[19]);
             }
         }
         __syncthreads();
@@ -755,7 +745,7 @@ __device__ __forceinline__ void book_body(const BookLaunch& a) {
                     for (int d = tid; d < D; d += 256) {
                         const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
                         const uint32_t dl = gr[32 + 7] ^ gr[32 + 8 + c];
-                        uint4* r = (uint4*)(rec + book_slot(d, Gs) * 20 + 16);
+                        uint4* r = (uint4*)(rec + block_slot(d, Gs) * 20 + 16);
                         const uint4 v = *r;
                         *r = make_uint4(v.x ^ dl, v.y ^ dl, v.z ^ dl, v.w ^ dl);
                     }

@@ -93,12 +93,9 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
         sl.hi_gray = 0;
         sl.lds_mode = lds_mode;
     }
-    if (NZ != NZ_PRNG && lds_mode) {
-        for (int i = tid; i < D * 16; i += 256) {
-            const int d = i >> 4, k = i & 15;
-            rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
-        }
-    }
+    // a uniform bridge of 16-date blocks reads its records block-contiguous
+    const int gslot = (NZ == NZ_SOBOL_BB && B == 16 && a.bb_uniform) ? G : 0;
+    if (NZ != NZ_PRNG && lds_mode) rec_fill(rec, grec, D, gslot, tid);
     double acc[NACC];
 #pragma unroll
     for (int i = 0; i < NACC; i++) acc[i] = 0.0;
@@ -118,7 +115,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
             const uint32_t b32 = (uint32_t)base;
             if (lds_mode) {
                 __syncthreads();
-                sobol_tile_words(rec, grec, D, (uint32_t)(tile0 + it), it == 0, tid);
+                sobol_tile_words(rec, grec, D, (uint32_t)(tile0 + it), it == 0, tid, gslot);
                 __syncthreads();
             } else {
                 const uint32_t hi = b32 + (sl.w << 6);
@@ -152,7 +149,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                     if constexpr (B == 1) {
                         dw[0] = sk[(i + 1) * 256] - sk[i * 256];
                     } else if constexpr (UNI) {
-                        bb_block_uniform<true>(a, sl, itab, G, lgG, i, sk[i * 256], sk[(i + 1) * 256], dw);
+                        bb_block_uniform<true, true>(a, sl, itab, G, lgG, i, sk[i * 256], sk[(i + 1) * 256], dw);
                     } else {
                         float W[B + 1];
                         W[0] = sk[i * 256];
@@ -174,7 +171,7 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
     #pragma unroll
                     for (int k = 0; k < B; k++) {
                         const int g = i * B + k;
-                        dw[k] = (g < n) ? icdf_u32_tab(sobol_coord(sl, g), itab) * a.sqrt_h : 0.0f;
+                        dw[k] = (g < n) ? icdf_u32_tab<true>(sobol_coord(sl, g), itab) * a.sqrt_h : 0.0f;
                     }
                 } else {
     #pragma unroll

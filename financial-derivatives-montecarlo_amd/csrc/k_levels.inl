@@ -171,9 +171,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         } else if constexpr (NZ == NZ_SOBOL_RAW) {
             if constexpr (is_heston(MK)) {
                 for (int k = 0; k < nfine; k++)
-                    cp.fineH(P, E, icdf_u32_tab(sobol_coord(sl, 2 * k), itab), icdf_u32_tab(sobol_coord(sl, 2 * k + 1), itab), M);
+                    cp.fineH(P, E, icdf_u32_tab<true>(sobol_coord(sl, 2 * k), itab), icdf_u32_tab<true>(sobol_coord(sl, 2 * k + 1), itab), M);
             } else {
-                for (int k = 0; k < nfine; k++) cp.fine1(P, E, icdf_u32_tab(sobol_coord(sl, k), itab) * E.sqrt_hf, k, M);
+                for (int k = 0; k < nfine; k++) cp.fine1(P, E, icdf_u32_tab<true>(sobol_coord(sl, k), itab) * E.sqrt_hf, k, M);
             }
         } else {  // Brownian bridge over n_fine steps (one-factor models)
             const float4* bw = (const float4*)E.bbw;
@@ -181,13 +181,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             // W_M of dyadic node j from the ends of its interval, bb_precompute's weights
             auto mid = [&](float WL, float WR, int j) {
                 const float4 c = bw[j];
-                return c.x * WL + c.y * WR + c.z * icdf_u32_tab(sobol_coord(sl, j), itab);
+                return c.x * WL + c.y * WR + c.z * icdf_u32_tab<true>(sobol_coord(sl, j), itab);
             };
             if (LB == 0 || nfine < 16) {
                 // fewer than 16 dates: the whole skeleton in LDS, one step per entry
                 float* sk = skel + tid;
                 sk[0] = 0.0f;
-                sk[G * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
+                sk[G * 256] = bw[0].z * icdf_u32_tab<true>(sobol_coord(sl, 0), itab);
                 for (int j = 1; j < G; j++) {
                     const int lev = 31 - __builtin_clz(j);
                     const int q = j - (1 << lev);
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                     return mid(WL, WR, (1 << (m - 1 - k)) + (L >> (k + 1)));
                 };
                 float Wp = 0.0f;
-                rt[m * 256] = bw[0].z * icdf_u32_tab(sobol_coord(sl, 0), itab);
+                rt[m * 256] = bw[0].z * icdf_u32_tab<true>(sobol_coord(sl, 0), itab);
                 for (int k = m - 1; k >= 0; k--) rt[k * 256] = node(k, 0, Wp, rt[(k + 1) * 256]);
                 for (int i = 0; i < G; i++) {
                     if (i > 0) {

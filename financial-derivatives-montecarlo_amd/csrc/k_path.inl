@@ -69,12 +69,34 @@ __device__ __forceinline__ uint32_t sobol_coord(const SobolLane& s, int d) {
 // sum (its loads are independent); otherwise the records hold tile n - 1, and
 // gray(256 n) differs from gray(256 (n - 1)) in bits 7 and 8 + ctz(n) only, so
 // two direction numbers are XORed in.  Threads d = tid, tid + 256, ... < D.
+// LDS slot of bridge dimension d on a uniform bridge of G >= 1 blocks of 16
+// dates: the skeleton's dimensions (d < G) keep their index, and the 15
+// in-block dimensions of block i (G 2^lv + 2^lv i + q at in-block depth lv) sit
+// together at slots G + 15 i + 2^lv - 1 + q, so one block's records are read
+// from one base address with immediate offsets.
+__device__ __forceinline__ int block_slot(int d, int G) {
+    if (d < G) return d;
+    const int lv = 31 - __builtin_clz((uint32_t)d / (uint32_t)G);
+    const int r = d - (G << lv);
+    return G + 15 * (r >> lv) + (1 << lv) - 1 + (r & ((1 << lv) - 1));
+}
+
+// The direction records of dimensions 0..D-1 into LDS, at block_slot(d, G)
+// for G > 0, at d for G = 0.
+__device__ __forceinline__ void rec_fill(uint32_t* rec, const uint32_t* grec, int D, int G, int tid) {
+    for (int i = tid; i < D * 16; i += 256) {
+        const int d = i >> 4, k = i & 15;
+        rec[(G > 0 ? block_slot(d, G) : d) * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
+    }
+}
+
+// G > 0: records at block_slot(d, G) (rec_fill with the same G).
 __device__ __forceinline__ void sobol_tile_words(uint32_t* rec, const uint32_t* grec, int D, uint32_t n, bool full,
-                                                 int tid) {
+                                                 int tid, int G = 0) {
     const uint32_t b32 = n << 8, gbase = b32 ^ (b32 >> 1);
     for (int d = tid; d < D; d += 256) {
         const uint32_t* gr = grec + (int64_t)d * SDIR_WORDS;
-        uint4* rw = (uint4*)(rec + d * 20 + 16);
+        uint4* rw = (uint4*)(rec + (G > 0 ? block_slot(d, G) : d) * 20 + 16);
         if (full) {
             uint32_t s = 0;
 #pragma unroll
@@ -106,18 +128,18 @@ __device__ __forceinline__ float bb_mid(const PathLaunch& a, const SobolLane& sl
                                         float WR, int j, int depth) {
     if constexpr (UNI) {
         const float sd = a.bb_sd[depth];
-        return 0.5f * (WL + WR) - icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j), itab, sd);
+        return 0.5f * (WL + WR) - icdf_u32_tab_scaled_neg<true>(bb_coord<LDS>(sl, j), itab, sd);
     } else {
         const float4 c = ((const float4*)a.bbw)[j];
-        return c.x * WL + c.y * WR + c.z * icdf_u32_tab(bb_coord<LDS>(sl, j), itab);
+        return c.x * WL + c.y * WR + c.z * icdf_u32_tab<true>(bb_coord<LDS>(sl, j), itab);
     }
 }
 
 // W at the last date: sd sqrt(T) times coordinate 0's normal.
 template <bool UNI, bool LDS>
 __device__ __forceinline__ float bb_terminal(const PathLaunch& a, const SobolLane& sl, const float4* itab) {
-    if constexpr (UNI) return -icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, 0), itab, a.bb_sd[0]);
-    else return ((const float4*)a.bbw)[0].z * icdf_u32_tab(bb_coord<LDS>(sl, 0), itab);
+    if constexpr (UNI) return -icdf_u32_tab_scaled_neg<true>(bb_coord<LDS>(sl, 0), itab, a.bb_sd[0]);
+    else return ((const float4*)a.bbw)[0].z * icdf_u32_tab<true>(bb_coord<LDS>(sl, 0), itab);
 }
 
 // The 16 increments of block i of a uniform-grid bridge (weights 1/2, one sd
@@ -130,17 +152,20 @@ __device__ __forceinline__ float bb_terminal(const PathLaunch& a, const SobolLan
 // the 16 increments come out 16 times their size, and the steps take them with
 // the vol parameters times 1/16 (vol_scaled: a power of two, so sigma dw is the
 // same product).
-template <bool LDS>
+// SLOT: the records are at block_slot(j, G) (LDS only), so the block's 15
+// nodes are read from one base.
+template <bool LDS, bool SLOT>
 __device__ __forceinline__ void bb_block_uniform(const PathLaunch& a, const SobolLane& sl, const float4* itab, int G,
                                                  int lgG, int i, float WL, float WR, float (&dw)[16]) {
+    static_assert(LDS || !SLOT, "slots are an LDS layout");
     dw[0] = WR - WL;
 #pragma unroll
     for (int l = 0; l < 4; l++) {
-        const int j0 = (G << l) + (i << l);
+        const int j0 = SLOT ? G + 15 * i + (1 << l) - 1 : (G << l) + (i << l);
         const float sds = a.bb_sd[lgG + l + 1] * (float)(2 << l);
 #pragma unroll
         for (int q = (1 << l) - 1; q >= 0; q--) {
-            const float y = icdf_u32_tab_scaled_neg(bb_coord<LDS>(sl, j0 + q), itab, 1.0f);
+            const float y = icdf_u32_tab_scaled_neg<true>(bb_coord<LDS>(sl, j0 + q), itab, 1.0f);
             const float d = dw[q];
             dw[2 * q] = fmaf(-sds, y, d);
             dw[2 * q + 1] = fmaf(sds, y, d);
@@ -247,12 +272,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         sl.hi_gray = 0;
         sl.lds_mode = lds_mode;
     }
-    if (SOBOL && lds_mode) {
-        for (int i = tid; i < D * 16; i += 256) {
-            const int d = i >> 4, k = i & 15;
-            rec[d * 20 + k] = grec[(int64_t)d * SDIR_WORDS + k];
-        }
-    }
+    // a uniform bridge of 16-date blocks reads its records block-contiguous
+    const int gslot = (NZ == NZ_SOBOL_BB && LB == 4 && a.bb_uniform) ? a.G : 0;
+    if (SOBOL && lds_mode) rec_fill(rec, grec, D, gslot, tid);
 
     double acc[NACC];
 #pragma unroll
@@ -273,7 +295,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const uint32_t b32 = (uint32_t)base;
             if (lds_mode) {
                 __syncthreads();
-                sobol_tile_words(rec, grec, D, (uint32_t)(tile0 + it), it == 0, tid);
+                sobol_tile_words(rec, grec, D, (uint32_t)(tile0 + it), it == 0, tid, gslot);
                 __syncthreads();
             } else {
                 const uint32_t hi = b32 + (sl.w << 6);
@@ -316,13 +338,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         } else if constexpr (NZ == NZ_SOBOL_RAW) {
             if constexpr (is_heston(MK)) {
                 for (int k = 0; k < a.n_steps; k++) {
-                    const float d1 = icdf_u32_tab(sobol_coord(sl, 2 * k), itab) * a.sqrt_h;
-                    const float d2 = icdf_u32_tab(sobol_coord(sl, 2 * k + 1), itab) * a.sqrt_h;
+                    const float d1 = icdf_u32_tab<true>(sobol_coord(sl, 2 * k), itab) * a.sqrt_h;
+                    const float d2 = icdf_u32_tab<true>(sobol_coord(sl, 2 * k + 1), itab) * a.sqrt_h;
                     ps.stepH(a, d1, a.leg[0].L10 * d1 + a.leg[0].L11 * d2);
                 }
             } else {
                 for (int k = 0; k < a.n_steps; k++)
-                    ps.step(a, icdf_u32_tab(sobol_coord(sl, k), itab) * a.sqrt_h, k);
+                    ps.step(a, icdf_u32_tab<true>(sobol_coord(sl, k), itab) * a.sqrt_h, k);
             }
         } else {  // NZ_SOBOL_BB, NZ_SOBOL_BB_GLOBAL (one-factor models)
             // W_M from the ends of its interval and coordinate j's normal: with
@@ -383,7 +405,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                         }
                         if constexpr (UNI && B == 16) {
                             float dw[16];
-                            bb_block_uniform<LDS>(a, sl, itab, G, lgG, i, Wp, rt[0], dw);
+                            bb_block_uniform<LDS, LDS>(a, sl, itab, G, lgG, i, Wp, rt[0], dw);
 #pragma unroll
                             for (int k = 0; k < B; k++) ps.step(a16, dw[k], i * B + k);
                         } else {

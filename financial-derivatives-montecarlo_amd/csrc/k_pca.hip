@@ -60,7 +60,7 @@ constexpr int GZ_TW = 72;   // staged row: 64 values + 8 of padding
 // when packed FP32 instructions are disabled.
 __device__ __forceinline__ float icdf_u32_tab_sgn(uint32_t s, const float4* tab) {
     const uint32_t b = __float_as_uint(fmaf((float)(int32_t)s, 0x1p-32f, 0x1p-33f));
-    const float4 c = tab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
+    const float4 c = icdf_row<false>(tab, b);
     const float v = __uint_as_float((b & 0x7FFFFu) | 0x3F800000u);
     const float p = fmaf(fmaf(fmaf(c.w, v, c.z), v, c.y), v, c.x);
     return __uint_as_float(__builtin_amdgcn_bitop3_b32(__float_as_uint(p), s, 0x80000000u, 0xF2));
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
                         for (int j = 0; j < 8; j++) {
                             const uint32_t b = __float_as_uint(fmaf((float)(int32_t)cur[j0 + j], 0x1p-32f, 0x1p-33f));
-                            c[j] = itab[((b >> 19) & 0xFFFu) - FDMC_ICDF_ROW0];
+                            c[j] = icdf_row<false>(itab, b);
                             vv[j] = __uint_as_float((b & 0x7FFFFu) | 0x3F800000u);
                         }
 #pragma unroll
