@@ -298,7 +298,12 @@ __global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int col = lane & 15, ks = lane >> 4;
     const int r = blockIdx.y;
-    const int n = b.n, nb = (n + 15) >> 4;           // nb <= NB (checked on the host)
+    // NB = ceil(n / 16) exactly (the host picks the instantiation): the block
+    // loops below have compile-time bounds, with no branch between the MFMAs
+    // (uniform early exits there made the compiler shuttle the accumulators
+    // between AGPRs and VGPRs at every join, about ten moves per MFMA)
+    const int n = b.n;
+    constexpr int nb = NB;
     // B fragments: lane (ks, col) of pair (q, j) holds L[16 j + col][16 q + 4 ks + t]
     // for t = 0..3, zero above the diagonal and outside n
     constexpr int NPAIRS = NB * (NB + 1) / 2;
@@ -356,22 +361,33 @@ __global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
                             if (c0 + t < n) za[t] = icdf_u32_tab<true>(sobol_coord_glb(sl, jk + c0 + t), itab) * b.sqrt_h;
                     } else {
                         // normals jk + c0 + t of the path, in groups of four from
-                        // index 4 g (the offset jk & 3 is the same for every lane)
-                        const int j0 = jk + c0, off = j0 & 3;
-                        float g0[4], g1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                        // index 4 g; the offset jk & 3 is the same for every lane
+                        // (c0 is a multiple of 4), so it is computed from uniform
+                        // values only and its test is a scalar branch: aligned
+                        // (always so when n is a multiple of 4) one call and no
+                        // selection
+                        const int j0 = jk + c0, off = jk & 3;
+                        float g0[4];
                         prng_normals4(b.seed + (uint32_t)r, b.tag, pa, (uint32_t)(j0 >> 2), g0);
-                        if (off) prng_normals4(b.seed + (uint32_t)r, b.tag, pa, (uint32_t)(j0 >> 2) + 1u, g1);
+                        if (off == 0) {
 #pragma unroll
-                        for (int t = 0; t < 4; t++) {
-                            const int u = off + t;
-                            float z = g0[0];
-                            z = (u == 1) ? g0[1] : z;
-                            z = (u == 2) ? g0[2] : z;
-                            z = (u == 3) ? g0[3] : z;
-                            z = (u == 4) ? g1[0] : z;
-                            z = (u == 5) ? g1[1] : z;
-                            z = (u == 6) ? g1[2] : z;
-                            if (c0 + t < n) za[t] = z * b.sqrt_h;
+                            for (int t = 0; t < 4; t++)
+                                if (c0 + t < n) za[t] = g0[t] * b.sqrt_h;
+                        } else {
+                            float g1[4];
+                            prng_normals4(b.seed + (uint32_t)r, b.tag, pa, (uint32_t)(j0 >> 2) + 1u, g1);
+#pragma unroll
+                            for (int t = 0; t < 4; t++) {
+                                const int u = off + t;
+                                float z = g0[0];
+                                z = (u == 1) ? g0[1] : z;
+                                z = (u == 2) ? g0[2] : z;
+                                z = (u == 3) ? g0[3] : z;
+                                z = (u == 4) ? g1[0] : z;
+                                z = (u == 5) ? g1[1] : z;
+                                z = (u == 6) ? g1[2] : z;
+                                if (c0 + t < n) za[t] = z * b.sqrt_h;
+                            }
                         }
                     }
                 }
@@ -388,6 +404,10 @@ __global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
                         const float bv = t == 0 ? bf[j].x : t == 1 ? bf[j].y : t == 2 ? bf[j].z : bf[j].w;
                         dw[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(za[t], bv, dw[j], 0, 0, 0);
                     }
+                // block q's normals are drawn next to its MFMAs: hoisting every
+                // block's Philox draws ahead of the products (the scheduler's
+                // choice) doubles the registers and halves the waves per SIMD
+                __builtin_amdgcn_sched_barrier(0);
             }
             // Euler step of every (path, asset) the lane holds (kernel_multi_dupire's
             // S + mu S h + sigma S dW, methods_cuda.cu:526-551)
@@ -437,15 +457,28 @@ __global__ __launch_bounds__(256) void k_basket_corr(BasketWide b) {
     }
 }
 
-// Column blocks of 16 assets the correlated kernel is compiled for: n <= 16 NB.
-static int corr_nb(int n) { return n <= 64 ? 4 : n <= 128 ? 8 : 16; }
+// Column blocks of 16 assets the correlated kernel is compiled for: exactly
+// ceil(n / 16), 3..16 for the 33..256 assets of this path.
+static int corr_nb(int n) { return (n + 15) >> 4; }
 
 template <int MK, int SRC>
 static const void* pick_corr(int n) {
     switch (corr_nb(n)) {
+    case 3: return (const void*)&k_basket_corr<MK, SRC, 3>;
     case 4: return (const void*)&k_basket_corr<MK, SRC, 4>;
+    case 5: return (const void*)&k_basket_corr<MK, SRC, 5>;
+    case 6: return (const void*)&k_basket_corr<MK, SRC, 6>;
+    case 7: return (const void*)&k_basket_corr<MK, SRC, 7>;
     case 8: return (const void*)&k_basket_corr<MK, SRC, 8>;
-    default: return (const void*)&k_basket_corr<MK, SRC, 16>;
+    case 9: return (const void*)&k_basket_corr<MK, SRC, 9>;
+    case 10: return (const void*)&k_basket_corr<MK, SRC, 10>;
+    case 11: return (const void*)&k_basket_corr<MK, SRC, 11>;
+    case 12: return (const void*)&k_basket_corr<MK, SRC, 12>;
+    case 13: return (const void*)&k_basket_corr<MK, SRC, 13>;
+    case 14: return (const void*)&k_basket_corr<MK, SRC, 14>;
+    case 15: return (const void*)&k_basket_corr<MK, SRC, 15>;
+    case 16: return (const void*)&k_basket_corr<MK, SRC, 16>;
+    default: return nullptr;
     }
 }
 
