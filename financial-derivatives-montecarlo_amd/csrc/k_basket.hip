@@ -229,20 +229,26 @@ __global__ __launch_bounds__(WIDE_WG) void k_basket_wide(BasketWide b) {
                 }
                 for (int k = 0; k < b.n_steps; k++) {
                     const int j0 = k * n + a0, off = j0 & 3;
-                    float g0[4], g1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    float g0[4];
                     prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j0 >> 2), g0);
-                    if (off) prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j0 >> 2) + 1u, g1);
+                    if (off == 0) {          // aligned (always so when n is a multiple of 4): no selection
 #pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const int u = off + t;
-                        float z = g0[0];
-                        z = (u == 1) ? g0[1] : z;
-                        z = (u == 2) ? g0[2] : z;
-                        z = (u == 3) ? g0[3] : z;
-                        z = (u == 4) ? g1[0] : z;
-                        z = (u == 5) ? g1[1] : z;
-                        z = (u == 6) ? g1[2] : z;
-                        S[t] = wide_step<MK>(b, S[t], sg[t], z * b.sqrt_h, k);
+                        for (int t = 0; t < 4; t++) S[t] = wide_step<MK>(b, S[t], sg[t], g0[t] * b.sqrt_h, k);
+                    } else {
+                        float g1[4];
+                        prng_normals4(b.seed + (uint32_t)r, b.tag, (uint64_t)idx, (uint32_t)(j0 >> 2) + 1u, g1);
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            const int u = off + t;
+                            float z = g0[0];
+                            z = (u == 1) ? g0[1] : z;
+                            z = (u == 2) ? g0[2] : z;
+                            z = (u == 3) ? g0[3] : z;
+                            z = (u == 4) ? g1[0] : z;
+                            z = (u == 5) ? g1[1] : z;
+                            z = (u == 6) ? g1[2] : z;
+                            S[t] = wide_step<MK>(b, S[t], sg[t], z * b.sqrt_h, k);
+                        }
                     }
                 }
 #pragma unroll
