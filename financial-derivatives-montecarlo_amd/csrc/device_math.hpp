@@ -189,10 +189,20 @@ __device__ __forceinline__ float step_1f(const LegParams& P, float S, float dw, 
 // Full-truncation exponential-Euler Heston step (methods_cuda.cu:278-284) with
 // e^{-kappa h} precomputed; MIL adds the Milstein term mil_q (dW2^2 - h),
 // mil_q = xi^2/4, of the optional scheme.
+// max(V, 0) as one v_max_f32, written as the instruction: fmaxf (or any
+// builtin the compiler maps to it) costs two, the compiler first quieting a
+// possible signalling NaN in V (v_max_f32 V, V).  The same value for every
+// finite V, and V is finite on every path (each step forms it from finite
+// terms).
+__device__ __forceinline__ float max0(float x) {
+    float r;
+    asm("v_max_f32 %0, 0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 template <bool MIL>
 __device__ __forceinline__ void step_heston(const LegParams& P, float& S, float& V, float dw1,
                                             float dw2, float h, float em) {
-    const float sq = fast_sqrt(fmaxf(V, 0.0f));
+    const float sq = fast_sqrt(max0(V));
     S = S + P.mu * S * h + sq * S * dw1;
     const float v = P.theta + em * (V - P.theta) + P.xi * sq * dw2;
     if constexpr (MIL) V = fmaf(P.mil_q, dw2 * dw2 - h, v);
