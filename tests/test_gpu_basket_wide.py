@@ -65,7 +65,8 @@ def test_wide_basket_prng_matches_oracle(engine, model, payoff, n_steps):
 # shapes of the MFMA correlation kernel's edges: asset counts that are not a
 # multiple of 4 (Philox groups of four normals straddle asset blocks and dates)
 # or of 16 (a partial last block), offsets inside a 64-path tile, and the
-# 16-block compilation above 128 assets
+# kernel's compilations for ceil(n / 16) blocks: 3 (37), 8 (113), 10 (150),
+# 13 (201) and the largest, 16 (256 assets, full blocks)
 GBM37 = fdmc.MultiGBMParams(37, [95.0 + a * 0.25 for a in range(37)], [0.12 + 0.005 * a for a in range(37)], 0.04, 1.0,
                             L=np.linalg.cholesky(_corr_100(37, 5)).ravel())
 PAY37 = fdmc.Basket(101.0, 0.04, 1.0, 37, weights=[1.0 / 37 + (a - 18) * 2e-4 for a in range(37)])
@@ -75,13 +76,21 @@ PAY150 = fdmc.Basket(100.0, 0.05, 1.0, 150)
 GBM113 = fdmc.MultiGBMParams(113, [100.0] * 113, [0.2 + 0.001 * a for a in range(113)], 0.05, 1.0,
                              L=np.linalg.cholesky(_corr_100(113, 9)).ravel())
 PAY113 = fdmc.Basket(100.0, 0.05, 1.0, 113)
+GBM201 = fdmc.MultiGBMParams(201, [100.0 + 0.02 * a for a in range(201)], [0.15 + 0.0005 * a for a in range(201)],
+                             0.05, 1.0, L=np.linalg.cholesky(_corr_100(201, 11)).ravel())
+PAY201 = fdmc.Basket(100.0, 0.05, 1.0, 201)
+GBM256 = fdmc.MultiGBMParams(256, [100.0] * 256, [0.18 + 0.0004 * a for a in range(256)], 0.03, 1.0,
+                             L=np.linalg.cholesky(_corr_100(256, 13)).ravel())
+PAY256 = fdmc.Basket(99.0, 0.03, 1.0, 256)
 
 
 @pytest.mark.parametrize("model,payoff,n_steps,first,npts", [
     (GBM37, PAY37, 7, 37, 300),
     (GBM113, PAY113, 9, 64, 128),
     (DUP150, PAY150, 5, 5, 200),
-], ids=["gbm37", "gbm113", "dupire150"])
+    (GBM201, PAY201, 3, 17, 150),
+    (GBM256, PAY256, 2, 0, 128),
+], ids=["gbm37", "gbm113", "dupire150", "gbm201", "gbm256"])
 def test_corr_basket_edge_shapes_qmc(engine, model, payoff, n_steps, first, npts):
     R = 2
     print(model.n, "assets,", n_steps, "dates, points", first, "+", npts, flush=True)
@@ -91,8 +100,9 @@ def test_corr_basket_edge_shapes_qmc(engine, model, payoff, n_steps, first, npts
     assert np.allclose(got[:, 1] / npts, ref[:, 1] / npts, rtol=5e-5, atol=1e-6)
 
 
-@pytest.mark.parametrize("model,payoff,n_steps", [(GBM37, PAY37, 6), (GBM113, PAY113, 3), (DUP150, PAY150, 4)],
-                         ids=["gbm37", "gbm113", "dupire150"])
+@pytest.mark.parametrize("model,payoff,n_steps", [(GBM37, PAY37, 6), (GBM113, PAY113, 3), (DUP150, PAY150, 4),
+                                                   (GBM201, PAY201, 3), (GBM256, PAY256, 2)],
+                         ids=["gbm37", "gbm113", "dupire150", "gbm201", "gbm256"])
 def test_corr_basket_edge_shapes_prng(engine, model, payoff, n_steps):
     print(model.n, "assets,", n_steps, "dates, Philox", flush=True)
     gm, gv = engine.run_mc_fixed(model, payoff, n_steps, 1000, 7)
