@@ -215,12 +215,23 @@ __device__ __forceinline__ float run_init(const LegParams& P) {
     if constexpr (PK == PK_LOOKBACK) return P.S0;
     else return 0.0f;
 }
-template <int PK>
+// HW: the running minimum / maximum as one v_min_f32 / v_max_f32 written as
+// the instruction, for loops that carry `run` from one iteration to the next
+// one step at a time (fminf / fmaxf then cost two: the compiler quiets a
+// possible signalling NaN in the loop-carried value first).  Unrolled blocks
+// of steps keep fminf / fmaxf, which the compiler chains into v_min3 / v_max3.
+// The two agree for every input the paths produce (quiet NaNs included).
+template <int PK, bool HW = false>
 __device__ __forceinline__ void run_update(float& run, float S) {
     if constexpr (PK == PK_ASIAN) run += S;
     else if constexpr (PK == PK_GEOM_ASIAN) run += fast_ln(S);
-    else if constexpr (PK == PK_LOOKBACK) run = fminf(run, S);
-    else if constexpr (PK == PK_BARRIER) run = fmaxf(run, S);
+    else if constexpr (PK == PK_LOOKBACK) {
+        if constexpr (HW) asm("v_min_f32 %0, %1, %2" : "=v"(run) : "v"(run), "v"(S));
+        else run = fminf(run, S);
+    } else if constexpr (PK == PK_BARRIER) {
+        if constexpr (HW) asm("v_max_f32 %0, %1, %2" : "=v"(run) : "v"(run), "v"(S));
+        else run = fmaxf(run, S);
+    }
 }
 template <int PK>
 __device__ __forceinline__ constexpr bool is_path_dep() {

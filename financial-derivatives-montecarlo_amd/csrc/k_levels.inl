@@ -48,10 +48,10 @@ struct Coupled {
         a1 += dw1;
         a2 += dw2;
         step_heston<MK == MK_HESTON_MIL>(P, Sf, Vf, dw1, dw2, E.h_f, E.em_f);
-        if constexpr (is_path_dep<PK>()) run_update<PK>(rf, Sf);
+        if constexpr (is_path_dep<PK>()) run_update<PK, true>(rf, Sf);
         if (++cnt == M) {
             step_heston<MK == MK_HESTON_MIL>(P, Sc, Vc, a1, a2, E.h_c, E.em_c);
-            if constexpr (is_path_dep<PK>()) run_update<PK>(rc, Sc);
+            if constexpr (is_path_dep<PK>()) run_update<PK, true>(rc, Sc);
             a1 = a2 = 0.0f;
             cnt = 0;
         }

@@ -197,7 +197,7 @@ struct Leg {
     }
     __device__ __forceinline__ void stepH(const LegParams& P, float dw1, float dw2, float h, float em) {
         step_heston<MK == MK_HESTON_MIL>(P, S, V, dw1, dw2, h, em);
-        if constexpr (is_path_dep<PK>()) run_update<PK>(run, S);
+        if constexpr (is_path_dep<PK>()) run_update<PK, true>(run, S);
     }
     __device__ __forceinline__ float pay(const LegParams& P, int n, float bgk) const {
         return payoff<PK>(P, S, run, n, bgk);
