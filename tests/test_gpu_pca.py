@@ -327,15 +327,16 @@ def test_pca_normals_are_the_oracle_points(engine, torch_cuda, bf16, off):
     """The GEMM's A operand itself: with M = I the transform returns the
     half-precision normals unchanged (products with 1, sums with 0), so they
     are compared with the oracle's scrambled-Sobol normals rounded the same
-    way, row for row.  786,688 points are 3,073 tiles, more than the 3,072
-    workgroups k_gen_z launches, so workgroups run two tiles and the rows
-    checked include second tiles (Sobol words advanced in registers from the
+    way, row for row.  786,688 points are 3,073 tiles, more than k_gen_z
+    launches workgroups (three per CU), so workgroups run several tiles and
+    the rows checked include later tiles (Sobol words advanced in registers from the
     first tile by the XOR of two direction numbers per dimension, v_7 and
     v_{8 + ctz(n)} of tile n); D = 160 has a partial 64-dimension chunk and a
-    padding chunk (mp = 256).  The two high offsets start at an odd tile, so
-    that tile 2^16 (points 2^24..) and tile 2^23 (points 2^31..) are second
-    tiles (ctz 16 and 23, the update rows that are not precomputed); rows
-    2779.. are those tiles."""
+    padding chunk (mp = 256).  At the two high offsets tile 2^16 (points
+    2^24..) and tile 2^23 (points 2^31..) are the 12th tile of the range, not
+    the first of a workgroup for either workgroup size (two or five tiles), so
+    their words come from the tile-to-tile update with ctz 16 and 23 (the
+    update rows that are not precomputed); rows 2779.. are those tiles."""
     torch = torch_cuda
     D, npts, salt = 160, 3073 * 256, 77
     pca = engine.pca_upload(D, np.eye(D).ravel())
