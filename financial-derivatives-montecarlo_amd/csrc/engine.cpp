@@ -503,7 +503,19 @@ int run_basket_wide(fdmc_context* c, const PassSpec& s, std::vector<double>& out
     const int wg = basket_wide_block(corr), tile = basket_wide_tile(corr);
     const int64_t t_first = s.first / tile, t_last = (s.first + s.count - 1) / tile;
     const int64_t ntiles = t_last - t_first + 1;
-    const int64_t per_rep = std::max<int64_t>(1, (int64_t)c->n_cu * (corr ? 4 : 16) / s.R);
+    // as many workgroups as are resident at once: the correlated kernel's
+    // residency depends on its block count's registers and LDS (three per CU
+    // for 100 assets), and all workgroups carry equal work, so a second,
+    // partly filled round would leave most of the chip idle
+    const size_t lds = basket_wide_lds_bytes(n, corr);
+    if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "wide basket LDS footprint too large");
+    if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int resident = corr ? 4 : 16;
+    if (corr) {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, fn, wg, lds) == hipSuccess && o > 0) resident = o;
+    }
+    const int64_t per_rep = std::max<int64_t>(1, (int64_t)c->n_cu * resident / s.R);
     const int64_t tpw = std::max<int64_t>(1, (ntiles + per_rep - 1) / per_rep);
     const int64_t nwg = (ntiles + tpw - 1) / tpw;
     if (nwg > 0x7FFFFFFF || tpw > 0x7FFFFFFF) fail(FDMC_E_RANGE, "too many workgroups");
@@ -513,9 +525,6 @@ int run_basket_wide(fdmc_context* c, const PassSpec& s, std::vector<double>& out
     b.nwg_x = (int)nwg;
     b.partials = (double*)c->partials.ensure((size_t)s.R * nwg * 2 * sizeof(double));
     double* d_sums = (double*)c->sums.ensure((size_t)s.R * 2 * sizeof(double));
-    const size_t lds = basket_wide_lds_bytes(n, corr);
-    if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "wide basket LDS footprint too large");
-    if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     void* args[] = {(void*)&b};
     HIPC(hipLaunchKernel(fn, dim3((unsigned)nwg, (unsigned)s.R), dim3(wg), args, lds, c->stream));
     HIPC(launch_reduce_partials(b.partials, s.R, (int)nwg, 2, d_sums, c->stream));
