@@ -119,7 +119,7 @@ typedef enum { FDMC_NOISE_RAW = 0, FDMC_NOISE_BB = 1, FDMC_NOISE_PCA = 2,
 #define FDMC_PCA_BF16 0x100
 
 #define FDMC_FLAG_PRNG_FALLBACK 1u   /* QMC dimension > 20000: pseudo-random points were used */
-#define FDMC_FLAG_NOT_CONVERGED 2u   /* stopping rule not met within the configured limits */
+#define FDMC_FLAG_NOT_CONVERGED 2u   /* stopping rule not met within the configured limits (book: or an estimate that is not finite) */
 
 typedef struct fdmc_result {     /* MCResult (methods_cuda.cuh:41-46) + additive fields */
     double price;
