@@ -187,14 +187,14 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
             // ---- forward
             float S = P.S0, run = run_init<PK>(P), crun = run_init<CPK>(a.leg[1]);
             int kmin = 0;
-            for (int i = 0; i < G; i++) {
-                float dw[B];
-                gen_block(i, dw);
-                ck[i * 256] = S;
+            // a block that ends by date n takes a copy of the step loop without
+            // the per-step test (as selects it cost three per step); the last,
+            // partial block of an n that is no multiple of B keeps it
+            auto fwd_block = [&](int i, const float (&dw)[B], auto full) {
     #pragma unroll
                 for (int k = 0; k < B; k++) {
                     const int g = i * B + k;
-                    if (g < n) {
+                    if (decltype(full)::value || g < n) {
                         S = step_1f<MK>(PS, S, dw[k], (float)g * h, h);
                         if constexpr (PK == PK_LOOKBACK) {
                             if (S < run) { run = S; kmin = g + 1; }
@@ -204,6 +204,13 @@ __global__ __launch_bounds__(256) void k_greeks(PathLaunch a) {
                         if constexpr (CV == CV_SHARED) run_update<CPK>(crun, S);
                     }
                 }
+            };
+            for (int i = 0; i < G; i++) {
+                float dw[B];
+                gen_block(i, dw);
+                ck[i * 256] = S;
+                if ((i + 1) * B <= n) fwd_block(i, dw, std::true_type{});
+                else fwd_block(i, dw, std::false_type{});
             }
             const float ym = payoff<PK>(P, S, run, n, a.bgk[0]);
             float yc = 0.0f;
