@@ -330,9 +330,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             } else {
                 for (int b = 0; 4 * b < nn; b++) {
                     prng_normals4(a.seed + (uint32_t)r, a.tag, (uint64_t)idx, (uint32_t)b, z);
+                    if (4 * b + 4 <= a.n_steps) {      // a full group: no per-step date test (a select per step)
 #pragma unroll
-                    for (int c = 0; c < 4; c++)
-                        if (4 * b + c < a.n_steps) ps.step(a, z[c] * a.sqrt_h, 4 * b + c);
+                        for (int c = 0; c < 4; c++) ps.step(a, z[c] * a.sqrt_h, 4 * b + c);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+                            if (4 * b + c < a.n_steps) ps.step(a, z[c] * a.sqrt_h, 4 * b + c);
+                    }
                 }
             }
         } else if constexpr (NZ == NZ_SOBOL_RAW) {
