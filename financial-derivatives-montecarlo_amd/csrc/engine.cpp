@@ -503,15 +503,15 @@ int run_basket_wide(fdmc_context* c, const PassSpec& s, std::vector<double>& out
     const int wg = basket_wide_block(corr), tile = basket_wide_tile(corr);
     const int64_t t_first = s.first / tile, t_last = (s.first + s.count - 1) / tile;
     const int64_t ntiles = t_last - t_first + 1;
-    // as many workgroups as are resident at once: the correlated kernel's
-    // residency depends on its block count's registers and LDS (three per CU
-    // for 100 assets), and all workgroups carry equal work, so a second,
-    // partly filled round would leave most of the chip idle
+    // as many workgroups as are resident at once: residency depends on the
+    // kernel's registers and LDS (the correlated kernel's on its block count:
+    // three per CU for 100 assets), and all workgroups carry equal work, so a
+    // second, partly filled round would leave most of the chip idle
     const size_t lds = basket_wide_lds_bytes(n, corr);
     if (lds > LDS_DYNAMIC_MAX) fail(FDMC_E_UNSUPPORTED, "wide basket LDS footprint too large");
     if (lds > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int resident = corr ? 4 : 16;
-    if (corr) {
+    {
         int o = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, fn, wg, lds) == hipSuccess && o > 0) resident = o;
     }
